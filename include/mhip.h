@@ -1,0 +1,216 @@
+/*
+ * mhip.h - C ABI of libmhip.so, the gfx950 (MI355X) boundary-element hot path
+ * of MarineHydro.jl: Green-function panel integrals, fused S/D/K influence
+ * matrix assembly, batched dense complex LU solve, boundary conditions and
+ * forces.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer unless the function name ends in
+ *    `_host`.  Short parameter lists (wavenumbers, frequencies, headings) are
+ *    host arrays, as noted.
+ *  - Matrices are column-major complex128 (interleaved re, im), exactly the
+ *    memory layout of a Julia Matrix{ComplexF64}; (i, j) of batch item b sits
+ *    at complex offset b*stride + i + j*ld.  Strides and leading dimensions
+ *    are counted in complex elements.
+ *  - Mesh arrays use the reference's Julia layout (src/meshes.jl:12-37):
+ *    vertices nv x 3, faces nf x 4 (0-based, src/meshes.jl:179), centers and
+ *    normals nf x 3, all column-major; areas and radii nf.
+ *  - Functions return MH_OK (0) or an error code; mh_last_error() returns a
+ *    message for the calling thread.  No exception crosses the ABI.
+ *  - All work of one context is ordered on that context's HIP stream; the
+ *    call returns after enqueueing unless the function says it synchronises.
+ *    Contexts are not thread-safe; use one per host thread.
+ *
+ * Reference paths below are relative to the MarineHydro.jl repository root.
+ */
+#ifndef MHIP_H
+#define MHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHIP_VERSION 100
+
+enum mh_status {
+    MH_OK = 0,
+    MH_ERR_ARGUMENT = 1,        /* bad size, pointer or option            */
+    MH_ERR_HIP = 2,             /* HIP runtime error                      */
+    MH_ERR_SINGULAR = 3,        /* exact zero pivot in the LU             */
+    MH_ERR_NOT_SUPPORTED = 4,   /* option not available in this build     */
+    MH_ERR_NO_DEVICE = 5,       /* no usable gfx950 device                */
+    MH_ERR_NOT_CONVERGED = 6    /* iterative refinement did not converge  */
+};
+
+/* Green-function terms, summed like the reference's tuple of GreensFunction
+ * objects (src/green_functions/abstract_greens_function.jl:18-83).       */
+enum mh_gf {
+    MH_GF_RANKINE = 1,               /* Rankine(), rankine.jl               */
+    MH_GF_RANKINE_REFLECTED = 2,     /* RankineReflected()                  */
+    MH_GF_RANKINE_REFLECTED_NEG = 4, /* RankineReflectedNegative()          */
+    MH_GF_WU = 8,                    /* GFWu(), wu.jl                       */
+    MH_GF_DELHOMMEAU = 16            /* ExactGuevelDelhommeau()             */
+};
+/* The tuple used by calculate_radiation_forces / DiffractionForce
+ * (src/matrix_assembly.jl:100-102, src/waves.jl:41-45). */
+#define MH_GF_DEFAULT (MH_GF_RANKINE | MH_GF_RANKINE_REFLECTED | MH_GF_WU)
+
+/* Formulation: the reference's `direct` keyword (src/matrix_assembly.jl:4). */
+enum mh_formulation { MH_DIRECT = 1, MH_INDIRECT = 2, MH_BOTH = 3 };
+
+/* Per-pair operation for mh_gf_eval, one per reference generic function
+ * (src/green_functions/abstract_greens_function.jl:16-83). */
+enum mh_gf_op {
+    MH_OP_GREENS = 0,             /* greens(gf, e1, e2, k)                 */
+    MH_OP_GRADIENT_GREENS = 1,    /* gradient_greens(...; wrt_first)       */
+    MH_OP_INTEGRAL = 2,           /* integral(gf, e1, e2, k)               */
+    MH_OP_INTEGRAL_GRADIENT = 3   /* integral_gradient(...; wrt_first)     */
+};
+
+/* Arithmetic of the dense solve. */
+enum mh_precision {
+    MH_FP64 = 0,    /* FP64 throughout                                       */
+    MH_MIXED = 1    /* FP32 factorisation + FP64 iterative refinement         */
+};
+
+typedef struct mh_context* mh_context_t;
+
+/* One source panel, the fields the reference reads through its element
+ * accessors (src/meshes.jl:157-205): center, vertices (row v = vertex v),
+ * normal, area, radius.  160 bytes. */
+typedef struct {
+    double center[3];
+    double vertices[4][3];
+    double normal[3];
+    double area;
+    double radius;
+} mh_panel_t;
+
+/* Device-resident mesh, field for field the reference's Mesh
+ * (src/meshes.jl:12-37). */
+typedef struct {
+    int64_t nfaces;
+    int64_t nvertices;
+    const double* vertices; /* nvertices x 3, column-major           */
+    const int64_t* faces;   /* nfaces x 4, column-major, 0-based     */
+    const double* centers;  /* nfaces x 3, column-major              */
+    const double* normals;  /* nfaces x 3, column-major              */
+    const double* areas;    /* nfaces                                */
+    const double* radii;    /* nfaces                                */
+} mh_mesh_t;
+
+/* ---- context ---------------------------------------------------------- */
+int mh_version(void);
+const char* mh_last_error(void);
+/* stream: a hipStream_t, or NULL for the null stream.  No reference
+ * counterpart (the reference runs on the host). */
+int mh_context_create(int device, void* stream, mh_context_t* out);
+int mh_context_destroy(mh_context_t ctx);
+int mh_synchronize(mh_context_t ctx);
+
+/* Kernel timer (no reference counterpart; the reference is profiled with
+ * BenchmarkTools, benchmark/*.jl).  When enabled, every kernel the context
+ * launches is bracketed by HIP events on the context's stream.
+ * mh_profile_report synchronises, writes a JSON object
+ * {"kernel": {"launches": n, "ms": t, "flops": f, "bytes": b}, ...} with the
+ * algorithmic flops and bytes of those launches into buf (NUL-terminated,
+ * at most cap bytes) and clears the records. */
+int mh_profile_enable(mh_context_t ctx, int on);
+int mh_profile_report(mh_context_t ctx, char* buf, int64_t cap);
+
+/* ---- Green functions ---------------------------------------------------
+ * Batched per-pair evaluation of the sum of the selected terms.
+ * Replaces greens / gradient_greens / integral / integral_gradient
+ * (src/green_functions/abstract_greens_function.jl:16-83; terms in
+ * rankine.jl:8-128, rankine_reflected.jl:1-44,
+ * rankine_reflected_negative.jl:1-17, wu.jl:5-47,
+ * exact_Guevel_Delhommeau.jl:28-75).
+ *   points : npairs x 3, column-major (centre of element_1)
+ *   panels : npairs records (element_2)
+ *   out    : complex128, npairs values (ops 0, 2) or npairs x 3 column-major
+ *            complex 3-vectors (ops 1, 3). */
+int mh_gf_eval(mh_context_t ctx, int gf_mask, int op, int wrt_first,
+               int64_t npairs, const double* points, const mh_panel_t* panels,
+               double wavenumber, double* out);
+
+/* Dimensionless wave terms G(r, z), dG/dr, dG/dz for npoints (r, z) pairs:
+ * term = MH_GF_WU (wu.jl:5-30) or MH_GF_DELHOMMEAU
+ * (exact_Guevel_Delhommeau.jl:15-59).  out: npoints x 3 complex,
+ * column-major. */
+int mh_wave_term_eval(mh_context_t ctx, int term, int64_t npoints,
+                      const double* r, const double* z, double* out);
+
+/* ---- assembly ----------------------------------------------------------
+ * assemble_matrices(green_functions, mesh, k; direct) for nk wavenumbers in
+ * one launch (src/matrix_assembly.jl:4-42, :97; the Wu tuple of :100-102).
+ * S_ij = -1/(4 pi) sum_gf integral; D_ij = delta_ij/2 - 1/(4 pi) n_j . grad_xi;
+ * K_ij = delta_ij/2 - 1/(4 pi) n_i . grad_x.  MH_BOTH writes D and K.
+ * wavenumbers: HOST array of nk values.  S, D, K: nk matrices each (D or K
+ * may be NULL when not requested). */
+int mh_assemble(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
+                int formulation, int64_t nk, const double* wavenumbers,
+                double* S, double* D, double* K, int64_t ld, int64_t stride);
+
+/* ---- dense solve -------------------------------------------------------
+ * Batched LU with partial pivoting (zgetrf semantics; pivot = max |re|+|im|)
+ * in place; ipiv: n x batch, 0-based (pivot row of step k of item b at
+ * ipiv[b*n + k]).  Returns MH_ERR_SINGULAR if any exact zero pivot occurs
+ * (synchronises to read the flag).  Replaces the LAPACK factorisation behind
+ * `\` / ImplicitAD.implicit_linear (src/matrix_assembly.jl:108-122). */
+int mh_lu_factor(mh_context_t ctx, int64_t n, int64_t batch, double* A,
+                 int64_t ld, int64_t stride, int64_t* ipiv);
+/* Solve with the factors: trans=0 A X = B, trans=1 A^T X = B (the adjoint
+ * of the complex-linear solve).  B: n x nrhs per item, in place. */
+int mh_lu_solve(mh_context_t ctx, int64_t n, int64_t batch, const double* LU,
+                int64_t ld, int64_t stride, const int64_t* ipiv, int trans,
+                int64_t nrhs, double* B, int64_t ldb, int64_t bstride);
+/* solve(D, S, bc; direct) (src/matrix_assembly.jl:105-133) for a batch:
+ * direct: phi = D \ (S bc); indirect: phi = S (K \ bc).  DK is overwritten
+ * by its LU factors.  bc, phi: n x nrhs per item. */
+int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct,
+             double* DK, const double* S, int64_t ld, int64_t stride,
+             const double* bc, int64_t nrhs, double* phi, int64_t ldb,
+             int64_t bstride);
+
+/* ---- one frequency sweep, end to end -----------------------------------
+ * For every omega: k = omega^2/g; assemble; radiation BCs for each dof
+ * field (radiation_bc, src/waves.jl:89-102) and Airy diffraction BCs for
+ * each heading (AiryBC, src/waves.jl:121-142); one LU and all right-hand
+ * sides; pressure and integrate_pressure (src/waves.jl:21-37, :55-59,
+ * :72-85).  Outputs (device):
+ *   A, B : nomega x ndof x ndof, A[w][j][i] = added mass on influenced dof
+ *          i due to radiating dof j, = Re F / omega^2 (waves.jl:36);
+ *          B = Im F / omega.
+ *   F    : nomega x nbeta x ndof complex128, diffraction force on dof i for
+ *          heading b (DiffractionForce, src/waves.jl:40-52).
+ * dof_fields: device, ndof fields of nfaces x 3 (column-major) each.
+ * omegas, betas: host arrays.  precision: MH_FP64 or MH_MIXED. */
+int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
+                  int direct, int precision, int64_t nomega,
+                  const double* omegas, double g, double rho, int64_t ndof,
+                  const double* dof_fields, int64_t nbeta, const double* betas,
+                  double* A, double* B, double* F);
+
+/* ---- host-array entry points (one call, synchronous) -------------------
+ * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
+ * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.
+ * Mesh arrays are host arrays in the reference's layout.  out[0] = A,
+ * out[1] = B. */
+int mh_calculate_radiation_forces_host(
+    int64_t nfaces, int64_t nvertices, const double* vertices,
+    const int64_t* faces, const double* centers, const double* normals,
+    const double* areas, const double* radii, const double* dof3,
+    double omega, double g, double rho, double* out);
+/* DiffractionForce(mesh, omega, dof) -> ComplexF64 (src/waves.jl:40-52),
+ * heading 0.  out[0] = Re F, out[1] = Im F. */
+int mh_diffraction_force_host(
+    int64_t nfaces, int64_t nvertices, const double* vertices,
+    const int64_t* faces, const double* centers, const double* normals,
+    const double* areas, const double* radii, const double* dof3,
+    double omega, double g, double rho, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHIP_H */

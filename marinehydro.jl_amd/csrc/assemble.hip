@@ -1,0 +1,353 @@
+// Fused influence-matrix assembly for gfx950.
+//
+// One pass over every panel pair (i = collocation row, j = source column)
+// evaluates the Rankine and image-Rankine panel integrals and their gradients
+// once, then the wave term for a group of wavenumbers, and writes S together
+// with D (direct), K (indirect) or both.  Reference semantics:
+// src/matrix_assembly.jl:4-42 (comprehension assembly; S at :14, D/K at
+// :16-26), summing the terms of src/green_functions/*.jl.
+//
+// Launch shape: a 256-thread workgroup covers a 64 x 64 tile of (i, j);
+// each wave owns 16 source columns and its 64 lanes own 64 consecutive rows.
+// The source panel is wave-uniform (scalar loads of a packed 192-byte record)
+// and the 64 lanes of a wave write one contiguous 1 KiB run of each
+// column-major complex128 matrix.  Rankine terms do not depend on the
+// wavenumber, so they are computed once per pair and reused for every
+// wavenumber of the group (blockIdx.z).
+#include "mh_internal.hpp"
+
+namespace mh {
+
+__global__ void k_pack_panels(int64_t nf, int64_t nv, const double* __restrict__ V,
+                              const int64_t* __restrict__ F, const double* __restrict__ C,
+                              const double* __restrict__ N, const double* __restrict__ A,
+                              const double* __restrict__ R, PanelRec* __restrict__ out,
+                              int* __restrict__ flag) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nf) return;
+    PanelRec p;
+    for (int c = 0; c < 3; ++c) {
+        p.c[c] = C[j + c * nf];
+        p.n[c] = N[j + c * nf];
+    }
+    p.area = A[j];
+    p.radius = R[j];
+    for (int v = 0; v < 4; ++v) {
+        int64_t iv = F[j + v * nf];  // 0-based, reference src/meshes.jl:179
+        if (iv < 0 || iv >= nv) {
+            atomicOr(flag, 1);
+            iv = 0;
+        }
+        for (int c = 0; c < 3; ++c) p.v[v][c] = V[iv + c * nv];
+    }
+    for (int c = 0; c < 4; ++c) p.pad[c] = 0.0;
+    out[j] = p;
+}
+
+// Rankine + image Rankine: value and gradients w.r.t. the source point (gxi)
+// and the collocation point (gx) for one pair.
+__device__ __forceinline__ void rankine_terms(int gf, double x0, double x1, double x2,
+                                              const PanelRec& p, double& I, double gxi[3],
+                                              double gx[3]) {
+    I = 0.0;
+    gxi[0] = gxi[1] = gxi[2] = 0.0;
+    gx[0] = gx[1] = gx[2] = 0.0;
+    double v, g[3];
+    if (gf & GF_RANKINE) {  // rankine.jl:24-128
+        rankine_panel(x0, x1, x2, p, v, g);
+        I += v;
+        gxi[0] += g[0];
+        gxi[1] += g[1];
+        gxi[2] += g[2];
+        gx[0] -= g[0];
+        gx[1] -= g[1];
+        gx[2] -= g[2];
+    }
+    if (gf & (GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) {
+        // collocation point reflected across z = 0 (rankine_reflected.jl:19-35)
+        rankine_panel(x0, x1, -x2, p, v, g);
+        const double s = ((gf & GF_RANKINE_REFLECTED) ? 1.0 : 0.0) -
+                         ((gf & GF_RANKINE_REFLECTED_NEG) ? 1.0 : 0.0);
+        I += s * v;
+        gxi[0] += s * g[0];
+        gxi[1] += s * g[1];
+        gxi[2] += s * g[2];
+        // w.r.t. x: vertical reflection of minus the source gradient
+        gx[0] -= s * g[0];
+        gx[1] -= s * g[1];
+        gx[2] += s * g[2];
+    }
+}
+
+// Wave term for one pair at wavenumber k, already multiplied by the source
+// area (one-point rule, wu.jl:33-47): value, gradient w.r.t. the source
+// point (z component shared with the collocation gradient) and the
+// horizontal factors.  Reduced coordinates per
+// abstract_greens_function.jl:88-155.
+template <int WAVE>
+__device__ __forceinline__ void wave_term(double k, double dx, double dy, double zsum,
+                                          double area, cplx& Sv, cplx gxi[3], cplx gx[3]) {
+    const double r = k * horiz_dist(dx, dy);
+    const double z = k * zsum;
+    cplx G, dGdr, dGdz;
+    if (WAVE == 1) wu_dimless<true>(r, z, G, dGdr, dGdz);
+    double d1 = 0.0, d2 = 0.0;  // dr/dxi_1, dr/dxi_2
+    if (fabs(r) > 1e-6) {
+        const double kk = k * k / r;
+        d1 = kk * (-dx);
+        d2 = kk * (-dy);
+    }
+    const double ka = k * area;
+    Sv.re = ka * G.re;
+    Sv.im = ka * G.im;
+    gxi[0].re = ka * (d1 * dGdr.re);
+    gxi[0].im = ka * (d1 * dGdr.im);
+    gxi[1].re = ka * (d2 * dGdr.re);
+    gxi[1].im = ka * (d2 * dGdr.im);
+    gxi[2].re = ka * (k * dGdz.re);
+    gxi[2].im = ka * (k * dGdz.im);
+    gx[0].re = -gxi[0].re;
+    gx[0].im = -gxi[0].im;
+    gx[1].re = -gxi[1].re;
+    gx[1].im = -gxi[1].im;
+    gx[2] = gxi[2];
+}
+
+constexpr int kFG = 4;  // wavenumbers per thread (blockIdx.z group)
+
+template <int WAVE, int FORM>
+__global__ __launch_bounds__(256) void k_assemble(
+    int64_t n, const double* __restrict__ C, const double* __restrict__ N,
+    const PanelRec* __restrict__ panels, int gf, int64_t nk, const double* __restrict__ ks,
+    double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K, int64_t ld,
+    int64_t stride) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double x0 = C[ic], x1 = C[ic + n], x2 = C[ic + 2 * n];
+    double ni0 = 0.0, ni1 = 0.0, ni2 = 0.0;
+    if (FORM & 2) {
+        ni0 = N[ic];
+        ni1 = N[ic + n];
+        ni2 = N[ic + 2 * n];
+    }
+    const int64_t jb = (int64_t)blockIdx.y * 64;
+    const int64_t f0 = (int64_t)blockIdx.z * kFG;
+    const int64_t nfg = (nk - f0) < kFG ? (nk - f0) : kFG;
+    for (int jj = wave; jj < 64; jj += 4) {
+        const int64_t j = jb + jj;
+        if (j >= n) break;
+        const PanelRec& p = panels[j];
+        double IR, gxiR[3], gxR[3];
+        rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        const double nj0 = p.n[0], nj1 = p.n[1], nj2 = p.n[2];
+        const double diag = (i == j) ? 0.5 : 0.0;
+        for (int64_t f = 0; f < nfg; ++f) {
+            const int64_t m = f0 + f;
+            cplx Sv = {0.0, 0.0}, gxi[3] = {{0, 0}, {0, 0}, {0, 0}}, gx[3] = {{0, 0}, {0, 0}, {0, 0}};
+            if (WAVE) wave_term<WAVE>(ks[m], x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], p.area, Sv, gxi, gx);
+            if (!row_ok) continue;
+            const int64_t off = m * stride + i + j * ld;
+            S[off] = make_double2(-kInv4Pi * (IR + Sv.re), -kInv4Pi * Sv.im);
+            if (FORM & 1) {
+                const double re = nj0 * (gxiR[0] + gxi[0].re) + nj1 * (gxiR[1] + gxi[1].re) +
+                                  nj2 * (gxiR[2] + gxi[2].re);
+                const double im = nj0 * gxi[0].im + nj1 * gxi[1].im + nj2 * gxi[2].im;
+                D[off] = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+            }
+            if (FORM & 2) {
+                const double re = ni0 * (gxR[0] + gx[0].re) + ni1 * (gxR[1] + gx[1].re) +
+                                  ni2 * (gxR[2] + gx[2].re);
+                const double im = ni0 * gx[0].im + ni1 * gx[1].im + ni2 * gx[2].im;
+                K[off] = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+            }
+        }
+    }
+}
+
+int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out) {
+    const int64_t nf = m->nfaces;
+    if (nf == 0) return MH_OK;
+    MH_HIP_CHECK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_pack_panels, dim3((unsigned)((nf + 127) / 128)), dim3(128), 0, ctx->stream,
+                       nf, m->nvertices, m->vertices, m->faces, m->centers, m->normals, m->areas,
+                       m->radii, out, ctx->d_flag);
+    MH_HIP_CHECK(hipGetLastError());
+    int flag = 0;
+    MH_HIP_CHECK(hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (flag) {
+        set_error("mesh: a face references a vertex index outside [0, nvertices)");
+        return MH_ERR_ARGUMENT;
+    }
+    return MH_OK;
+}
+
+template <int WAVE>
+static void dispatch_form(int form, dim3 grid, hipStream_t st, int64_t n, const double* C,
+                          const double* N, const PanelRec* P, int gf, int64_t nk,
+                          const double* ks, double2* S, double2* D, double2* K, int64_t ld,
+                          int64_t stride) {
+    switch (form) {
+        case 1:
+            hipLaunchKernelGGL((k_assemble<WAVE, 1>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
+                               S, D, K, ld, stride);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_assemble<WAVE, 2>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
+                               S, D, K, ld, stride);
+            break;
+        default:
+            hipLaunchKernelGGL((k_assemble<WAVE, 3>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
+                               S, D, K, ld, stride);
+            break;
+    }
+}
+
+int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, int gf, int form,
+                    int64_t nk, const double* d_k, double2* S, double2* D, double2* K, int64_t ld,
+                    int64_t stride) {
+    const int64_t n = m->nfaces;
+    if (n == 0 || nk == 0) return MH_OK;
+    if (gf & GF_DELHOMMEAU) {
+        set_error("assemble: the Delhommeau wave term is not available in this build");
+        return MH_ERR_NOT_SUPPORTED;
+    }
+    const int64_t tiles = (n + 63) / 64;
+    const int64_t groups = (nk + kFG - 1) / kFG;
+    if (tiles > 0x7fffffff || groups > 65535) {
+        set_error("assemble: problem too large for one launch");
+        return MH_ERR_ARGUMENT;
+    }
+    dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)groups);
+    const double nmat = 1.0 + ((form & 1) ? 1.0 : 0.0) + ((form & 2) ? 1.0 : 0.0);
+    ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
+    if (gf & GF_WU)
+        dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
+                         D, K, ld, stride);
+    else
+        dispatch_form<0>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
+                         D, K, ld, stride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair evaluation (greens / gradient_greens / integral / integral_gradient)
+// ---------------------------------------------------------------------------
+__global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
+                          const double* __restrict__ pts, const mh_panel_t* __restrict__ pan,
+                          double k, double2* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const double x0 = pts[t], x1 = pts[t + np], x2 = pts[t + 2 * np];
+    const mh_panel_t& q = pan[t];
+    PanelRec p;
+    for (int c = 0; c < 3; ++c) {
+        p.c[c] = q.center[c];
+        p.n[c] = q.normal[c];
+        for (int v = 0; v < 4; ++v) p.v[v][c] = q.vertices[v][c];
+    }
+    p.area = q.area;
+    p.radius = q.radius;
+    double val_re = 0.0, val_im = 0.0;
+    double g_re[3] = {0, 0, 0}, g_im[3] = {0, 0, 0};
+    const bool grad = (op == MH_OP_GRADIENT_GREENS || op == MH_OP_INTEGRAL_GRADIENT);
+    const bool panel_int = (op == MH_OP_INTEGRAL || op == MH_OP_INTEGRAL_GRADIENT);
+    // Rankine terms
+    const double sref = ((gf & GF_RANKINE_REFLECTED) ? 1.0 : 0.0) -
+                        ((gf & GF_RANKINE_REFLECTED_NEG) ? 1.0 : 0.0);
+    if (panel_int) {
+        double I, gxi[3], gx[3];
+        rankine_terms(gf, x0, x1, x2, p, I, gxi, gx);
+        val_re += I;
+        for (int c = 0; c < 3; ++c) g_re[c] += wrt_first ? gx[c] : gxi[c];
+    } else {
+        // point values: rankine.jl:8-22, rankine_reflected.jl:1-14
+        if (gf & GF_RANKINE) {
+            const double d0 = x0 - p.c[0], d1 = x1 - p.c[1], d2 = x2 - p.c[2];
+            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            val_re += 1.0 / r;
+            const double r3 = r * r * r;
+            const double s = wrt_first ? 1.0 : -1.0;  // grad_x = (xi - x)/r^3
+            g_re[0] += s * (-d0) / r3;
+            g_re[1] += s * (-d1) / r3;
+            g_re[2] += s * (-d2) / r3;
+        }
+        if (sref != 0.0) {
+            // value with the collocation point reflected
+            const double d0 = x0 - p.c[0], d1 = x1 - p.c[1], d2 = -x2 - p.c[2];
+            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            val_re += sref / r;
+            const double r3 = r * r * r;
+            if (wrt_first) {
+                // gradient_greens(Rankine, e1, reflect(e2); true) = (reflect(xi) - x)/r^3
+                g_re[0] += sref * (p.c[0] - x0) / r3;
+                g_re[1] += sref * (p.c[1] - x1) / r3;
+                g_re[2] += sref * (-p.c[2] - x2) / r3;
+            } else {
+                // gradient_greens(Rankine, reflect(e1), e2; false) = (reflect(x) - xi)/r^3
+                g_re[0] += sref * (x0 - p.c[0]) / r3;
+                g_re[1] += sref * (x1 - p.c[1]) / r3;
+                g_re[2] += sref * (-x2 - p.c[2]) / r3;
+            }
+        }
+    }
+    if (gf & GF_WU) {
+        cplx Sv, gxi[3], gx[3];
+        const double area = panel_int ? p.area : 1.0;
+        wave_term<1>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
+        val_re += Sv.re;
+        val_im += Sv.im;
+        for (int c = 0; c < 3; ++c) {
+            g_re[c] += wrt_first ? gx[c].re : gxi[c].re;
+            g_im[c] += wrt_first ? gx[c].im : gxi[c].im;
+        }
+    }
+    if (grad) {
+        for (int c = 0; c < 3; ++c) out[t + c * np] = make_double2(g_re[c], g_im[c]);
+    } else {
+        out[t] = make_double2(val_re, val_im);
+    }
+}
+
+int launch_gf_eval(Context* ctx, int gf, int op, int wrt_first, int64_t np, const double* pts,
+                   const mh_panel_t* pan, double k, double2* out) {
+    if (np == 0) return MH_OK;
+    if (gf & GF_DELHOMMEAU) {
+        set_error("gf_eval: the Delhommeau wave term is not available in this build");
+        return MH_ERR_NOT_SUPPORTED;
+    }
+    hipLaunchKernelGGL(k_gf_eval, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, ctx->stream, gf,
+                       op, wrt_first, np, pts, pan, k, out);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+__global__ void k_wave_term(int term, int64_t n, const double* __restrict__ r,
+                            const double* __restrict__ z, double2* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cplx G, dr, dz;
+    wu_dimless<true>(r[t], z[t], G, dr, dz);
+    out[t] = make_double2(G.re, G.im);
+    out[t + n] = make_double2(dr.re, dr.im);
+    out[t + 2 * n] = make_double2(dz.re, dz.im);
+}
+
+int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const double* z,
+                     double2* out) {
+    if (term != GF_WU) {
+        set_error("wave_term: only MH_GF_WU is available in this build");
+        return MH_ERR_NOT_SUPPORTED;
+    }
+    if (n == 0) return MH_OK;
+    hipLaunchKernelGGL(k_wave_term, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream,
+                       term, n, r, z, out);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+}  // namespace mh

@@ -1,0 +1,494 @@
+// extern "C" entry points of libmhip.so (declared in include/mhip.h).
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "mh_internal.hpp"
+
+namespace mh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+bool ensure_workspace(Context* ctx, size_t bytes) {
+    if (bytes <= ctx->ws_bytes) return true;
+    if (ctx->ws) {
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
+        (void)hipFree(ctx->ws);
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->ws, bytes) != hipSuccess) {
+        set_error("workspace: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        ctx->ws = nullptr;
+        return false;
+    }
+    ctx->ws_bytes = bytes;
+    return true;
+}
+
+// Bump allocator over the context workspace (256-byte aligned slices).
+struct Bump {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    }
+};
+
+static int check_ctx(mh_context_t c) {
+    if (!c) {
+        set_error("null context");
+        return MH_ERR_ARGUMENT;
+    }
+    hipError_t e = hipSetDevice(reinterpret_cast<Context*>(c)->device);
+    if (e != hipSuccess) {
+        set_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return MH_ERR_HIP;
+    }
+    return MH_OK;
+}
+
+static int check_mesh(const mh_mesh_t* m) {
+    MH_REQUIRE(m != nullptr, "mesh: null");
+    MH_REQUIRE(m->nfaces >= 0 && m->nvertices >= 0, "mesh: negative size");
+    if (m->nfaces > 0)
+        MH_REQUIRE(m->vertices && m->faces && m->centers && m->normals && m->areas && m->radii,
+                   "mesh: null array");
+    return MH_OK;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+#define CTX(c) reinterpret_cast<mh::Context*>(c)
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != MH_OK) return rc_; \
+    } while (0)
+
+extern "C" {
+
+int mh_version(void) { return MHIP_VERSION; }
+const char* mh_last_error(void) { return mh::last_error(); }
+
+int mh_context_create(int device, void* stream, mh_context_t* out) {
+    if (!out) {
+        set_error("mh_context_create: null output pointer");
+        return MH_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device");
+        return MH_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        set_error("device index out of range");
+        return MH_ERR_ARGUMENT;
+    }
+    MH_HIP_CHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MH_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error(std::string("libmhip is built for gfx950; device is ") + prop.gcnArchName);
+        return MH_ERR_NO_DEVICE;
+    }
+    Context* c = new Context();
+    c->device = device;
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+    if (hipMalloc(&c->d_flag, 256) != hipSuccess) {
+        delete c;
+        set_error("mh_context_create: hipMalloc failed");
+        return MH_ERR_HIP;
+    }
+    *out = reinterpret_cast<mh_context_t>(c);
+    return MH_OK;
+}
+
+int mh_context_destroy(mh_context_t ctx) {
+    if (!ctx) return MH_OK;
+    Context* c = CTX(ctx);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->ws) (void)hipFree(c->ws);
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    delete c;
+    return MH_OK;
+}
+
+int mh_synchronize(mh_context_t ctx) {
+    TRY(check_ctx(ctx));
+    MH_HIP_CHECK(hipStreamSynchronize(CTX(ctx)->stream));
+    return MH_OK;
+}
+
+int mh_profile_enable(mh_context_t ctx, int on) {
+    TRY(check_ctx(ctx));
+    CTX(ctx)->profiling = on != 0;
+    return MH_OK;
+}
+
+int mh_profile_report(mh_context_t ctx, char* buf, int64_t cap) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(buf && cap > 0, "profile_report: null buffer");
+    Context* c = CTX(ctx);
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    struct Agg {
+        std::string name;
+        int64_t n = 0;
+        double ms = 0, flops = 0, bytes = 0;
+    };
+    std::vector<Agg> agg;
+    for (ProfRec& r : c->prof) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.start, r.stop);
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+        Agg* a = nullptr;
+        for (Agg& x : agg)
+            if (x.name == r.name) a = &x;
+        if (!a) {
+            agg.push_back(Agg());
+            a = &agg.back();
+            a->name = r.name;
+        }
+        a->n += 1;
+        a->ms += ms;
+        a->flops += r.flops;
+        a->bytes += r.bytes;
+    }
+    c->prof.clear();
+    std::string js = "{";
+    for (size_t q = 0; q < agg.size(); ++q) {
+        char tmp[256];
+        snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"launches\": %lld, \"ms\": %.6f, \"flops\": %.6e, \"bytes\": %.6e}",
+                 q ? ", " : "", agg[q].name.c_str(), (long long)agg[q].n, agg[q].ms, agg[q].flops,
+                 agg[q].bytes);
+        js += tmp;
+    }
+    js += "}";
+    if ((int64_t)js.size() + 1 > cap) {
+        set_error("profile_report: buffer too small");
+        return MH_ERR_ARGUMENT;
+    }
+    memcpy(buf, js.c_str(), js.size() + 1);
+    return MH_OK;
+}
+
+int mh_gf_eval(mh_context_t ctx, int gf_mask, int op, int wrt_first, int64_t npairs,
+               const double* points, const mh_panel_t* panels, double k, double* out) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(npairs >= 0, "gf_eval: negative npairs");
+    MH_REQUIRE(op >= 0 && op <= 3, "gf_eval: unknown op");
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "gf_eval: bad Green-function mask");
+    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
+               "gf_eval: choose one wave term");
+    if (npairs == 0) return MH_OK;
+    MH_REQUIRE(points && panels && out, "gf_eval: null array");
+    return launch_gf_eval(CTX(ctx), gf_mask, op, wrt_first, npairs, points, panels, k,
+                          reinterpret_cast<double2*>(out));
+}
+
+int mh_wave_term_eval(mh_context_t ctx, int term, int64_t npoints, const double* r,
+                      const double* z, double* out) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(npoints >= 0, "wave_term: negative size");
+    if (npoints == 0) return MH_OK;
+    MH_REQUIRE(r && z && out, "wave_term: null array");
+    return launch_wave_term(CTX(ctx), term, npoints, r, z, reinterpret_cast<double2*>(out));
+}
+
+static int assemble_impl(Context* c, const mh_mesh_t* mesh, int gf, int form, int64_t nk,
+                         const double* ks_host, double* S, double* D, double* K, int64_t ld,
+                         int64_t stride, Bump& bump, size_t base) {
+    const int64_t n = mesh->nfaces;
+    const size_t o_p = bump.take(sizeof(PanelRec) * (size_t)n);
+    const size_t o_k = bump.take(sizeof(double) * (size_t)nk);
+    if (!ensure_workspace(c, base + bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws) + base;
+    PanelRec* P = reinterpret_cast<PanelRec*>(ws + o_p);
+    double* dk = reinterpret_cast<double*>(ws + o_k);
+    MH_HIP_CHECK(hipMemcpyAsync(dk, ks_host, sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
+    TRY(launch_pack_panels(c, mesh, P));
+    return launch_assemble(c, mesh, P, gf, form, nk, dk, reinterpret_cast<double2*>(S),
+                           reinterpret_cast<double2*>(D), reinterpret_cast<double2*>(K), ld, stride);
+}
+
+int mh_assemble(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int formulation, int64_t nk,
+                const double* wavenumbers, double* S, double* D, double* K, int64_t ld,
+                int64_t stride) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble: bad Green-function mask");
+    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
+               "assemble: choose one wave term");
+    MH_REQUIRE(formulation >= 1 && formulation <= 3, "assemble: formulation must be 1, 2 or 3");
+    MH_REQUIRE(nk >= 0, "assemble: negative nk");
+    const int64_t n = mesh->nfaces;
+    if (n == 0 || nk == 0) return MH_OK;
+    MH_REQUIRE(wavenumbers && S, "assemble: null array");
+    MH_REQUIRE(!(formulation & 1) || D, "assemble: D required for the direct formulation");
+    MH_REQUIRE(!(formulation & 2) || K, "assemble: K required for the indirect formulation");
+    MH_REQUIRE(ld >= n && (nk == 1 || stride >= ld * n), "assemble: ld/stride too small");
+    Bump bump;
+    return assemble_impl(CTX(ctx), mesh, gf_mask, formulation, nk, wavenumbers, S, D, K, ld,
+                         stride, bump, 0);
+}
+
+int mh_lu_factor(mh_context_t ctx, int64_t n, int64_t batch, double* A, int64_t ld, int64_t stride,
+                 int64_t* ipiv) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(n >= 0 && batch >= 0, "lu_factor: negative size");
+    if (n == 0 || batch == 0) return MH_OK;
+    MH_REQUIRE(A && ipiv, "lu_factor: null array");
+    MH_REQUIRE(ld >= n && (batch == 1 || stride >= ld * n), "lu_factor: ld/stride too small");
+    Context* c = CTX(ctx);
+    Bump bump;
+    const size_t o_info = bump.take(sizeof(int) * (size_t)batch);
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    int* info = reinterpret_cast<int*>(static_cast<char*>(c->ws) + o_info);
+    MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
+    TRY(lu_factor(c, n, n, batch, reinterpret_cast<double2*>(A), ld, stride, ipiv, info));
+    std::vector<int> h(batch);
+    MH_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t b = 0; b < batch; ++b)
+        if (h[b]) {
+            set_error("lu_factor: exact zero pivot in batch item " + std::to_string(b));
+            return MH_ERR_SINGULAR;
+        }
+    return MH_OK;
+}
+
+int mh_lu_solve(mh_context_t ctx, int64_t n, int64_t batch, const double* LU, int64_t ld,
+                int64_t stride, const int64_t* ipiv, int trans, int64_t nrhs, double* B, int64_t ldb,
+                int64_t bstride) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(n >= 0 && batch >= 0 && nrhs >= 0, "lu_solve: negative size");
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    MH_REQUIRE(LU && ipiv && B, "lu_solve: null array");
+    MH_REQUIRE(ld >= n && ldb >= n, "lu_solve: leading dimension too small");
+    return lu_solve(CTX(ctx), n, batch, reinterpret_cast<const double2*>(LU), ld, stride, ipiv,
+                    trans ? 1 : 0, nrhs, reinterpret_cast<double2*>(B), ldb, bstride);
+}
+
+// shared by mh_solve and mh_freq_sweep: LU + solve for `batch` systems whose
+// right-hand sides are already in rhs (overwritten by phi)
+static int solve_impl(Context* c, int64_t n, int64_t batch, int direct, double2* DK,
+                      const double2* S, int64_t ld, int64_t stride, double2* rhs, int64_t nrhs,
+                      int64_t ldb, int64_t bstride, double2* tmp, int64_t* ipiv, int* info) {
+    MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
+    TRY(lu_factor(c, n, n, batch, DK, ld, stride, ipiv, info));
+    if (direct) {  // phi = D \ (S bc)
+        TRY(zgemm_nn_batched(c, n, nrhs, batch, S, ld, stride, rhs, ldb, bstride, tmp, ldb, bstride));
+        TRY(lu_solve(c, n, batch, DK, ld, stride, ipiv, 0, nrhs, tmp, ldb, bstride));
+        MH_HIP_CHECK(hipMemcpyAsync(rhs, tmp, sizeof(double2) * (size_t)(bstride * batch),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    } else {  // sigma = K \ bc; phi = S sigma
+        TRY(lu_solve(c, n, batch, DK, ld, stride, ipiv, 0, nrhs, rhs, ldb, bstride));
+        TRY(zgemm_nn_batched(c, n, nrhs, batch, S, ld, stride, rhs, ldb, bstride, tmp, ldb, bstride));
+        MH_HIP_CHECK(hipMemcpyAsync(rhs, tmp, sizeof(double2) * (size_t)(bstride * batch),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    }
+    return MH_OK;
+}
+
+static int check_info(Context* c, const int* info, int64_t batch) {
+    std::vector<int> h(batch);
+    MH_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t b = 0; b < batch; ++b)
+        if (h[b]) {
+            set_error("solve: exact zero pivot in batch item " + std::to_string(b));
+            return MH_ERR_SINGULAR;
+        }
+    return MH_OK;
+}
+
+int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK, const double* S,
+             int64_t ld, int64_t stride, const double* bc, int64_t nrhs, double* phi, int64_t ldb,
+             int64_t bstride) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(n >= 0 && batch >= 0 && nrhs >= 0, "solve: negative size");
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    MH_REQUIRE(DK && S && bc && phi, "solve: null array");
+    MH_REQUIRE(ld >= n && ldb >= n, "solve: leading dimension too small");
+    MH_REQUIRE(batch == 1 || (stride >= ld * n && bstride >= ldb * nrhs), "solve: stride too small");
+    Context* c = CTX(ctx);
+    const int64_t bs = (batch == 1) ? ldb * nrhs : bstride;
+    Bump bump;
+    const size_t o_tmp = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * batch));
+    const size_t o_info = bump.take(sizeof(int) * (size_t)batch);
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    if (phi != bc)
+        MH_HIP_CHECK(hipMemcpyAsync(phi, bc, sizeof(double2) * (size_t)(bs * batch),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    TRY(solve_impl(c, n, batch, direct, reinterpret_cast<double2*>(DK),
+                   reinterpret_cast<const double2*>(S), ld, stride, reinterpret_cast<double2*>(phi),
+                   nrhs, ldb, bs, reinterpret_cast<double2*>(ws + o_tmp),
+                   reinterpret_cast<int64_t*>(ws + o_piv), reinterpret_cast<int*>(ws + o_info)));
+    return check_info(c, reinterpret_cast<int*>(ws + o_info), batch);
+}
+
+int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
+                  int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
+                  const double* dof_fields, int64_t nbeta, const double* betas, double* A,
+                  double* B, double* F) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "freq_sweep: bad Green-function mask");
+    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
+               "freq_sweep: choose one wave term");
+    MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0, "freq_sweep: negative size");
+    MH_REQUIRE(g > 0.0 && rho > 0.0, "freq_sweep: g and rho must be positive");
+    if (precision != MH_FP64) {
+        set_error("freq_sweep: only MH_FP64 is available in this build");
+        return MH_ERR_NOT_SUPPORTED;
+    }
+    const int64_t n = mesh->nfaces;
+    const int64_t nr = ndof + nbeta;
+    if (n == 0 || nomega == 0 || nr == 0) return MH_OK;
+    MH_REQUIRE(omegas, "freq_sweep: null omegas");
+    MH_REQUIRE(ndof == 0 || (dof_fields && A && B), "freq_sweep: null dof/output array");
+    MH_REQUIRE(nbeta == 0 || (betas && F), "freq_sweep: null heading/output array");
+    for (int64_t w = 0; w < nomega; ++w)
+        MH_REQUIRE(omegas[w] > 0.0, "freq_sweep: frequencies must be positive");
+    Context* c = CTX(ctx);
+    // frequencies are processed in chunks that keep S and D/K under 32 GiB
+    const size_t mat_bytes = sizeof(double2) * (size_t)n * (size_t)n;
+    int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_bytes));
+    if (chunk < 1) chunk = 1;
+    if (chunk > nomega) chunk = nomega;
+    const int64_t ld = n, stride = n * n, ldb = n, bstride = n * nr;
+    Bump bump;
+    const size_t o_S = bump.take(mat_bytes * chunk);
+    const size_t o_D = bump.take(mat_bytes * chunk);
+    const size_t o_rhs = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
+    const size_t o_tmp = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
+    const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * chunk));
+    const size_t o_info = bump.take(sizeof(int) * (size_t)chunk);
+    const size_t o_om = bump.take(sizeof(double) * (size_t)nomega);
+    const size_t o_k = bump.take(sizeof(double) * (size_t)nomega);
+    const size_t o_be = bump.take(sizeof(double) * (size_t)(nbeta > 0 ? nbeta : 1));
+    const size_t o_P = bump.take(sizeof(PanelRec) * (size_t)n);
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    double2* dS = reinterpret_cast<double2*>(ws + o_S);
+    double2* dD = reinterpret_cast<double2*>(ws + o_D);
+    double2* rhs = reinterpret_cast<double2*>(ws + o_rhs);
+    double2* tmp = reinterpret_cast<double2*>(ws + o_tmp);
+    int64_t* piv = reinterpret_cast<int64_t*>(ws + o_piv);
+    int* info = reinterpret_cast<int*>(ws + o_info);
+    double* d_om = reinterpret_cast<double*>(ws + o_om);
+    double* d_k = reinterpret_cast<double*>(ws + o_k);
+    double* d_be = reinterpret_cast<double*>(ws + o_be);
+    PanelRec* P = reinterpret_cast<PanelRec*>(ws + o_P);
+    std::vector<double> hk(nomega);
+    for (int64_t w = 0; w < nomega; ++w) hk[w] = omegas[w] * omegas[w] / g;  // waves.jl:30
+    MH_HIP_CHECK(hipMemcpyAsync(d_om, omegas, sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
+    MH_HIP_CHECK(hipMemcpyAsync(d_k, hk.data(), sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
+    if (nbeta)
+        MH_HIP_CHECK(hipMemcpyAsync(d_be, betas, sizeof(double) * nbeta, hipMemcpyHostToDevice, c->stream));
+    TRY(launch_pack_panels(c, mesh, P));
+    const int form = direct ? MH_DIRECT : MH_INDIRECT;
+    for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
+        const int64_t nw = (nomega - w0) < chunk ? (nomega - w0) : chunk;
+        TRY(launch_assemble(c, mesh, P, gf_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
+                            direct ? nullptr : dD, ld, stride));
+        TRY(launch_bcs(c, mesh, nw, d_om + w0, g, ndof, dof_fields, nbeta, d_be, rhs, ldb, bstride));
+        TRY(solve_impl(c, n, nw, direct, dD, dS, ld, stride, rhs, nr, ldb, bstride, tmp, piv, info));
+        TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, rhs, ldb, bstride,
+                          A ? A + w0 * ndof * ndof : nullptr, B ? B + w0 * ndof * ndof : nullptr,
+                          F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr));
+        TRY(check_info(c, info, nw));
+    }
+    return MH_OK;
+}
+
+// ---- host-array entry points -----------------------------------------------
+static int host_single(int64_t nf, int64_t nv, const double* V, const int64_t* Fc,
+                       const double* C, const double* N, const double* Ar, const double* R,
+                       const double* dof3, double omega, double g, double rho, int want_diff,
+                       double* out) {
+    MH_REQUIRE(nf > 0 && nv > 0, "host entry: empty mesh");
+    MH_REQUIRE(V && Fc && C && N && Ar && R && dof3 && out, "host entry: null array");
+    mh_context_t ctx;
+    TRY(mh_context_create(0, nullptr, &ctx));
+    std::vector<void*> bufs;
+    auto dev = [&](const void* h, size_t bytes) -> void* {
+        void* d = nullptr;
+        if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+        bufs.push_back(d);
+        if (h && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return d;
+    };
+    int rc = MH_OK;
+    std::vector<double> field(nf * 3);
+    for (int c = 0; c < 3; ++c)
+        for (int64_t i = 0; i < nf; ++i) field[i + c * nf] = dof3[c];
+    mh_mesh_t m;
+    m.nfaces = nf;
+    m.nvertices = nv;
+    m.vertices = (const double*)dev(V, sizeof(double) * nv * 3);
+    m.faces = (const int64_t*)dev(Fc, sizeof(int64_t) * nf * 4);
+    m.centers = (const double*)dev(C, sizeof(double) * nf * 3);
+    m.normals = (const double*)dev(N, sizeof(double) * nf * 3);
+    m.areas = (const double*)dev(Ar, sizeof(double) * nf);
+    m.radii = (const double*)dev(R, sizeof(double) * nf);
+    double* dfield = (double*)dev(field.data(), sizeof(double) * nf * 3);
+    double* dA = (double*)dev(nullptr, sizeof(double));
+    double* dB = (double*)dev(nullptr, sizeof(double));
+    double* dF = (double*)dev(nullptr, sizeof(double) * 2);
+    if (!m.vertices || !m.faces || !m.centers || !m.normals || !m.areas || !m.radii || !dfield ||
+        !dA || !dB || !dF) {
+        set_error("host entry: device allocation or copy failed");
+        rc = MH_ERR_HIP;
+    }
+    const double beta = 0.0;
+    if (rc == MH_OK)
+        rc = mh_freq_sweep(ctx, &m, MH_GF_DEFAULT, 1, MH_FP64, 1, &omega, g, rho, 1, dfield,
+                           want_diff ? 1 : 0, &beta, dA, dB, want_diff ? dF : nullptr);
+    if (rc == MH_OK) {
+        if (want_diff) {
+            if (hipMemcpy(out, dF, sizeof(double) * 2, hipMemcpyDeviceToHost) != hipSuccess) rc = MH_ERR_HIP;
+        } else {
+            if (hipMemcpy(out, dA, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(out + 1, dB, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = MH_ERR_HIP;
+        }
+        if (rc != MH_OK) set_error("host entry: copy back failed");
+    }
+    for (void* p : bufs) (void)hipFree(p);
+    mh_context_destroy(ctx);
+    return rc;
+}
+
+int mh_calculate_radiation_forces_host(int64_t nfaces, int64_t nvertices, const double* vertices,
+                                       const int64_t* faces, const double* centers,
+                                       const double* normals, const double* areas,
+                                       const double* radii, const double* dof3, double omega,
+                                       double g, double rho, double* out) {
+    MH_REQUIRE(omega > 0.0, "calculate_radiation_forces: omega must be positive");
+    return host_single(nfaces, nvertices, vertices, faces, centers, normals, areas, radii, dof3,
+                       omega, g, rho, 0, out);
+}
+
+int mh_diffraction_force_host(int64_t nfaces, int64_t nvertices, const double* vertices,
+                              const int64_t* faces, const double* centers, const double* normals,
+                              const double* areas, const double* radii, const double* dof3,
+                              double omega, double g, double rho, double* out) {
+    MH_REQUIRE(omega > 0.0, "DiffractionForce: omega must be positive");
+    return host_single(nfaces, nvertices, vertices, faces, centers, normals, areas, radii, dof3,
+                       omega, g, rho, 1, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// Device-side Green-function terms for the boundary-element pair kernels.
+//
+// Each function restates one term of the reference (MarineHydro.jl, file:line
+// given per function) for gfx950 in FP64.  The Rankine near-field integral and
+// its gradient are evaluated in one pass over the four edges (the reference
+// walks the edges twice, once in `integral` and once in
+// `both_integral_and_integral_gradient`), and the wave term returns value and
+// both derivatives together.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mh {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
+
+// Green-function term mask (same bits as the C-ABI header).
+enum : int {
+    GF_RANKINE = 1,
+    GF_RANKINE_REFLECTED = 2,
+    GF_RANKINE_REFLECTED_NEG = 4,
+    GF_WU = 8,
+    GF_DELHOMMEAU = 16,
+};
+
+// Packed source-panel record (one per face, 192 B), read wave-uniformly by
+// the pair kernels so it lands in scalar registers.
+struct __attribute__((aligned(64))) PanelRec {
+    double c[3];
+    double n[3];
+    double area;
+    double radius;
+    double v[4][3];
+    double pad[4];
+};
+static_assert(sizeof(PanelRec) == 192, "PanelRec layout");
+
+struct cplx {
+    double re, im;
+};
+
+// ---------------------------------------------------------------------------
+// Rankine panel integral and gradient, reference
+// src/green_functions/rankine.jl:24-63 (value) and :65-123 (gradient).
+// Output: I = integral of 1/|x - xi| over the panel; g = gradient w.r.t. the
+// source point xi (the reference's with_respect_to_first_variable=false).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rankine_panel(double x0, double x1, double x2,
+                                              const PanelRec& p, double& I,
+                                              double g[3]) {
+    const double rb0 = x0 - p.c[0], rb1 = x1 - p.c[1], rb2 = x2 - p.c[2];
+    const double r = sqrt(rb0 * rb0 + rb1 * rb1 + rb2 * rb2);
+    const double R = p.radius;
+    if (r > 7.0 * R) {  // rankine.jl:33-34, :74-77
+        const double ir = 1.0 / r;
+        I = p.area * ir;
+        const double s = (ir * ir * ir) * p.area;
+        g[0] = rb0 * s;
+        g[1] = rb1 * s;
+        g[2] = rb2 * s;
+        return;
+    }
+    const double n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
+    const double GZ = rb0 * n0 + rb1 * n1 + rb2 * n2;
+    const double aGZ = fabs(GZ);
+    const double sg = (GZ > 0.0) ? 1.0 : ((GZ < 0.0) ? -1.0 : 0.0);
+    const bool use_at = aGZ >= 1e-4 * R;
+    double RR[4], D[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        D[v][0] = x0 - p.v[v][0];
+        D[v][1] = x1 - p.v[v][1];
+        D[v][2] = x2 - p.v[v][2];
+        RR[v] = sqrt(D[v][0] * D[v][0] + D[v][1] * D[v][1] + D[v][2] * D[v][2]);
+    }
+    double acc = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int w = (v + 1) & 3;
+        const double e0 = p.v[w][0] - p.v[v][0];
+        const double e1 = p.v[w][1] - p.v[v][1];
+        const double e2 = p.v[w][2] - p.v[v][2];
+        const double L = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+        if (L >= 1e-3 * R) {  // degenerate edges skipped, rankine.jl:45, :90
+            const double iL = 1.0 / L;
+            const double u0 = e0 * iL, u1 = e1 * iL, u2 = e2 * iL;
+            const double y0 = n1 * u2 - n2 * u1;  // GYX = n x (edge / L)
+            const double y1 = n2 * u0 - n0 * u2;
+            const double y2 = n0 * u1 - n1 * u0;
+            const double GY = D[v][0] * y0 + D[v][1] * y1 + D[v][2] * y2;
+            const double s = RR[w] + RR[v];
+            const double ANT = 2.0 * GY * L;
+            const double DNT = s * s - L * L + 2.0 * aGZ * s;
+            const double ANL = s + L;
+            const double DNL = s - L;
+            const double ALDEN = log(ANL / DNL);
+            const double AT = use_at ? atan2(ANT, DNT) : 0.0;
+            // value (rankine.jl:47-58)
+            if (use_at) acc -= 2.0 * aGZ * AT;
+            if (fabs(GY) > 1e-5) acc += GY * ALDEN;
+            // gradient (rankine.jl:105-114)
+            const double iw = 1.0 / RR[w], iv = 1.0 / RR[v];
+            const double a0 = D[w][0] * iw + D[v][0] * iv;  // ANLX
+            const double a1 = D[w][1] * iw + D[v][1] * iv;
+            const double a2 = D[w][2] * iw + D[v][2] * iv;
+            const double cL = GY * (DNL - ANL) / (ANL * DNL);
+            const double den = ANT * ANT + DNT * DNT;
+            const double cT = 2.0 * aGZ / den;
+            const double dnt_a = 2.0 * (s + aGZ);
+            const double dnt_n = 2.0 * sg * s;
+            const double at_n = 2.0 * sg * AT;
+            {
+                const double ANTX = 2.0 * L * y0, DNTX = dnt_a * a0 + dnt_n * n0;
+                g0 -= ALDEN * y0 - at_n * n0 + cL * a0 - cT * (ANTX * DNT - DNTX * ANT);
+            }
+            {
+                const double ANTX = 2.0 * L * y1, DNTX = dnt_a * a1 + dnt_n * n1;
+                g1 -= ALDEN * y1 - at_n * n1 + cL * a1 - cT * (ANTX * DNT - DNTX * ANT);
+            }
+            {
+                const double ANTX = 2.0 * L * y2, DNTX = dnt_a * a2 + dnt_n * n2;
+                g2 -= ALDEN * y2 - at_n * n2 + cL * a2 - cT * (ANTX * DNT - DNTX * ANT);
+            }
+        }
+    }
+    I = acc;
+    g[0] = g0;
+    g[1] = g1;
+    g[2] = g2;
+}
+
+// ---------------------------------------------------------------------------
+// Wu approximation of the wave term, reference src/green_functions/wu.jl.
+// Coefficients are the published fits the reference uses (must be identical).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double struve_h0(double xx) {  // wu.jl:295-327
+    if (xx <= 3.0) {
+        const double yy = (xx / 3.0) * (xx / 3.0);
+        const double h = 1.909859164 +
+                   (-1.909855001 +
+                    (0.687514637 +
+                     (-0.126164557 + (0.013828813 + -0.000876918 * yy) * yy) * yy) *
+                        yy) *
+                       yy;
+        return h * (xx / 3.0);
+    }
+    const double yy = (3.0 / xx) * (3.0 / xx);
+    const double c1 = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
+    const double c2 = kPi * xx * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
+    return c1 / c2 + y0(xx);
+}
+
+__device__ __forceinline__ double struve_h1(double xx) {  // wu.jl:331-362
+    if (xx <= 3.0) {
+        const double yy = (xx / 3.0) * (xx / 3.0);
+        return (1.909859286 +
+                (-1.145914713 +
+                 (0.294656958 +
+                  (-0.042070508 + (0.003785727 + -0.000207183 * yy) * yy) * yy) *
+                     yy) *
+                    yy) *
+               yy;
+    }
+    const double yy = (3.0 / xx) * (3.0 / xx);
+    const double c1 = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
+    const double c2 = kPi * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
+    return c1 / c2 + y1(xx);
+}
+
+// Horner in the reference's association: (c[d-1] + c[d] t) t, (c[i] + acc) t,
+// ..., c[0] + acc.
+template <int DEG>
+__device__ __forceinline__ double horner_ref(const double (&c)[DEG + 1], double t) {
+    double acc = (c[DEG - 1] + c[DEG] * t) * t;
+#pragma unroll
+    for (int i = DEG - 2; i >= 1; --i) acc = (c[i] + acc) * t;
+    return c[0] + acc;
+}
+
+// G(h, v), dG/dh, dG/dv of the Wu wave term in reduced coordinates
+// (greens wu.jl:5-13, both_greens_and_gradient_greens wu.jl:19-30).
+// If WANT_GRAD is false only G is computed (the reference's `greens`).
+template <bool WANT_GRAD>
+__device__ __forceinline__ void wu_dimless(double hh, double vv, cplx& G, cplx& dGdh,
+                                           cplx& dGdv) {
+    constexpr double A[10] = {1.21, -13.328, 215.896, -1763.96, 8418.94,
+                              -24314.21, 42002.57, -41592.9, 21859.0, -4838.6};
+    constexpr double B[10] = {0.938, 5.737, -67.92, 796.534, -4780.77,
+                              17137.74, -36618.81, 44894.06, -29030.24, 7671.22};
+    constexpr double Cc[8] = {1.268, -9.747, 209.653, -1397.89, 5155.67, -9844.35, 9136.4, -3272.62};
+    constexpr double Dd[10] = {0.632, -40.97, 667.16, -6072.07, 31127.39,
+                               -96293.05, 181856.75, -205690.43, 128170.2, -33744.6};
+    const double dd = sqrt(hh * hh + vv * vv);
+    const double alpha = -vv / dd;
+    const double beta = hh / dd;
+    const double rho = dd / (1.0 + dd);
+    const double ev = exp(vv);
+    const double d3 = dd * dd * dd;
+    const double om = 1.0 - rho;
+    const double om3 = om * om * om;
+    // L0 (wu.jl:282-293) with Lp (wu.jl:266-280)
+    double PP = log(0.5 * (dd - vv)) + 0.5772156649 - 2.0 * (dd * dd);
+    PP = ev * PP;
+    PP = PP + dd * dd - vv;
+    double RR = (1.0 - beta) * horner_ref<9>(A, rho);
+    RR -= beta * horner_ref<9>(B, rho);
+    RR -= alpha * horner_ref<7>(Cc, rho) / (1.0 + 6.0 * alpha * rho * (1.0 - rho));
+    RR += beta * (1.0 - beta) * horner_ref<9>(Dd, rho);
+    const double L0 = 2.0 * PP / (1.0 + d3) + 2.0 * (rho * om3 * RR);
+    const double twopi = 2.0 * kPi;
+    const double H0 = struve_h0(hh), J0 = j0(hh);
+    G.re = -L0 - twopi * H0 * ev;  // -L0 - W, W = 2 pi (H0 - i J0) e^v
+    G.im = twopi * J0 * ev;
+    if (WANT_GRAD) {
+        constexpr double dA[10] = {2.948, -24.53, 249.69, -754.85, -1187.71,
+                                   16370.75, -48811.41, 68220.87, -46688.0, 12622.25};
+        constexpr double dB[10] = {1.11, 2.894, -76.765, 1565.35, -11336.19,
+                                   44270.15, -97014.11, 118879.26, -76209.82, 19923.28};
+        constexpr double dC[6] = {14.19, -148.24, 847.8, -2318.58, 3168.35, -1590.27};
+        // Ls (wu.jl:145-157) with Lsp (wu.jl:131-143)
+        double PS = (beta + hh) / (dd - vv);
+        PS = PS - 2.0 * beta + 2.0 * ev * dd - hh;
+        double QS = exp(-dd) * (1.0 - beta);
+        QS = QS * (1.0 + dd / (1.0 + d3));
+        double R2 = beta * horner_ref<9>(dA, rho);
+        R2 -= (1.0 - alpha) * horner_ref<9>(dB, rho);
+        R2 += beta * (1.0 - beta) * rho * (1.0 - 2.0 * rho) * horner_ref<5>(dC, rho);
+        const double Ls = 2.0 * PS / (1.0 + d3) - 4.0 * QS + 2.0 * (rho * om3 * R2);
+        const double H1 = struve_h1(hh), J1 = j1(hh);
+        // -Ls - Wh, Wh = 2 pi (2/pi - H1 + i J1) e^v (wu.jl:122-129)
+        dGdh.re = -Ls - twopi * (2.0 / kPi - H1) * ev;
+        dGdh.im = -(twopi * J1 * ev);
+        dGdv.re = G.re + 2.0 / dd;  // wu.jl:27
+        dGdv.im = G.im;
+    }
+}
+
+// Horizontal distance as the reference computes it, including its
+// perturbation at zero (abstract_greens_function.jl:91-99).
+__device__ __forceinline__ double horiz_dist(double dx, double dy) {
+    const double d2 = dx * dx + dy * dy;
+    const double purt = 1e-18;
+    return (d2 == 0.0) ? sqrt(d2 + purt * purt) - purt : sqrt(d2);
+}
+
+}  // namespace mh

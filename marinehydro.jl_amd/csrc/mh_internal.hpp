@@ -1,0 +1,108 @@
+// Host-side internals shared by the libmhip translation units: the context,
+// error reporting and the launchers each unit exports to the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mhip.h"
+#include "mh_device.hpp"
+
+namespace mh {
+
+// Thread-local last error message (read through mh_last_error()).
+void set_error(const std::string& msg);
+const char* last_error();
+
+// One timed kernel launch (kernel timer, enabled per context).
+struct ProfRec {
+    const char* name;
+    hipEvent_t start, stop;
+    double flops, bytes;  // algorithmic work of the launch
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // growable device scratch: one bump region per call
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    int* d_flag = nullptr;  // device error / info flag (one int)
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+};
+
+// Brackets one kernel launch with events on the context's stream when the
+// kernel timer is on; no cost otherwise.
+struct ProfScope {
+    Context* c;
+    int idx = -1;
+    ProfScope(Context* ctx, const char* name, double flops, double bytes) : c(ctx) {
+        if (!c->profiling) return;
+        ProfRec r{name, nullptr, nullptr, flops, bytes};
+        if (hipEventCreate(&r.start) != hipSuccess || hipEventCreate(&r.stop) != hipSuccess) return;
+        (void)hipEventRecord(r.start, c->stream);
+        c->prof.push_back(r);
+        idx = (int)c->prof.size() - 1;
+    }
+    ~ProfScope() {
+        if (idx >= 0) (void)hipEventRecord(c->prof[idx].stop, c->stream);
+    }
+};
+
+// Grows ctx->ws to at least `bytes` (synchronous hipMalloc; only when larger).
+bool ensure_workspace(Context* ctx, size_t bytes);
+
+#define MH_HIP_CHECK(expr)                                                   \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            ::mh::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return MH_ERR_HIP;                                               \
+        }                                                                    \
+    } while (0)
+
+#define MH_REQUIRE(cond, msg)                                                \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            ::mh::set_error(msg);                                            \
+            return MH_ERR_ARGUMENT;                                          \
+        }                                                                    \
+    } while (0)
+
+// ---- launchers (assemble.hip) ---------------------------------------------
+// Packs the mesh into PanelRec records; flags a face index out of range in
+// *d_flag.
+int launch_pack_panels(Context* ctx, const mh_mesh_t* mesh, PanelRec* out);
+int launch_assemble(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels,
+                    int gf_mask, int form, int64_t nk, const double* d_k, double2* S,
+                    double2* D, double2* K, int64_t ld, int64_t stride);
+int launch_gf_eval(Context* ctx, int gf_mask, int op, int wrt_first, int64_t npairs,
+                   const double* points, const mh_panel_t* panels, double k, double2* out);
+int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const double* z,
+                     double2* out);
+
+// ---- launchers (solve.hip) --------------------------------------------------
+int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
+              int64_t stride, int64_t* ipiv, int* d_info);
+int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
+             int64_t stride, const int64_t* ipiv, int trans, int64_t nrhs, double2* B,
+             int64_t ldb, int64_t bstride);
+// Y[b] = S[b] * X[b]  (n x n times n x nrhs, complex)
+int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
+                     int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
+                     double2* Y, int64_t ldy, int64_t ystride);
+
+// ---- launchers (physics.hip) ------------------------------------------------
+// rhs[w] columns: ndof radiation BCs then nbeta Airy BCs, for each omega.
+int launch_bcs(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, const double* d_omega,
+               double g, int64_t ndof, const double* dof_fields, int64_t nbeta,
+               const double* d_beta, double2* rhs, int64_t ldb, int64_t bstride);
+int launch_forces(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, const double* d_omega,
+                  double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+                  const double2* phi, int64_t ldb, int64_t bstride, double* A, double* B,
+                  double2* F);
+
+}  // namespace mh

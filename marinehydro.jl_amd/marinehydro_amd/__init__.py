@@ -1,0 +1,20 @@
+"""marinehydro_amd: MI355X (gfx950) boundary-element hot path of MarineHydro.jl.
+
+The numerical work runs in libmhip.so (hand-written HIP kernels behind the C
+ABI of include/mhip.h); this package is the host-side mirror of the
+reference's Julia API for that path.
+"""
+from . import _lib
+from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineReflected,
+                  RankineReflectedNegative, GFWu, ExactGuevelDelhommeau, gf_mask, Element, element,
+                  gf_eval, greens, gradient_greens, integral, integral_gradient,
+                  both_integral_and_integral_gradient, DeviceMesh, assemble_matrices,
+                  assemble_matrices_batched, assemble_matrix_wu, solve, radiation_bc,
+                  integrate_pressure, airy_waves_potential, airy_waves_velocity,
+                  airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep,
+                  calculate_radiation_forces, DiffractionForce)
+from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
+                     mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
+                     translational_dofs, rotational_dofs, rigid_dofs)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,130 @@
+"""ctypes binding of libmhip.so (C ABI declared in include/mhip.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C marinehydro.jl_amd/csrc``).  There is no fallback: if the library
+is missing or no gfx950 device is present every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmhip.so")
+
+MH_OK = 0
+_ERRORS = {
+    1: ValueError,          # MH_ERR_ARGUMENT
+    2: RuntimeError,        # MH_ERR_HIP
+    3: ArithmeticError,     # MH_ERR_SINGULAR
+    4: NotImplementedError, # MH_ERR_NOT_SUPPORTED
+    5: RuntimeError,        # MH_ERR_NO_DEVICE
+    6: ArithmeticError,     # MH_ERR_NOT_CONVERGED
+}
+
+# Green-function term bits, include/mhip.h
+GF_RANKINE = 1
+GF_RANKINE_REFLECTED = 2
+GF_RANKINE_REFLECTED_NEG = 4
+GF_WU = 8
+GF_DELHOMMEAU = 16
+
+DIRECT, INDIRECT, BOTH = 1, 2, 3
+OP_GREENS, OP_GRADIENT_GREENS, OP_INTEGRAL, OP_INTEGRAL_GRADIENT = 0, 1, 2, 3
+FP64, MIXED = 0, 1
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_d = C.c_double
+_int = C.c_int
+
+_SIGS = {
+    "mh_version": (_int, []),
+    "mh_last_error": (C.c_char_p, []),
+    "mh_context_create": (_int, [_int, _vp, C.POINTER(_vp)]),
+    "mh_context_destroy": (_int, [_vp]),
+    "mh_synchronize": (_int, [_vp]),
+    "mh_profile_enable": (_int, [_vp, _int]),
+    "mh_profile_report": (_int, [_vp, C.c_char_p, _i64]),
+    "mh_gf_eval": (_int, [_vp, _int, _int, _int, _i64, _vp, _vp, _d, _vp]),
+    "mh_wave_term_eval": (_int, [_vp, _int, _i64, _vp, _vp, _vp]),
+    "mh_assemble": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
+    "mh_lu_factor": (_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "mh_lu_solve": (_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _int, _i64, _vp, _i64, _i64]),
+    "mh_solve": (_int, [_vp, _i64, _i64, _int, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
+    "mh_freq_sweep": (_int, [_vp, _vp, _int, _int, _int, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp,
+                             _vp, _vp, _vp]),
+    "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  _d, _d, _d, _vp]),
+    "mh_diffraction_force_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,
+                                         _d, _vp]),
+}
+
+
+class MeshT(C.Structure):
+    """mh_mesh_t"""
+    _fields_ = [("nfaces", _i64), ("nvertices", _i64), ("vertices", _vp), ("faces", _vp),
+                ("centers", _vp), ("normals", _vp), ("areas", _vp), ("radii", _vp)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise RuntimeError(
+                        f"libmhip.so not found at {LIB_PATH}; build it with "
+                        "`python -c 'import __graft_entry__ as g; g.build()'` or "
+                        "`make -C marinehydro.jl_amd/csrc`")
+                L = C.CDLL(LIB_PATH)
+                for name, (res, args) in _SIGS.items():
+                    f = getattr(L, name)
+                    f.restype = res
+                    f.argtypes = args
+                _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != MH_OK:
+        msg = lib().mh_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, RuntimeError)(f"{what}: {msg}" if what else msg)
+
+
+_contexts: dict = {}
+
+
+def context(device: int = 0):
+    """Library context bound to `device` and torch's current stream there."""
+    import torch
+
+    stream = torch.cuda.current_stream(device).cuda_stream
+    key = (device, stream)
+    ctx = _contexts.get(key)
+    if ctx is None:
+        h = _vp()
+        check(lib().mh_context_create(device, _vp(stream) if stream else None, C.byref(h)),
+              "mh_context_create")
+        _contexts[key] = h
+        ctx = h
+    return ctx
+
+
+def profile_enable(on=True, device=0):
+    check(lib().mh_profile_enable(context(device), int(bool(on))), "profile_enable")
+
+
+def profile_report(device=0) -> dict:
+    """Per-kernel launches, milliseconds, algorithmic flops and bytes since
+    the last report (kernel timer on the context's stream)."""
+    import json
+
+    buf = C.create_string_buffer(1 << 16)
+    check(lib().mh_profile_report(context(device), buf, len(buf)), "profile_report")
+    return json.loads(buf.value.decode())

@@ -1,0 +1,455 @@
+"""Host-side mirror of MarineHydro.jl's hot-path API over libmhip.so.
+
+Same names and argument meaning as the reference's exported functions
+(reference src/MarineHydro.jl:13-50):
+
+  SETTINGS, set_g, set_rho                 src/constants.jl:2-12
+  Rankine, RankineReflected, RankineReflectedNegative, GFWu,
+  ExactGuevelDelhommeau                    src/green_functions/*.jl
+  greens, gradient_greens, integral, integral_gradient
+                                           src/green_functions/abstract_greens_function.jl:16-83
+  element                                  src/meshes.jl:171-179 (0-based here)
+  assemble_matrices, assemble_matrix_wu    src/matrix_assembly.jl:4-42, :97-102
+  solve                                    src/matrix_assembly.jl:105-133
+  radiation_bc, AiryBC, airy_waves_velocity, airy_waves_potential,
+  airy_waves_pressure, integrate_pressure  src/waves.jl:4-25, :112-154
+  calculate_radiation_forces, DiffractionForce, diffraction_force
+                                           src/waves.jl:29-59
+
+Every numerical function on the N x N path runs in the HIP library; the O(N)
+helpers (boundary conditions, pressure integration) are numpy restatements
+offered for API completeness, and the batched sweep ``freq_sweep`` runs them
+on the GPU.  Matrices come back as torch complex128 CUDA tensors whose memory
+is column-major (the Julia layout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .meshes import Mesh
+
+# ---------------------------------------------------------------------------
+# settings (src/constants.jl:2-12)
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class _Constants:
+    rho: float = 1000.0  # fluid density [kg/m^3]
+    g: float = 9.81      # gravity [m/s^2]
+
+
+SETTINGS = _Constants()
+
+
+def set_g(val):
+    SETTINGS.g = float(val)
+
+
+def set_rho(val):
+    SETTINGS.rho = float(val)
+
+
+# ---------------------------------------------------------------------------
+# Green-function terms
+# ---------------------------------------------------------------------------
+class GreensFunction:
+    mask = 0
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self))
+
+
+class Rankine(GreensFunction):
+    """src/green_functions/rankine.jl"""
+    mask = L.GF_RANKINE
+
+
+class RankineReflected(GreensFunction):
+    """src/green_functions/rankine_reflected.jl"""
+    mask = L.GF_RANKINE_REFLECTED
+
+
+class RankineReflectedNegative(GreensFunction):
+    """src/green_functions/rankine_reflected_negative.jl"""
+    mask = L.GF_RANKINE_REFLECTED_NEG
+
+
+class GFWu(GreensFunction):
+    """src/green_functions/wu.jl"""
+    mask = L.GF_WU
+
+
+class ExactGuevelDelhommeau(GreensFunction):
+    """src/green_functions/exact_Guevel_Delhommeau.jl"""
+    mask = L.GF_DELHOMMEAU
+
+
+def gf_mask(gfs) -> int:
+    """Bit mask of a GreensFunction or an iterable of them (the reference
+    sums over tuples/vectors, abstract_greens_function.jl:18-83)."""
+    if isinstance(gfs, GreensFunction):
+        return gfs.mask
+    m = 0
+    for g in gfs:
+        if not isinstance(g, GreensFunction):
+            raise TypeError(f"not a GreensFunction: {g!r}")
+        if m & g.mask:
+            raise ValueError(f"{g!r} appears twice in the Green-function tuple")
+        m |= g.mask
+    if (m & L.GF_WU) and (m & L.GF_DELHOMMEAU):
+        raise ValueError("choose one wave term (GFWu or ExactGuevelDelhommeau)")
+    if m == 0:
+        raise ValueError("empty Green-function tuple")
+    return m
+
+
+# ---------------------------------------------------------------------------
+# elements (src/meshes.jl:157-205)
+# ---------------------------------------------------------------------------
+@dataclass
+class Element:
+    center: np.ndarray
+    vertices: np.ndarray = None  # 4 x 3
+    normal: np.ndarray = None
+    area: float = 1.0
+    radius: float = 0.0
+
+
+def element(mesh: Mesh, j: int) -> Element:
+    """Panel j of mesh (0-based)."""
+    return Element(center=np.array(mesh.centers[j]), vertices=np.array(mesh.vertices[mesh.faces[j]]),
+                   normal=np.array(mesh.normals[j]), area=float(mesh.areas[j]),
+                   radius=float(mesh.radii[j]))
+
+
+def _field(e, name, default=None):
+    if isinstance(e, dict):
+        return e.get(name, default)
+    return getattr(e, name, default)
+
+
+def _panel_record(e) -> np.ndarray:
+    """mh_panel_t: center[3], vertices[4][3], normal[3], area, radius."""
+    rec = np.zeros(20)
+    rec[0:3] = np.asarray(_field(e, "center"), dtype=np.float64).reshape(3)
+    v = _field(e, "vertices")
+    if v is not None:
+        rec[3:15] = np.asarray(v, dtype=np.float64).reshape(4, 3).reshape(-1)
+    n = _field(e, "normal")
+    if n is not None:
+        rec[15:18] = np.asarray(n, dtype=np.float64).reshape(3)
+    a = _field(e, "area")
+    rec[18] = 1.0 if a is None else float(a)
+    r = _field(e, "radius")
+    rec[19] = 0.0 if r is None else float(r)
+    return rec
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("marinehydro_amd needs a ROCm GPU (gfx950); none is visible")
+    return torch
+
+
+def gf_eval(gfs, points, panels, wavenumber=1.0, op=L.OP_INTEGRAL, wrt_first=False, device=0):
+    """Batched per-pair evaluation on the GPU.
+
+    points: (npairs, 3); panels: (npairs, 20) mh_panel_t records.
+    Returns complex (npairs,) for value ops, (npairs, 3) for gradient ops.
+    """
+    torch = _torch()
+    dev = f"cuda:{device}"
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    pan = np.ascontiguousarray(np.asarray(panels, dtype=np.float64).reshape(-1, 20))
+    npairs = pts.shape[0]
+    if pan.shape[0] != npairs:
+        raise ValueError("points and panels must have the same length")
+    d_pts = torch.from_numpy(np.ascontiguousarray(pts.T)).to(dev)
+    d_pan = torch.from_numpy(pan).to(dev)
+    grad = op in (L.OP_GRADIENT_GREENS, L.OP_INTEGRAL_GRADIENT)
+    out = torch.empty((3 if grad else 1, npairs), dtype=torch.complex128, device=dev)
+    ctx = L.context(device)
+    L.check(L.lib().mh_gf_eval(ctx, gf_mask(gfs), op, int(bool(wrt_first)), npairs,
+                               d_pts.data_ptr(), d_pan.data_ptr(), float(wavenumber),
+                               out.data_ptr()), "gf_eval")
+    res = out.cpu().numpy()
+    return res.T if grad else res[0]
+
+
+def _k(w):
+    return 1.0 if w is None else float(w)
+
+
+def greens(gf, element_1, element_2, wavenumber=None):
+    """abstract_greens_function.jl:5-16; point value between two centres."""
+    rec = _panel_record(element_2)
+    return gf_eval(gf, [_field(element_1, "center")], [rec], _k(wavenumber), L.OP_GREENS)[0]
+
+
+def gradient_greens(gf, element_1, element_2, wavenumber=None, with_respect_to_first_variable=False):
+    """abstract_greens_function.jl:19-32."""
+    rec = _panel_record(element_2)
+    g = gf_eval(gf, [_field(element_1, "center")], [rec], _k(wavenumber), L.OP_GRADIENT_GREENS,
+                with_respect_to_first_variable)[0]
+    return g
+
+
+def integral(gf, element_1, element_2, wavenumber=None):
+    """abstract_greens_function.jl:35-48; integral over panel element_2."""
+    rec = _panel_record(element_2)
+    return gf_eval(gf, [_field(element_1, "center")], [rec], _k(wavenumber), L.OP_INTEGRAL)[0]
+
+
+def integral_gradient(gf, element_1, element_2, wavenumber=None, with_respect_to_first_variable=False):
+    """abstract_greens_function.jl:51-64."""
+    rec = _panel_record(element_2)
+    return gf_eval(gf, [_field(element_1, "center")], [rec], _k(wavenumber), L.OP_INTEGRAL_GRADIENT,
+                   with_respect_to_first_variable)[0]
+
+
+def both_integral_and_integral_gradient(gf, element_1, element_2, wavenumber=None,
+                                        with_respect_to_first_variable=False):
+    """abstract_greens_function.jl:67-80."""
+    return (integral(gf, element_1, element_2, wavenumber),
+            integral_gradient(gf, element_1, element_2, wavenumber, with_respect_to_first_variable))
+
+
+# ---------------------------------------------------------------------------
+# device mesh
+# ---------------------------------------------------------------------------
+class DeviceMesh:
+    """A Mesh copied to device memory in the reference's column-major layout."""
+
+    def __init__(self, mesh: Mesh, device=0):
+        torch = _torch()
+        self.mesh = mesh
+        self.device = device
+        dev = f"cuda:{device}"
+
+        def cm(a, dt):  # column-major (n, c) == row-major (c, n)
+            return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=dt).T)).to(dev)
+
+        self.vertices = cm(mesh.vertices, np.float64)
+        self.faces = cm(mesh.faces, np.int64)
+        self.centers = cm(mesh.centers, np.float64)
+        self.normals = cm(mesh.normals, np.float64)
+        self.areas = torch.from_numpy(np.ascontiguousarray(mesh.areas)).to(dev)
+        self.radii = torch.from_numpy(np.ascontiguousarray(mesh.radii)).to(dev)
+        self.struct = L.MeshT(mesh.nfaces, mesh.nvertices, self.vertices.data_ptr(),
+                              self.faces.data_ptr(), self.centers.data_ptr(),
+                              self.normals.data_ptr(), self.areas.data_ptr(),
+                              self.radii.data_ptr())
+
+    @property
+    def nfaces(self):
+        return self.mesh.nfaces
+
+
+def _dmesh(mesh, device=0) -> DeviceMesh:
+    return mesh if isinstance(mesh, DeviceMesh) else DeviceMesh(mesh, device)
+
+
+def _colmajor_empty(torch, n, batch, device):
+    """(batch, n, n) complex128 whose [b].T views are column-major matrices."""
+    return torch.empty((batch, n, n), dtype=torch.complex128, device=f"cuda:{device}")
+
+
+# ---------------------------------------------------------------------------
+# assembly (src/matrix_assembly.jl)
+# ---------------------------------------------------------------------------
+def assemble_matrices_batched(green_functions, mesh, wavenumbers, direct=True, device=0,
+                              formulation=None):
+    """Assemble for many wavenumbers in one launch.
+
+    Returns (S, DK) as (nk, n, n) tensors in logical [i, j] indexing (memory
+    column-major); formulation "both" returns (S, D, K).
+    """
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    n = dm.nfaces
+    ks = np.ascontiguousarray(np.asarray(wavenumbers, dtype=np.float64).reshape(-1))
+    nk = ks.shape[0]
+    form = formulation or (L.DIRECT if direct else L.INDIRECT)
+    S = _colmajor_empty(torch, n, nk, dm.device)
+    D = _colmajor_empty(torch, n, nk, dm.device) if form & L.DIRECT else None
+    K = _colmajor_empty(torch, n, nk, dm.device) if form & L.INDIRECT else None
+    ctx = L.context(dm.device)
+    L.check(L.lib().mh_assemble(ctx, C.byref(dm.struct), gf_mask(green_functions), form, nk,
+                                ks.ctypes.data, S.data_ptr(), D.data_ptr() if D is not None else None,
+                                K.data_ptr() if K is not None else None, n, n * n),
+            "assemble_matrices")
+    tr = lambda X: X.transpose(1, 2)
+    if form == L.BOTH:
+        return tr(S), tr(D), tr(K)
+    return tr(S), tr(D if D is not None else K)
+
+
+def assemble_matrices(green_functions, mesh, wavenumber, direct=True, device=0):
+    """src/matrix_assembly.jl:4-42 (the default `assemble_matrices`, :97).
+
+    Returns (S, D) when direct, else (S, K); n x n complex128 CUDA tensors.
+    """
+    S, DK = assemble_matrices_batched(green_functions, mesh, [wavenumber], direct, device)
+    return S[0], DK[0]
+
+
+def assemble_matrix_wu(mesh, wavenumber, direct=True, device=0):
+    """src/matrix_assembly.jl:100-102."""
+    return assemble_matrices((Rankine(), RankineReflected(), GFWu()), mesh, wavenumber, direct,
+                             device)
+
+
+def _as_colmajor_batch(torch, X, device):
+    """Copy an (n, n) or (b, n, n) matrix (torch or numpy, logical [i, j])
+    into a fresh (b, n, n) buffer whose memory is column-major per item."""
+    if not torch.is_tensor(X):
+        X = torch.from_numpy(np.asarray(X, dtype=np.complex128))
+    X = X.to(device=f"cuda:{device}", dtype=torch.complex128)
+    if X.dim() == 2:
+        X = X.unsqueeze(0)
+    # memory [b][j][i] = X[b, i, j]
+    return X.transpose(1, 2).contiguous()
+
+
+def solve(D, S, bc, direct=True, device=0):
+    """src/matrix_assembly.jl:105-133.  D (or K) is not modified."""
+    torch = _torch()
+    Dm = _as_colmajor_batch(torch, D, device)
+    Sm = _as_colmajor_batch(torch, S, device)
+    b = bc if torch.is_tensor(bc) else torch.from_numpy(np.asarray(bc, dtype=np.complex128))
+    b = b.to(device=f"cuda:{device}", dtype=torch.complex128)
+    vec = b.dim() == 1
+    if vec:
+        b = b.reshape(-1, 1)
+    n = Dm.shape[1]
+    if Dm.shape[0] != 1 or Sm.shape != Dm.shape or b.shape[0] != n:
+        raise ValueError("solve: D, S must be n x n and bc of length n")
+    nrhs = b.shape[1]
+    bm = b.transpose(0, 1).contiguous()  # memory column-major n x nrhs
+    phi = torch.empty_like(bm)
+    ctx = L.context(device)
+    L.check(L.lib().mh_solve(ctx, n, 1, int(bool(direct)), Dm.data_ptr(), Sm.data_ptr(), n, n * n,
+                             bm.data_ptr(), nrhs, phi.data_ptr(), n, n * nrhs), "solve")
+    out = phi.transpose(0, 1)
+    return out[:, 0] if vec else out
+
+
+# ---------------------------------------------------------------------------
+# waves, boundary conditions, forces (src/waves.jl) - O(N) host helpers
+# ---------------------------------------------------------------------------
+def radiation_bc(mesh, dof, omega):
+    """src/waves.jl:4-19 and :89-102 (dof: 3-vector or nfaces x 3 field)."""
+    d = np.broadcast_to(np.asarray(dof, dtype=np.float64), mesh.normals.shape)
+    return -1j * omega * np.sum(mesh.normals * d, axis=1)
+
+
+def integrate_pressure(mesh, pressure, dof):
+    """src/waves.jl:21-25."""
+    d = np.broadcast_to(np.asarray(dof, dtype=np.float64), mesh.normals.shape)
+    amp = -np.sum(d * mesh.normals, axis=1)
+    return np.sum(np.asarray(pressure).reshape(-1) * amp * mesh.areas)
+
+
+def airy_waves_potential(points, omega, beta=0.0):
+    """src/waves.jl:112-119."""
+    P = np.asarray(points, dtype=np.float64)
+    k = omega ** 2 / SETTINGS.g
+    wbar = P[:, 0] * np.cos(beta) + P[:, 1] * np.sin(beta)
+    return -1j * SETTINGS.g / omega * np.exp(k * P[:, 2]) * np.exp(1j * k * wbar)
+
+
+def airy_waves_velocity(points, omega, beta=0.0):
+    """src/waves.jl:121-134 (infinite depth)."""
+    P = np.asarray(points, dtype=np.float64)
+    k = omega ** 2 / SETTINGS.g
+    wbar = P[:, 0] * np.cos(beta) + P[:, 1] * np.sin(beta)
+    cih = np.exp(k * P[:, 2])
+    pre = SETTINGS.g * k / omega * np.exp(1j * k * wbar)
+    return pre[:, None] * np.stack([np.cos(beta) * cih, np.sin(beta) * cih, -1j * cih], axis=1)
+
+
+def airy_waves_pressure(points, omega, beta=0.0):
+    """src/waves.jl:150-154."""
+    return 1j * omega * SETTINGS.rho * airy_waves_potential(points, omega, beta)
+
+
+def AiryBC(mesh, omega, beta=0.0):
+    """src/waves.jl:138-142."""
+    return -np.sum(airy_waves_velocity(mesh.centers, omega, beta) * mesh.normals, axis=1)
+
+
+def diffraction_force(potential, mesh, omega, dof):
+    """src/waves.jl:55-59."""
+    pressure = 1j * SETTINGS.rho * np.asarray(potential) * omega
+    return integrate_pressure(mesh, pressure, dof)
+
+
+def FroudeKrylovForce(mesh, omega, dof, beta=0.0):
+    """src/waves.jl:62-66."""
+    return integrate_pressure(mesh, airy_waves_pressure(mesh.centers, omega, beta), dof)
+
+
+# ---------------------------------------------------------------------------
+# batched end-to-end sweep and the reference's two physics entry points
+# ---------------------------------------------------------------------------
+def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=True,
+               precision="fp64", device=0, g=None, rho=None):
+    """All frequencies in one batched pass (one LU per frequency, shared by
+    every radiating dof and heading).
+
+    dof_fields: list of 3-vectors or (nfaces, 3) fields.
+    Returns numpy (A, B, F): A[w, i, j], B[w, i, j] = added mass / damping on
+    influenced dof i due to radiating dof j; F[w, b, i] = diffraction force
+    on dof i for heading b.
+    """
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    n = dm.nfaces
+    gfs = green_functions if green_functions is not None else (Rankine(), RankineReflected(), GFWu())
+    om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
+    be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
+    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+    ndof, nbeta, nw = len(fields), be.shape[0], om.shape[0]
+    dev = f"cuda:{dm.device}"
+    dofs = torch.from_numpy(np.ascontiguousarray(
+        np.stack([f.T for f in fields]) if ndof else np.zeros((1, 3, n)))).to(dev)
+    A = torch.zeros((nw, max(ndof, 1), max(ndof, 1)), dtype=torch.float64, device=dev)
+    B = torch.zeros_like(A)
+    F = torch.zeros((nw, max(nbeta, 1), max(ndof, 1)), dtype=torch.complex128, device=dev)
+    prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
+    ctx = L.context(dm.device)
+    L.check(L.lib().mh_freq_sweep(ctx, C.byref(dm.struct), gf_mask(gfs), int(bool(direct)), prec,
+                                  nw, om.ctypes.data, float(g or SETTINGS.g),
+                                  float(rho or SETTINGS.rho), ndof, dofs.data_ptr(), nbeta,
+                                  be.ctypes.data if nbeta else None, A.data_ptr(), B.data_ptr(),
+                                  F.data_ptr()), "freq_sweep")
+    # library layout A[w][j][i] -> A[w, i, j]
+    A_ = A.transpose(1, 2).cpu().numpy()[:, :ndof, :ndof]
+    B_ = B.transpose(1, 2).cpu().numpy()[:, :ndof, :ndof]
+    F_ = F.cpu().numpy()[:, :nbeta, :ndof]
+    return A_, B_, F_
+
+
+def calculate_radiation_forces(mesh, dof, omega):
+    """src/waves.jl:29-37: [A, B] for one dof 3-vector (Wu, direct)."""
+    A, B, _ = freq_sweep(mesh, [omega], [dof])
+    return [float(A[0, 0, 0]), float(B[0, 0, 0])]
+
+
+def DiffractionForce(mesh, omega, dof):
+    """src/waves.jl:40-52: complex diffraction force, heading 0 (Wu, direct)."""
+    _, _, F = freq_sweep(mesh, [omega], [dof], betas=[0.0])
+    return complex(F[0, 0, 0])

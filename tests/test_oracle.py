@@ -1,0 +1,337 @@
+"""Pins the FP64 CPU oracle (oracle/mh_oracle.c) against independent checks.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails it:
+  - the reference's own known answers (test/greens_function.jl,
+    test/consistency_with_analytical_solutions.jl),
+  - mpmath fixtures at 30 digits (tests/golden/make_golden.py),
+  - scipy special functions, numpy LAPACK,
+  - brute-force quadrature and closed forms of the Rankine panel integral,
+  - closed-surface identities of the assembled operators.
+"""
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "delhommeau_mpmath.json")
+REF = "/root/reference"
+
+
+# fixed element pair of the reference tests (test/greens_function.jl:5-13)
+E1 = np.array([0.0, 0.0, -1.0])
+E2C = np.array([1.0, 1.0, -2.0])
+E2V = np.array([[-0.5, -0.5, 0.0], [0.5, -0.5, 0.0], [0.5, 0.5, 0.0], [-0.5, 0.5, 0.0]]) + E2C
+E2N = np.array([0.0, 0.0, 1.0])
+
+
+def e2panel(O):
+    return O.make_panel(E2C, E2V, E2N, 1.0, math.sqrt(2) / 2)
+
+
+def reflect(x):
+    return np.array([x[0], x[1], -x[2]])
+
+
+# ---------------------------------------------------------------- E1, Delhommeau
+def test_exp_e1_matches_mpmath(oracle):
+    d = json.load(open(GOLD))
+    for row in d["exp_e1"]:
+        z = complex(*row["z"])
+        ref = complex(*row["expE1"])
+        got = oracle.exp_e1(z)
+        assert abs(got - ref) <= 2e-15 * abs(ref), (z, got, ref)
+
+
+def test_delhommeau_matches_mpmath(oracle):
+    d = json.load(open(GOLD))
+    for row in d["dimless"]:
+        G, dGdr, dGdz = oracle.delhommeau_dimless(row["r"], row["z"])
+        g_ref = complex(*row["G"])
+        dr_ref = complex(*row["dGdr"])
+        assert abs(G - g_ref) <= 1e-12 * max(1.0, abs(g_ref)), row
+        assert abs(dGdr - dr_ref) <= 1e-12 * max(1.0, abs(dr_ref)), row
+
+
+def test_delhommeau_reference_constants(oracle):
+    # test/greens_function.jl:81-83 (Capytaine 1000-point rule, good to ~1e-5)
+    G, dGdr, _ = oracle.delhommeau_dimless(1.0, -1.0)
+    assert abs(G - (-1.84003542 + 1.76871985j)) < 1e-4
+    assert abs(dGdr - (-0.59117747 - 1.01715699j)) < 1e-4
+    # test/greens_function.jl:88-92, k = 1
+    g = oracle.greens(oracle.DELHOMMEAU, E1, E2C, 1.0)
+    assert abs(g - (-0.9289959399878619 + 0.17490912138602516j)) < 1e-4
+    gg = oracle.gradient_greens(oracle.DELHOMMEAU, E1, E2C, 1.0, wrt_first=True)
+    ref = np.array([-0.06819068 + 0.12043414j, -0.06819068 + 0.12043414j, -0.32597325 + 0.17490912j])
+    assert np.max(np.abs(gg - ref)) < 1e-4
+
+
+def test_delhommeau_z_derivative_identity(oracle):
+    # dG/dz = G + 2/hypot(r, z) (exact_Guevel_Delhommeau.jl:56) against a
+    # central difference of G in z: pins G's z-dependence.
+    for r, z in [(0.5, -0.7), (3.0, -0.2), (10.0, -2.0)]:
+        h = 1e-5
+        gp = oracle.delhommeau_dimless(r, z + h)[0]
+        gm = oracle.delhommeau_dimless(r, z - h)[0]
+        fd = (gp - gm) / (2 * h)
+        _, _, dz = oracle.delhommeau_dimless(r, z)
+        assert abs(fd - dz) < 1e-7 * max(1.0, abs(dz))
+        # and dG/dr against a central difference in r
+        gp = oracle.delhommeau_dimless(r + h, z)[0]
+        gm = oracle.delhommeau_dimless(r - h, z)[0]
+        _, dr, _ = oracle.delhommeau_dimless(r, z)
+        assert abs((gp - gm) / (2 * h) - dr) < 1e-7 * max(1.0, abs(dr))
+
+
+# ---------------------------------------------------------------- Rankine
+def test_rankine_reference_closed_forms(oracle):
+    # test/greens_function.jl:38-47
+    O = oracle
+    s3 = math.sqrt(3.0)
+    assert abs(O.greens(O.RANKINE, E1, E2C) - 1 / s3) < 1e-8
+    g1 = O.gradient_greens(O.RANKINE, E1, E2C, wrt_first=True)
+    assert np.allclose(g1, (E2C - E1) / s3 ** 3, atol=1e-8)
+    g2 = O.gradient_greens(O.RANKINE, E1, E2C, wrt_first=False)
+    assert np.allclose(g2, (E1 - E2C) / s3 ** 3, atol=1e-8)
+    p = e2panel(O)
+    assert abs(O.integral(O.RANKINE, E1, p) - 1 / s3) < 1e-1
+    assert np.allclose(O.integral_gradient(O.RANKINE, E1, p, wrt_first=True), (E2C - E1) / s3 ** 3,
+                       atol=1e-1)
+    assert np.allclose(O.integral_gradient(O.RANKINE, E1, p, wrt_first=False), (E1 - E2C) / s3 ** 3,
+                       atol=1e-1)
+
+
+def test_reflected_reference_closed_forms(oracle):
+    # test/greens_function.jl:55-73
+    O = oracle
+    s11 = math.sqrt(11.0)
+    assert abs(O.greens(O.RANKINE_REFLECTED, E1, E2C) - 1 / s11) < 1e-8
+    g = O.gradient_greens(O.RANKINE_REFLECTED, E1, E2C, wrt_first=True)
+    assert np.allclose(g, (reflect(E2C) - E1) / s11 ** 3, atol=1e-8)
+    g = O.gradient_greens(O.RANKINE_REFLECTED, E1, E2C, wrt_first=False)
+    assert np.allclose(g, reflect(E1 - reflect(E2C)) / s11 ** 3, atol=1e-8)
+    p = e2panel(O)
+    assert abs(O.integral(O.RANKINE_REFLECTED, E1, p) - 1 / s11) < 1e-1
+    # alternative definition: reflecting the panel instead of the point
+    pr = O.make_panel(reflect(E2C), np.array([reflect(v) for v in E2V[::-1]]), reflect(E2N), 1.0,
+                      math.sqrt(2) / 2)
+    assert abs(O.rankine_integral(reflect(E1), p) - O.rankine_integral(E1, pr)) < 1e-12
+    # negative image is a sign flip (rankine_reflected_negative.jl)
+    assert abs(O.integral(O.RANKINE_REFLECTED_NEG, E1, p) + O.integral(O.RANKINE_REFLECTED, E1, p)) == 0
+
+
+def _quad_integral(x, V, nq=80, sub=8, grad=False):
+    """Brute-force integral of 1/|x - xi| (or its xi-gradient) over a planar
+    quad by composite Gauss-Legendre on the bilinear map."""
+    t, w = np.polynomial.legendre.leggauss(nq)
+    u = np.concatenate([(t + 1) / (2 * sub) + k / sub for k in range(sub)])
+    wu = np.concatenate([w / (2 * sub)] * sub)
+    U, W = np.meshgrid(u, u, indexing="ij")
+    WW = np.outer(wu, wu)
+    P = ((1 - U)[..., None] * (1 - W)[..., None] * V[0] + U[..., None] * (1 - W)[..., None] * V[1] +
+         U[..., None] * W[..., None] * V[2] + (1 - U)[..., None] * W[..., None] * V[3])
+    dPu = (1 - W)[..., None] * (V[1] - V[0]) + W[..., None] * (V[2] - V[3])
+    dPw = (1 - U)[..., None] * (V[3] - V[0]) + U[..., None] * (V[2] - V[1])
+    J = np.linalg.norm(np.cross(dPu, dPw), axis=-1)
+    d = x - P
+    r = np.linalg.norm(d, axis=-1)
+    if grad:
+        return np.array([np.sum(WW * J * d[..., c] / r ** 3) for c in range(3)])
+    return np.sum(WW * J / r)
+
+
+def test_rankine_integral_vs_quadrature(oracle):
+    O = oracle
+    V = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [1.2, 0.9, 0.0], [-0.1, 1.0, 0.0]])
+    ctr = V.mean(axis=0)
+    n = np.array([0.0, 0.0, 1.0])
+    radius = np.max(np.linalg.norm(V - ctr, axis=1))
+    area = 0.5 * np.linalg.norm(np.cross(V[2] - V[0], V[3] - V[1]))
+    p = O.make_panel(ctr, V, n, area, radius)
+    for x in [ctr + [0.2, 0.1, 0.6], ctr + [1.5, -0.3, 0.25], ctr + [0.1, 0.0, -0.15],
+              ctr + [2.0, 2.0, 1.0], ctr + [-0.4, 0.3, 0.05]]:
+        ref = _quad_integral(x, V, nq=60, sub=12)
+        got = O.rankine_integral(x, p)
+        assert abs(got - ref) < 1e-9 * abs(ref), (x, got, ref)
+        gref = _quad_integral(x, V, nq=60, sub=12, grad=True)
+        g = O.rankine_integral_gradient(x, p, wrt_first=False)
+        assert np.max(np.abs(g - gref)) < 1e-7 * np.max(np.abs(gref)), (x, g, gref)
+
+
+def test_rankine_self_term_square(oracle):
+    # integral of 1/r over a square of side a from its centre = 4 a ln(1+sqrt 2)
+    O = oracle
+    for a in [0.3, 1.0, 2.5]:
+        V = a * np.array([[-0.5, -0.5, 0], [0.5, -0.5, 0], [0.5, 0.5, 0], [-0.5, 0.5, 0]])
+        p = O.make_panel([0, 0, 0], V, [0, 0, 1], a * a, a / math.sqrt(2))
+        assert abs(O.rankine_integral([0.0, 0.0, 0.0], p) - 4 * a * math.log(1 + math.sqrt(2))) < 1e-13 * a
+
+
+def test_rankine_gradient_vs_finite_difference(oracle):
+    # near-field gradient formula (rankine.jl:65-123) against a central
+    # difference of the value formula (rankine.jl:24-63): two separate code paths
+    O = oracle
+    V = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.2], [1.0, 1.0, 0.2], [0.0, 1.0, 0.0]])
+    ctr = V.mean(axis=0)
+    nrm = np.cross(V[2] - V[0], V[3] - V[1])
+    nrm /= np.linalg.norm(nrm)
+    p = O.make_panel(ctr, V, nrm, 1.02, np.max(np.linalg.norm(V - ctr, axis=1)))
+    for x in [ctr + [0.3, 0.2, 0.5], ctr + [1.2, 0.4, -0.3], ctr + [0.1, -0.2, 0.3]]:
+        g = O.rankine_integral_gradient(x, p, wrt_first=True)
+        h = 1e-6
+        fd = np.array([(O.rankine_integral(x + h * e, p) - O.rankine_integral(x - h * e, p)) / (2 * h)
+                       for e in np.eye(3)])
+        assert np.max(np.abs(g - fd)) < 1e-7, (x, g, fd)
+
+
+# ---------------------------------------------------------------- Wu
+def test_struve_approximations_vs_scipy(oracle):
+    sp = pytest.importorskip("scipy.special")
+    xs = np.linspace(0.0, 40.0, 4001)
+    e0 = max(abs(oracle.struve_h0(x) - sp.struve(0, x)) for x in xs)
+    e1 = max(abs(oracle.struve_h1(x) - sp.struve(1, x)) for x in xs)
+    # Newman's rational approximations are good to a few 1e-8
+    assert e0 < 3e-8 and e1 < 5e-8, (e0, e1)
+
+
+def test_wu_vs_exact_delhommeau_reference(oracle):
+    # test/greens_function.jl:96-99, atol 2e-2 for k in {0.1, 1, 10}
+    O = oracle
+    for k in [0.1, 1.0, 10.0]:
+        gw = O.greens(O.WU, E1, E2C, k)
+        gd = O.greens(O.DELHOMMEAU, E1, E2C, k)
+        assert abs(gw - gd) < 2e-2
+        dw = O.gradient_greens(O.WU, E1, E2C, k)
+        dd = O.gradient_greens(O.DELHOMMEAU, E1, E2C, k)
+        assert np.max(np.abs(dw - dd)) < 2e-2
+
+
+def test_wu_fit_accuracy_over_grid(oracle):
+    # Wu's fit against the exact term over the domain the assembly samples.
+    # Imaginary parts are exact in both (2 pi J0 e^z); the real part differs
+    # only by the fit error, so a wrong coefficient or sign shows up here.
+    O = oracle
+    worst_im, worst_re = 0.0, 0.0
+    for r in [0.0, 0.05, 0.4, 1.0, 2.0, 3.5, 6.0, 12.0]:
+        for z in [-0.05, -0.3, -1.0, -2.5, -6.0]:
+            gw, dwr, dwz = O.wu_dimless(r, z)
+            gd, ddr, ddz = O.delhommeau_dimless(r, z)
+            worst_im = max(worst_im, abs(gw.imag - gd.imag), abs(dwr.imag - ddr.imag))
+            worst_re = max(worst_re, abs(gw.real - gd.real) / max(1.0, abs(gd.real)),
+                           abs(dwr.real - ddr.real) / max(1.0, abs(ddr.real)))
+    assert worst_im < 1e-7, worst_im
+    assert worst_re < 3e-2, worst_re  # the fit is good to about 2e-2 near d -> 0
+
+
+def _wu_tables_from_reference():
+    path = os.path.join(REF, "src", "green_functions", "wu.jl")
+    if not os.path.exists(path):
+        pytest.skip("reference sources not present")
+    txt = open(path).read()
+    tables = {}
+    for fname in ["GF_dFuncC", "GF_dFuncB", "GF_dFuncA", "GF_FuncD", "GF_FuncC", "GF_FuncB",
+                  "GF_FuncA"]:
+        body = re.search(r"function %s\(tt\)(.*?)\n\s*%s = " % (fname, fname), txt, re.S).group(1)
+        vals = re.findall(r"^\s*[A-D](\d)\s*=\s*([-+0-9.eE]+)", body, re.M)
+        tables[fname] = [float(v) for _, v in sorted(vals, key=lambda t: int(t[0]))]
+    return tables
+
+
+def test_wu_coefficients_match_reference_text(oracle):
+    """The oracle's fit tables equal the reference's (read as text)."""
+    tabs = _wu_tables_from_reference()
+    src = open(os.path.join(HERE, "..", "oracle", "mh_oracle.c")).read()
+    names = {"GF_dFuncC": "WU_dC", "GF_dFuncB": "WU_dB", "GF_dFuncA": "WU_dA", "GF_FuncD": "WU_D",
+             "GF_FuncC": "WU_C", "GF_FuncB": "WU_B", "GF_FuncA": "WU_A"}
+    for fname, cname in names.items():
+        body = re.search(r"%s\[\d+\] = \{(.*?)\}" % cname, src, re.S).group(1)
+        vals = [float(v) for v in re.findall(r"[-+]?\d+\.?\d*(?:[eE][-+]?\d+)?", body)]
+        assert vals == tabs[fname], (fname, vals, tabs[fname])
+
+
+# ---------------------------------------------------------------- solve
+def test_lu_against_lapack(oracle):
+    rng = np.random.default_rng(3)
+    n = 67
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    LU, piv, info = oracle.zgetrf(A)
+    assert info == 0
+    X = oracle.zgetrs(LU, piv, B)
+    assert np.allclose(X, np.linalg.solve(A, B), rtol=1e-11, atol=1e-12)
+    XT = oracle.zgetrs(LU, piv, B, trans=True)
+    assert np.allclose(XT, np.linalg.solve(A.T, B), rtol=1e-11, atol=1e-12)
+    # singular: exact zero column
+    A2 = A.copy()
+    A2[:, 5] = 0
+    _, _, info = oracle.zgetrf(A2)
+    assert info == 6
+
+
+# ---------------------------------------------------------------- assembly
+def test_closed_sphere_operator_identities(oracle, mh):
+    # Rankine single layer of a unit density on a sphere of radius R is R at
+    # the surface (S row sums -> -R); the double layer is the solid angle
+    # (D row sums -> 1, exact up to rounding for flat panels).
+    O = oracle
+    m = mh.sphere(1.0, ntheta=12, nphi=12).mesh
+    S, D = O.assemble(O.RANKINE, m, 1.0, True)
+    S2, K = O.assemble(O.RANKINE, m, 1.0, False)
+    assert np.array_equal(S, S2)  # S(direct) == S(indirect), test/matrix_assembly.jl:21
+    assert not np.allclose(D, K)  # test/matrix_assembly.jl:22
+    assert np.all(np.abs(S.sum(1).real + 1.0) < 3e-2)
+    assert np.all(np.abs(D.sum(1).real - 1.0) < 1e-3)
+    assert np.all(np.abs(K.sum(1).real - 1.0) < 1e-1)
+    # a floating hemisphere with its free-surface image closes the sphere
+    h = mh.hemisphere(1.0, ntheta=12, nphi=12).mesh
+    Sh, Dh = O.assemble(O.RANKINE | O.RANKINE_REFLECTED, h, 1.0, True)
+    assert np.all(np.abs(Dh.sum(1).real - 1.0) < 1e-3)
+    assert np.all(np.abs(Sh.sum(1).real + 1.0) < 3e-2)
+
+
+def _isapprox(a, b, rtol):
+    # Julia's isapprox on arrays: norm(a-b) <= rtol * max(norm(a), norm(b))
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_hulme_hemisphere_heave(oracle, mh):
+    # test/consistency_with_analytical_solutions.jl:169-189 (Wu, rho 1025)
+    K = [0.05, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.2, 1.4, 1.6, 1.8, 2.0, 2.5, 3.0,
+         4.0, 5.0, 6.0, 7.0, 8.0, 9.0, 10.0]
+    A_h = [0.8764, 0.8627, 0.7938, 0.7157, 0.6452, 0.5861, 0.5381, 0.4999, 0.4698, 0.4464, 0.4284,
+           0.4047, 0.3924, 0.3871, 0.3864, 0.3884, 0.3988, 0.4111, 0.4322, 0.4471, 0.4574, 0.4647,
+           0.4700, 0.4740, 0.4771]
+    B_h = [0.1036, 0.1816, 0.2793, 0.3254, 0.3410, 0.3391, 0.3271, 0.3098, 0.2899, 0.2691, 0.2484,
+           0.2096, 0.1756, 0.1469, 0.1229, 0.1031, 0.0674, 0.0452, 0.0219, 0.0116, 0.0066, 0.0040,
+           0.0026, 0.0017, 0.0012]
+    rho = 1025.0
+    m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
+    O = oracle
+    A_b, B_b = [], []
+    for k in K:
+        w = math.sqrt(k * 9.8)
+        A, B, _ = O.freq_solve(O.RANKINE | O.RANKINE_REFLECTED | O.WU, m, w, [[0, 0, 1.0]], rho=rho)
+        A_b.append(A[0, 0] / ((2 / 3) * math.pi * rho))
+        B_b.append(B[0, 0] / ((2 / 3) * math.pi * rho) / w)
+    assert _isapprox(A_b, A_h, 0.1)
+    assert _isapprox(B_b, B_h, 0.1)
+
+
+def test_direct_and_indirect_converge(oracle, mh):
+    # two discretisations of the same problem: their gap shrinks under mesh
+    # refinement (it is discretisation error, not a formulation mistake)
+    O = oracle
+    gf = O.RANKINE | O.RANKINE_REFLECTED | O.WU
+    gaps = []
+    for res in (10, 20):
+        m = mh.hemisphere(1.0, ntheta=res, nphi=res).mesh
+        Ad, Bd, Fd = O.freq_solve(gf, m, 1.2, [[0, 0, 1.0]], betas=[0.0], direct=True)
+        Ai, Bi, Fi = O.freq_solve(gf, m, 1.2, [[0, 0, 1.0]], betas=[0.0], direct=False)
+        gaps.append(max(abs(Ad[0, 0] - Ai[0, 0]) / abs(Ad[0, 0]), abs(Bd[0, 0] - Bi[0, 0]) / abs(Bd[0, 0]),
+                        abs(Fd[0, 0] - Fi[0, 0]) / abs(Fd[0, 0])))
+    assert gaps[1] < 0.75 * gaps[0] and gaps[1] < 0.05, gaps
