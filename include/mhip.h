@@ -110,7 +110,7 @@ int mh_context_destroy(mh_context_t ctx);
 int mh_synchronize(mh_context_t ctx);
 
 /* Kernel timer (no reference counterpart; the reference is profiled with
- * BenchmarkTools, benchmark/*.jl).  When enabled, every kernel the context
+ * BenchmarkTools scripts under benchmark/).  When enabled, every kernel the context
  * launches is bracketed by HIP events on the context's stream.
  * mh_profile_report synchronises, writes a JSON object
  * {"kernel": {"launches": n, "ms": t, "flops": f, "bytes": b}, ...} with the

@@ -357,6 +357,8 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
     const int64_t n = mesh->nfaces;
     const int64_t nr = ndof + nbeta;
     if (n == 0 || nomega == 0 || nr == 0) return MH_OK;
+    MH_REQUIRE(n <= 4096, "freq_sweep: at most 4096 panels per body");
+    MH_REQUIRE(nr <= 16, "freq_sweep: at most 16 dofs + headings");
     MH_REQUIRE(omegas, "freq_sweep: null omegas");
     MH_REQUIRE(ndof == 0 || (dof_fields && A && B), "freq_sweep: null dof/output array");
     MH_REQUIRE(nbeta == 0 || (betas && F), "freq_sweep: null heading/output array");
@@ -364,16 +366,20 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
         MH_REQUIRE(omegas[w] > 0.0, "freq_sweep: frequencies must be positive");
     Context* c = CTX(ctx);
     // frequencies are processed in chunks that keep S and D/K under 32 GiB
-    const size_t mat_bytes = sizeof(double2) * (size_t)n * (size_t)n;
-    int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_bytes));
+    // Per frequency: S and [D | S bc] (direct) or [K | bc] (indirect), n x
+    // (n + nr) column-major; the right-hand sides ride along as extra columns
+    // of the factorisation, which leaves them forward-substituted.
+    const int64_t ncols = n + nr;
+    const int64_t ld = n, stride = n * ncols, ldb = n, bstride = n * nr;
+    const size_t mat_aug = sizeof(double2) * (size_t)n * (size_t)ncols;
+    int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_aug));
     if (chunk < 1) chunk = 1;
     if (chunk > nomega) chunk = nomega;
-    const int64_t ld = n, stride = n * n, ldb = n, bstride = n * nr;
     Bump bump;
-    const size_t o_S = bump.take(mat_bytes * chunk);
-    const size_t o_D = bump.take(mat_bytes * chunk);
+    const size_t o_S = bump.take(mat_aug * chunk);
+    const size_t o_D = bump.take(mat_aug * chunk);
     const size_t o_rhs = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
-    const size_t o_tmp = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
+    const size_t o_phi = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
     const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * chunk));
     const size_t o_info = bump.take(sizeof(int) * (size_t)chunk);
     const size_t o_om = bump.take(sizeof(double) * (size_t)nomega);
@@ -385,7 +391,7 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
     double2* dS = reinterpret_cast<double2*>(ws + o_S);
     double2* dD = reinterpret_cast<double2*>(ws + o_D);
     double2* rhs = reinterpret_cast<double2*>(ws + o_rhs);
-    double2* tmp = reinterpret_cast<double2*>(ws + o_tmp);
+    double2* phi = reinterpret_cast<double2*>(ws + o_phi);
     int64_t* piv = reinterpret_cast<int64_t*>(ws + o_piv);
     int* info = reinterpret_cast<int*>(ws + o_info);
     double* d_om = reinterpret_cast<double*>(ws + o_om);
@@ -400,13 +406,30 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
         MH_HIP_CHECK(hipMemcpyAsync(d_be, betas, sizeof(double) * nbeta, hipMemcpyHostToDevice, c->stream));
     TRY(launch_pack_panels(c, mesh, P));
     const int form = direct ? MH_DIRECT : MH_INDIRECT;
+    double2* aug = dD + n * ld;  // first right-hand-side column of item 0
     for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
         const int64_t nw = (nomega - w0) < chunk ? (nomega - w0) : chunk;
         TRY(launch_assemble(c, mesh, P, gf_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
                             direct ? nullptr : dD, ld, stride));
         TRY(launch_bcs(c, mesh, nw, d_om + w0, g, ndof, dof_fields, nbeta, d_be, rhs, ldb, bstride));
-        TRY(solve_impl(c, n, nw, direct, dD, dS, ld, stride, rhs, nr, ldb, bstride, tmp, piv, info));
-        TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, rhs, ldb, bstride,
+        if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
+            TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
+        } else {       // sigma = K \ bc; phi = S sigma   (:127-131)
+            MH_HIP_CHECK(hipMemcpy2DAsync(aug, sizeof(double2) * stride, rhs, sizeof(double2) * bstride,
+                                          sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
+                                          c->stream));
+        }
+        MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
+        TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
+        TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 0, 2, nr, aug, ld, stride));
+        if (direct) {
+            MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
+                                          sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
+                                          c->stream));
+        } else {
+            TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, aug, ld, stride, phi, ldb, bstride));
+        }
+        TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, phi, ldb, bstride,
                           A ? A + w0 * ndof * ndof : nullptr, B ? B + w0 * ndof * ndof : nullptr,
                           F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr));
         TRY(check_info(c, info, nw));

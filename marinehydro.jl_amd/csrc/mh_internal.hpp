@@ -64,6 +64,12 @@ bool ensure_workspace(Context* ctx, size_t bytes);
         }                                                                    \
     } while (0)
 
+#define TRY_RC(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != MH_OK) return rc_; \
+    } while (0)
+
 #define MH_REQUIRE(cond, msg)                                                \
     do {                                                                     \
         if (!(cond)) {                                                       \
@@ -90,6 +96,14 @@ int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A,
 int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
              int64_t stride, const int64_t* ipiv, int trans, int64_t nrhs, double2* B,
              int64_t ldb, int64_t bstride);
+// part: 1 = forward (pivots + unit L), 2 = back (U), 3 = both
+int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
+                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
+                  double2* B, int64_t ldb, int64_t bstride);
+// Y[b] = S[b]^T * X[b]
+int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
+                     int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
+                     double2* Y, int64_t ldy, int64_t ystride);
 // Y[b] = S[b] * X[b]  (n x n times n x nrhs, complex)
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,

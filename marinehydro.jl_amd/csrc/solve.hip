@@ -1,25 +1,32 @@
 // Batched dense complex LU (partial pivoting) and triangular solves, FP64.
 //
-// Right-looking blocked LU, one batch item per matrix (one frequency):
-//   for each block column k0 (width NB):
-//     k_lu_panel      factor the (n-k0) x NB panel, pivot = max |re|+|im|
-//                     (LAPACK zgetf2 / izamax semantics, first index on ties)
-//     k_lu_swap_trsm  apply the NB row interchanges to every other column and
-//                     solve the unit-lower L11 U12 = A12 block row
-//     k_lu_gemm       trailing update A22 -= L21 U12 (complex)
-// Reference: the LAPACK factorisation behind `\` / implicit_linear
-// (src/matrix_assembly.jl:108-126).
+// Right-looking LU with block columns of NB = 128 (one batch item per
+// frequency).  Each block column is factored recursively (left half, update
+// of the right half, right half; as in LAPACK's zgetrf2) down to leaves of
+// 8 columns, and every update is a trailing GEMM on FP64 MFMA:
+//   k_lu_leaf      register-resident factorisation of an (n-k0) x <=8 panel,
+//                  1024 threads, pivot = max |re|+|im| (izamax semantics,
+//                  first index on ties)
+//   k_lu_swap      row interchanges of a pivot block on a column range
+//   k_lu_trsm      unit-lower L11 X = B on a <=32-row block, column range
+//   k_lu_gemm      C -= A B on arbitrary row/column/depth ranges, complex
+//                  product as four real v_mfma_f64_16x16x4 products
+// Extra columns beyond n (ncols > n) are carried through every step, so a
+// right-hand side appended to the matrix leaves the factorisation already
+// forward-substituted.  Reference: the LAPACK factorisation behind `\` /
+// implicit_linear (src/matrix_assembly.jl:108-126).
 #include "mh_internal.hpp"
 
 namespace mh {
 
-constexpr int kNB = 32;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNBOuter = 128;  // block-column width of the trailing updates
+constexpr int kLeaf = 8;       // register-resident leaf width
+constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 csub(double2 a, double2 b) {
-    return make_double2(a.x - b.x, a.y - b.y);
 }
 // a - b*c
 __device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
@@ -37,35 +44,49 @@ __device__ __forceinline__ double2 crecip(double2 a) {
 __device__ __forceinline__ double2 cdiv(double2 a, double2 b) { return cmul(a, crecip(b)); }
 
 // ---------------------------------------------------------------------------
-// panel factorisation: one 256-thread workgroup per batch item
+// leaf panel: columns [k0, k0+w), rows [k0, n), w <= 8, held in registers.
+// Thread t owns local rows t + 1024 q, q < R.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_panel(int64_t n, double2* __restrict__ A, int64_t ld,
-                                                  int64_t stride, int64_t k0, int nb,
+template <int R>
+__global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict__ A, int64_t ld,
+                                                  int64_t stride, int64_t k0, int w,
                                                   int64_t* __restrict__ ipiv,
                                                   int* __restrict__ info) {
     double2* M = A + (int64_t)blockIdx.x * stride;
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ double s_v[4];
-    __shared__ int64_t s_i[4];
-    __shared__ double2 s_row[kNB];
-    __shared__ int64_t s_p;
-    __shared__ int s_zero;
-    for (int c = 0; c < nb; ++c) {
-        const int64_t col = k0 + c;
+    const int m = (int)(n - k0);
+    __shared__ double s_v[16];
+    __shared__ int s_i[16];
+    __shared__ double2 s_crow[kLeaf], s_prow[kLeaf];
+    __shared__ int s_p, s_zero;
+    double2 a[R][kLeaf];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = tid + 1024 * q;
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c)
+            a[q][c] = (r < m && c < w) ? M[(k0 + r) + (k0 + c) * ld] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int c = 0; c < kLeaf; ++c) {
+        if (c < w) {  // uniform
         double best = -1.0;
-        int64_t bi = n;
-        for (int64_t r = col + tid; r < n; r += 256) {
-            const double2 a = M[r + col * ld];
-            const double v = fabs(a.x) + fabs(a.y);
-            if (v > best) {
-                best = v;
-                bi = r;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = tid + 1024 * q;
+            if (r >= c && r < m) {
+                const double v = fabs(a[q][c].x) + fabs(a[q][c].y);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
             const double ov = __shfl_xor(best, off);
-            const int64_t oi = __shfl_xor(bi, off);
+            const int oi = __shfl_xor(bi, off);
             if (ov > best || (ov == best && oi < bi)) {
                 best = ov;
                 bi = oi;
@@ -76,182 +97,301 @@ __global__ __launch_bounds__(256) void k_lu_panel(int64_t n, double2* __restrict
             s_i[wave] = bi;
         }
         __syncthreads();
-        if (tid == 0) {
-            double b = s_v[0];
-            int64_t p = s_i[0];
-            for (int w = 1; w < 4; ++w)
-                if (s_v[w] > b || (s_v[w] == b && s_i[w] < p)) {
-                    b = s_v[w];
-                    p = s_i[w];
+        if (wave == 0) {
+            double b = (lane < 16) ? s_v[lane] : -1.0;
+            int p = (lane < 16) ? s_i[lane] : 0x7fffffff;
+            for (int off = 8; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(b, off);
+                const int oi = __shfl_xor(p, off);
+                if (ov > b || (ov == b && oi < p)) {
+                    b = ov;
+                    p = oi;
                 }
-            if (b <= 0.0) {  // exact zero pivot: record, no swap, no update
-                p = col;
-                s_zero = 1;
-                atomicOr(info + blockIdx.x, 1);
-            } else {
-                s_zero = 0;
             }
-            s_p = p;
-            piv[col] = p;
+            if (lane == 0) {
+                if (b <= 0.0) {  // exact zero pivot: record, no interchange, no update
+                    p = c;
+                    s_zero = 1;
+                    atomicOr(info + blockIdx.x, 1);
+                } else {
+                    s_zero = 0;
+                }
+                s_p = p;
+                piv[k0 + c] = k0 + p;
+            }
         }
         __syncthreads();
-        const int64_t p = s_p;
-        if (tid < nb && p != col) {
-            const int64_t cj = (k0 + tid) * ld;
-            const double2 t = M[col + cj];
-            M[col + cj] = M[p + cj];
-            M[p + cj] = t;
+        const int p = s_p;
+        if (tid == c) {
+#pragma unroll
+            for (int j = 0; j < kLeaf; ++j) s_crow[j] = a[0][j];
         }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (tid + 1024 * q == p) {
+#pragma unroll
+                for (int j = 0; j < kLeaf; ++j) s_prow[j] = a[q][j];
+            }
         __syncthreads();
-        if (tid < nb) s_row[tid] = M[col + (k0 + tid) * ld];
-        __syncthreads();
+        if (p != c) {
+            if (tid == c) {
+#pragma unroll
+                for (int j = 0; j < kLeaf; ++j) a[0][j] = s_prow[j];
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if (tid + 1024 * q == p) {
+#pragma unroll
+                    for (int j = 0; j < kLeaf; ++j) a[q][j] = s_crow[j];
+                }
+        }
         if (!s_zero) {
-            const double2 inv = crecip(s_row[c]);
-            for (int64_t r = col + 1 + tid; r < n; r += 256) {
-                const double2 l = cmul(M[r + col * ld], inv);
-                M[r + col * ld] = l;
-                for (int j = c + 1; j < nb; ++j) {
-                    const int64_t o = r + (k0 + j) * ld;
-                    M[o] = cfms(M[o], l, s_row[j]);
+            const double2 inv = crecip(s_prow[c]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int r = tid + 1024 * q;
+                if (r > c && r < m) {
+                    const double2 l = cmul(a[q][c], inv);
+                    a[q][c] = l;
+#pragma unroll
+                    for (int j = c + 1; j < kLeaf; ++j) a[q][j] = cfms(a[q][j], l, s_prow[j]);
                 }
             }
         }
         __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int r = tid + 1024 * q;
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c)
+            if (r < m && c < w) M[(k0 + r) + (k0 + c) * ld] = a[q][c];
     }
 }
 
 // ---------------------------------------------------------------------------
-// row interchanges on all non-panel columns + L11 solve on the right block row
+// row interchanges of pivots [k0, k0+npiv) on columns [c_lo, c_hi)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_swap_trsm(int64_t n, int64_t ncols,
-                                                      double2* __restrict__ A, int64_t ld,
-                                                      int64_t stride, int64_t k0, int nb,
-                                                      const int64_t* __restrict__ ipiv) {
+__global__ __launch_bounds__(256) void k_lu_swap(double2* __restrict__ A, int64_t ld, int64_t stride,
+                                                 int64_t n, const int64_t* __restrict__ ipiv,
+                                                 int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
     double2* M = A + (int64_t)blockIdx.y * stride;
     const int64_t* piv = ipiv + (int64_t)blockIdx.y * n;
-    __shared__ double2 sL[kNB * kNB];  // sL[c*nb + r] = L11(r, c)
-    __shared__ int64_t sp[kNB];
-    const int tid = threadIdx.x;
-    for (int q = tid; q < nb * nb; q += 256) {
-        const int r = q % nb, c = q / nb;
-        sL[q] = M[(k0 + r) + (k0 + c) * ld];
-    }
-    if (tid < nb) sp[tid] = piv[k0 + tid];
+    __shared__ int64_t sp[kNBOuter];
+    for (int t = threadIdx.x; t < npiv; t += 256) sp[t] = piv[k0 + t];
     __syncthreads();
-    const int64_t j = (int64_t)blockIdx.x * 256 + tid;
-    if (j >= ncols || (j >= k0 && j < k0 + nb)) return;
+    const int64_t j = c_lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= c_hi) return;
     double2* col = M + j * ld;
-    for (int c = 0; c < nb; ++c) {
-        const int64_t p = sp[c];
-        if (p != k0 + c) {
-            const double2 t = col[k0 + c];
-            col[k0 + c] = col[p];
-            col[p] = t;
+    for (int t = 0; t < npiv; ++t) {
+        const int64_t p = sp[t];
+        if (p != k0 + t) {
+            const double2 x = col[k0 + t];
+            col[k0 + t] = col[p];
+            col[p] = x;
         }
     }
-    if (j < k0 + nb) return;  // left of the panel: interchanges only
-    double2 x[kNB];
-#pragma unroll
-    for (int r = 0; r < kNB; ++r)
-        if (r < nb) x[r] = col[k0 + r];
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) {
-#pragma unroll
-        for (int r = c + 1; r < kNB; ++r)
-            if (r < nb) x[r] = cfms(x[r], sL[c * nb + r], x[c]);
-    }
-#pragma unroll
-    for (int r = 0; r < kNB; ++r)
-        if (r < nb) col[k0 + r] = x[r];
 }
 
 // ---------------------------------------------------------------------------
-// trailing update C -= A B, C = M[r0:n, c0:ncols], A = M[r0:n, k0:k0+nb],
-// B = M[k0:k0+nb, c0:ncols]; 64 x 64 complex tile per 256-thread workgroup,
-// 4 x 4 complex outputs per thread.
+// X = L11^{-1} X, L11 = unit lower block at (k0, k0) of size nb <= 32,
+// X = rows [k0, k0+nb) of columns [c_lo, c_hi).  32 lanes per column (lane r
+// holds row r, coalesced 512-byte column reads); forward substitution by
+// broadcasting x_c within the 32-lane group.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_gemm(int64_t n, int64_t ncols, double2* __restrict__ A,
-                                                 int64_t ld, int64_t stride, int64_t k0, int nb) {
-    double2* M = A + (int64_t)blockIdx.z * stride;
-    const int64_t r0 = k0 + nb, c0 = k0 + nb;
-    const int64_t tr = r0 + (int64_t)blockIdx.x * 64;
-    const int64_t tc = c0 + (int64_t)blockIdx.y * 64;
-    __shared__ double2 sA[kNB][64];  // sA[k][row]
-    __shared__ double2 sB[kNB][64];  // sB[k][col]
-    const int tid = threadIdx.x;
-    for (int q = tid; q < nb * 64; q += 256) {
-        const int r = q & 63, k = q >> 6;
-        const int64_t gr = tr + r;
-        sA[k][r] = (gr < n) ? M[gr + (k0 + k) * ld] : make_double2(0.0, 0.0);
-        const int cc = q / nb, kk = q % nb;  // B: walk down each column
-        const int64_t gc = tc + cc;
-        sB[kk][cc] = (gc < ncols) ? M[(k0 + kk) + gc * ld] : make_double2(0.0, 0.0);
+__global__ __launch_bounds__(256) void k_lu_trsm(double2* __restrict__ A, int64_t ld, int64_t stride,
+                                                 int64_t k0, int nb, int64_t c_lo, int64_t c_hi) {
+    double2* M = A + (int64_t)blockIdx.y * stride;
+    __shared__ double2 sL[kTrsmNB][kTrsmNB + 1];  // sL[c][r] = L11(r, c)
+    for (int q = threadIdx.x; q < kTrsmNB * kTrsmNB; q += 256) {
+        const int r = q % kTrsmNB, c = q / kTrsmNB;
+        sL[c][r] = (r < nb && c < nb) ? M[(k0 + r) + (k0 + c) * ld] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    const int tx = tid & 15, ty = tid >> 4;
-    double2 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-    for (int k = 0; k < nb; ++k) {
-        double2 av[4], bv[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) av[a] = sA[k][tx + 16 * a];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) bv[b] = sB[k][ty + 16 * b];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                acc[a][b].x = fma(av[a].x, bv[b].x, acc[a][b].x);
-                acc[a][b].x = fma(-av[a].y, bv[b].y, acc[a][b].x);
-                acc[a][b].y = fma(av[a].x, bv[b].y, acc[a][b].y);
-                acc[a][b].y = fma(av[a].y, bv[b].x, acc[a][b].y);
-            }
+    const int r = threadIdx.x & 31;
+    const int64_t j = c_lo + (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    const bool ok = j < c_hi;
+    double2* col = M + (ok ? j : c_lo) * ld;
+    double2 x = (ok && r < nb) ? col[k0 + r] : make_double2(0.0, 0.0);
+    for (int c = 0; c < nb; ++c) {
+        const double2 xc = make_double2(__shfl(x.x, c, 32), __shfl(x.y, c, 32));
+        if (r > c) x = cfms(x, sL[c][r], xc);
     }
+    if (ok && r < nb) col[k0 + r] = x;
+}
+
+// ---------------------------------------------------------------------------
+// C[r_lo:r_hi, c_lo:c_hi] -= A[r_lo:r_hi, k_lo:k_lo+kw] * A[k_lo:k_lo+kw, c_lo:c_hi]
+// 64 x 64 complex tile per 256-thread workgroup; wave (wr, wc) owns a 32 x 32
+// quarter as 2 x 2 MFMA tiles of 16 x 16, real and imaginary accumulators.
+// v_mfma_f64_16x16x4f64 lane layout: A[i = l%16][k = l/16],
+// B[k = l/16][j = l%16], D register r -> D[4r + l/16][l%16].
+// ---------------------------------------------------------------------------
+constexpr int kKC = 16;  // depth per LDS stage
+
+__global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_t ld, int64_t stride,
+                                                 int64_t r_lo, int64_t r_hi, int64_t c_lo,
+                                                 int64_t c_hi, int64_t k_lo, int kw) {
+    double2* M = A + (int64_t)blockIdx.z * stride;
+    const int64_t tr = r_lo + (int64_t)blockIdx.x * 64;
+    const int64_t tc = c_lo + (int64_t)blockIdx.y * 64;
+    __shared__ double2 sA[kKC][64];  // sA[k][row]
+    __shared__ double2 sB[kKC][64];  // sB[k][col]
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
+    d4 accr[2][2], acci[2][2];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int64_t gc = tc + ty + 16 * b;
-        if (gc >= ncols) continue;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int64_t gr = tr + tx + 16 * a;
-            if (gr >= n) continue;
-            const int64_t o = gr + gc * ld;
-            M[o] = csub(M[o], acc[a][b]);
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+            acci[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
         }
+    for (int kc = 0; kc < kw; kc += kKC) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = tid + 256 * t;
+            {
+                const int r = q & 63, k = q >> 6;
+                const int64_t gr = tr + r;
+                sA[k][r] = (gr < r_hi && kc + k < kw) ? M[gr + (k_lo + kc + k) * ld]
+                                                      : make_double2(0.0, 0.0);
+            }
+            {
+                const int k = q & 15, c = q >> 4;
+                const int64_t gc = tc + c;
+                sB[k][c] = (gc < c_hi && kc + k < kw) ? M[(k_lo + kc + k) + gc * ld]
+                                                      : make_double2(0.0, 0.0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            double2 av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = sA[kk + (l >> 4)][32 * wr + 16 * a + (l & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = sB[kk + (l >> 4)][32 * wc + 16 * b + (l & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, accr[a][b], 0, 0, 0);
+                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[a].y, bv[b].y, accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].y, acci[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].x, acci[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
     }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t gc = tc + 32 * wc + 16 * b + (l & 15);
+            if (gc >= c_hi) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 32 * wr + 16 * a + 4 * r + (l >> 4);
+                if (gr >= r_hi) continue;
+                double2* o = M + gr + gc * ld;
+                const double2 v = *o;
+                *o = make_double2(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// host-side recursion
+// ---------------------------------------------------------------------------
+struct LuPlan {
+    Context* ctx;
+    double2* A;
+    int64_t n, ncols, ld, stride, batch;
+    int64_t* ipiv;
+    int* info;
+};
+
+static int lu_swap(const LuPlan& P, int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
+    if (c_hi <= c_lo || npiv <= 0) return MH_OK;
+    const int64_t nc = c_hi - c_lo;
+    ProfScope ps(P.ctx, "lu_swap", 0.0, 64.0 * npiv * (double)nc * P.batch);
+    hipLaunchKernelGGL(k_lu_swap, dim3((unsigned)((nc + 255) / 256), (unsigned)P.batch), dim3(256), 0,
+                       P.ctx->stream, P.A, P.ld, P.stride, P.n, P.ipiv, k0, npiv, c_lo, c_hi);
+    return MH_OK;
+}
+
+static int lu_gemm(const LuPlan& P, int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
+                   int64_t k_lo, int kw) {
+    if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
+    const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
+    ProfScope ps(P.ctx, "lu_gemm", 8.0 * (double)mr * (double)mc * kw * P.batch,
+                 16.0 * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
+    hipLaunchKernelGGL(k_lu_gemm, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
+                       dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
+    return MH_OK;
+}
+
+// X = L[k0:k0+w, k0:k0+w]^{-1} X on columns [c_lo, c_hi), blocked by 32 rows
+static int lu_trsm(const LuPlan& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+    if (c_hi <= c_lo) return MH_OK;
+    const int64_t nc = c_hi - c_lo;
+    for (int s = 0; s < w; s += kTrsmNB) {
+        const int t = (w - s) < kTrsmNB ? (w - s) : kTrsmNB;
+        {
+            ProfScope ps(P.ctx, "lu_trsm", 4.0 * t * t * (double)nc * P.batch, 32.0 * t * (double)nc * P.batch);
+            hipLaunchKernelGGL(k_lu_trsm, dim3((unsigned)((nc + 7) / 8), (unsigned)P.batch), dim3(256), 0,
+                               P.ctx->stream, P.A, P.ld, P.stride, k0 + s, t, c_lo, c_hi);
+        }
+        if (s + t < w) TRY_RC(lu_gemm(P, k0 + s + t, k0 + w, c_lo, c_hi, k0 + s, t));
+    }
+    return MH_OK;
+}
+
+static int lu_leaf(const LuPlan& P, int64_t k0, int w) {
+    const int64_t m = P.n - k0;
+    ProfScope ps(P.ctx, "lu_leaf", 8.0 * (double)m * w * w / 2.0 * P.batch, 32.0 * (double)m * w * P.batch);
+    if (m <= 1024)
+        hipLaunchKernelGGL((k_lu_leaf<1>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+                           P.ld, P.stride, k0, w, P.ipiv, P.info);
+    else if (m <= 2048)
+        hipLaunchKernelGGL((k_lu_leaf<2>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+                           P.ld, P.stride, k0, w, P.ipiv, P.info);
+    else
+        hipLaunchKernelGGL((k_lu_leaf<4>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+                           P.ld, P.stride, k0, w, P.ipiv, P.info);
+    return MH_OK;
+}
+
+// recursive factorisation of the panel [k0, k0+w) (rows k0..n), touching only
+// the panel's columns (zgetrf2 order)
+static int lu_panel(const LuPlan& P, int64_t k0, int w) {
+    if (w <= kLeaf) return lu_leaf(P, k0, w);
+    int h = ((w / 2) + kLeaf - 1) / kLeaf * kLeaf;
+    if (h >= w) h = w - kLeaf;
+    TRY_RC(lu_panel(P, k0, h));
+    TRY_RC(lu_swap(P, k0, h, k0 + h, k0 + w));
+    TRY_RC(lu_trsm(P, k0, h, k0 + h, k0 + w));
+    TRY_RC(lu_gemm(P, k0 + h, P.n, k0 + h, k0 + w, k0, h));
+    TRY_RC(lu_panel(P, k0 + h, w - h));
+    TRY_RC(lu_swap(P, k0 + h, w - h, k0, k0 + h));
+    return MH_OK;
 }
 
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
               int64_t stride, int64_t* ipiv, int* d_info) {
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
-    for (int64_t k0 = 0; k0 < n; k0 += kNB) {
-        const int nb = (int)((n - k0) < kNB ? (n - k0) : kNB);
-        const double m_ = (double)(n - k0);
-        const double rem = (double)(ncols - k0 - nb);
-        {
-            ProfScope ps(ctx, "lu_panel", 8.0 * m_ * nb * nb / 2.0 * batch, 32.0 * m_ * nb * batch);
-        hipLaunchKernelGGL(k_lu_panel, dim3((unsigned)batch), dim3(256), 0, ctx->stream, n, A, ld,
-                           stride, k0, nb, ipiv, d_info);
-        }
-        {
-            ProfScope ps(ctx, "lu_swap_trsm", 4.0 * nb * nb * rem * batch,
-                         16.0 * 2.0 * nb * (double)ncols * batch);
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((unsigned)((ncols + 255) / 256), (unsigned)batch),
-                           dim3(256), 0, ctx->stream, n, ncols, A, ld, stride, k0, nb, ipiv);
-        }
-        const int64_t rem_r = n - k0 - nb, rem_c = ncols - k0 - nb;
-        if (rem_r > 0 && rem_c > 0) {
-            ProfScope ps(ctx, "lu_gemm", 8.0 * (double)rem_r * (double)rem_c * nb * batch,
-                         16.0 * (2.0 * rem_r * rem_c + nb * (double)(rem_r + rem_c)) * batch);
-            hipLaunchKernelGGL(k_lu_gemm,
-                               dim3((unsigned)((rem_r + 63) / 64), (unsigned)((rem_c + 63) / 64),
-                                    (unsigned)batch),
-                               dim3(256), 0, ctx->stream, n, ncols, A, ld, stride, k0, nb);
-        }
+    MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
+    LuPlan P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info};
+    for (int64_t k0 = 0; k0 < n; k0 += kNBOuter) {
+        const int w = (int)((n - k0) < kNBOuter ? (n - k0) : kNBOuter);
+        TRY_RC(lu_panel(P, k0, w));
+        TRY_RC(lu_swap(P, k0, w, 0, k0));
+        TRY_RC(lu_swap(P, k0, w, k0 + w, ncols));
+        TRY_RC(lu_trsm(P, k0, w, k0 + w, ncols));
+        TRY_RC(lu_gemm(P, k0 + w, n, k0 + w, ncols, k0, w));
     }
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
@@ -259,164 +399,230 @@ int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A,
 
 // ---------------------------------------------------------------------------
 // triangular solves with the LU factors, one 256-thread workgroup per
-// (batch item, right-hand side); the vector lives in LDS.
+// (batch item, right-hand side).  The vector stays in global memory (it is
+// small and cache-resident); each 32 x 32 diagonal block is staged in LDS
+// and solved by one wave, then the rest of the vector is updated by the whole
+// workgroup with coalesced column reads.
+//   part: 1 = lower (unit L, with the pivots), 2 = upper (U), 3 = both.
 // ---------------------------------------------------------------------------
-constexpr int kMaxSolveN = 4096;  // vector held in 64 KiB of LDS
+// Applies the row interchanges ipiv[0..n) to a vector b (global memory) with
+// the whole workgroup: the net permutation is built in LDS by one thread,
+// then every thread scatters its entries.  forward: b <- P b (interchanges in
+// order k = 0..n-1); otherwise the inverse (k = n-1..0).
+constexpr int kMaxSolveN = 4096;
+constexpr int kPerThread = kMaxSolveN / 256;
+
+__device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, double2* __restrict__ b,
+                             bool forward, int* s_perm, int* s_inv) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < n; i += 256) s_perm[i] = i;
+    __syncthreads();
+    if (tid == 0)
+        for (int k = 0; k < n; ++k) {
+            const int p = (int)piv[k];
+            if (p != k) {
+                const int t = s_perm[k];
+                s_perm[k] = s_perm[p];
+                s_perm[p] = t;
+            }
+        }
+    __syncthreads();
+    // forward: b_new[i] = b_old[perm[i]]  <=>  b_new[inv[j]] = b_old[j]
+    // inverse: b_new[perm[i]] = b_old[i]
+    if (forward) {
+        for (int i = tid; i < n; i += 256) s_inv[s_perm[i]] = i;
+        __syncthreads();
+    }
+    const int* dst = forward ? s_inv : s_perm;
+    double2 v[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + 256 * q;
+        if (j < n) v[q] = b[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + 256 * q;
+        if (j < n) b[dst[j]] = v[q];
+    }
+    __syncthreads();
+}
+
+constexpr int kSB = 32;
 
 __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __restrict__ A,
                                                   int64_t ld, int64_t stride,
-                                                  const int64_t* __restrict__ ipiv, int trans,
+                                                  const int64_t* __restrict__ ipiv, int trans, int part,
                                                   double2* __restrict__ B, int64_t ldb,
                                                   int64_t bstride) {
-    extern __shared__ double2 sb[];
     const int64_t item = blockIdx.y;
     const double2* M = A + item * stride;
     const int64_t* piv = ipiv + item * n;
     double2* b = B + item * bstride + (int64_t)blockIdx.x * ldb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int64_t i = tid; i < n; i += 256) sb[i] = b[i];
-    __syncthreads();
+    __shared__ double2 sD[kSB][kSB + 1];  // diagonal block, sD[r][c]
+    __shared__ double2 sx[kSB];
+    __shared__ int s_perm[kMaxSolveN], s_inv[kMaxSolveN];
+    const bool lower = part & 1, upper = part & 2;
     if (!trans) {
-        if (tid == 0)
-            for (int64_t k = 0; k < n; ++k) {
-                const int64_t p = piv[k];
-                if (p != k) {
-                    const double2 t = sb[k];
-                    sb[k] = sb[p];
-                    sb[p] = t;
+        if (lower) {
+            apply_pivots(n, piv, b, true, s_perm, s_inv);
+            for (int64_t j0 = 0; j0 < n; j0 += kSB) {  // L y = b
+                const int jn = (int)((n - j0) < kSB ? (n - j0) : kSB);
+                for (int q = tid; q < kSB * kSB; q += 256) {
+                    const int r = q % kSB, c = q / kSB;
+                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : make_double2(0.0, 0.0);
                 }
-            }
-        __syncthreads();
-        // L y = b (unit lower), 64-column blocks
-        for (int64_t j0 = 0; j0 < n; j0 += 64) {
-            const int64_t jn = (n - j0) < 64 ? (n - j0) : 64;
-            if (wave == 0) {
-                double2 v = (lane < jn) ? sb[j0 + lane] : make_double2(0.0, 0.0);
-                for (int c = 0; c < jn; ++c) {
-                    const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
-                    if (lane > c && lane < jn) v = cfms(v, M[(j0 + lane) + (j0 + c) * ld], yc);
+                __syncthreads();
+                if (wave == 0) {
+                    double2 v = (lane < jn) ? b[j0 + lane] : make_double2(0.0, 0.0);
+                    for (int c = 0; c < jn; ++c) {
+                        const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        if (lane > c && lane < jn) v = cfms(v, sD[lane][c], yc);
+                    }
+                    if (lane < jn) {
+                        b[j0 + lane] = v;
+                        sx[lane] = v;
+                    }
                 }
-                if (lane < jn) sb[j0 + lane] = v;
+                __syncthreads();
+                for (int64_t i = j0 + jn + tid; i < n; i += 256) {
+                    double2 v = b[i];
+                    for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
+                    b[i] = v;
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            for (int64_t i = j0 + jn + tid; i < n; i += 256) {
-                double2 v = sb[i];
-                for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sb[j0 + c]);
-                sb[i] = v;
-            }
-            __syncthreads();
         }
-        // U x = y (upper), blocks from the bottom
-        for (int64_t j1 = n; j1 > 0; j1 -= 64) {
-            const int64_t j0 = (j1 - 64) > 0 ? (j1 - 64) : 0;
-            const int64_t jn = j1 - j0;
-            if (wave == 0) {
-                double2 v = (lane < jn) ? sb[j0 + lane] : make_double2(0.0, 0.0);
-                for (int c = (int)jn - 1; c >= 0; --c) {
-                    if (lane == c) v = cdiv(v, M[(j0 + c) + (j0 + c) * ld]);
-                    const double2 xc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
-                    if (lane < c) v = cfms(v, M[(j0 + lane) + (j0 + c) * ld], xc);
+        if (upper) {
+            for (int64_t j1 = n; j1 > 0; j1 -= kSB) {  // U x = y
+                const int64_t j0 = (j1 - kSB) > 0 ? (j1 - kSB) : 0;
+                const int jn = (int)(j1 - j0);
+                for (int q = tid; q < kSB * kSB; q += 256) {
+                    const int r = q % kSB, c = q / kSB;
+                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : make_double2(1.0, 0.0);
                 }
-                if (lane < jn) sb[j0 + lane] = v;
+                __syncthreads();
+                if (wave == 0) {
+                    double2 v = (lane < jn) ? b[j0 + lane] : make_double2(0.0, 0.0);
+                    for (int c = jn - 1; c >= 0; --c) {
+                        if (lane == c) v = cdiv(v, sD[c][c]);
+                        const double2 xc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        if (lane < c) v = cfms(v, sD[lane][c], xc);
+                    }
+                    if (lane < jn) {
+                        b[j0 + lane] = v;
+                        sx[lane] = v;
+                    }
+                }
+                __syncthreads();
+                for (int64_t i = tid; i < j0; i += 256) {
+                    double2 v = b[i];
+                    for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
+                    b[i] = v;
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            for (int64_t i = tid; i < j0; i += 256) {
-                double2 v = sb[i];
-                for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sb[j0 + c]);
-                sb[i] = v;
-            }
-            __syncthreads();
         }
     } else {
-        // U^T y = b (lower, non-unit): dot-product form over columns of U
-        for (int64_t i0 = 0; i0 < n; i0 += 64) {
-            const int64_t in = (n - i0) < 64 ? (n - i0) : 64;
-            // b[i] -= sum_{j < i0} U[j, i] y[j]; one wave per row, lanes over j
-            for (int64_t r = wave; r < in; r += 4) {
-                const int64_t i = i0 + r;
-                double sx = 0.0, sy = 0.0;
-                for (int64_t j = lane; j < i0; j += 64) {
-                    const double2 u = M[j + i * ld], y = sb[j];
-                    sx += u.x * y.x - u.y * y.y;
-                    sy += u.x * y.y + u.y * y.x;
+        if (upper) {
+            // U^T y = b (lower, non-unit): rows in blocks, dot products over columns of U
+            for (int64_t i0 = 0; i0 < n; i0 += kSB) {
+                const int in = (int)((n - i0) < kSB ? (n - i0) : kSB);
+                for (int r = wave; r < in; r += 4) {
+                    const int64_t i = i0 + r;
+                    double sx_ = 0.0, sy_ = 0.0;
+                    for (int64_t j = lane; j < i0; j += 64) {
+                        const double2 u = M[j + i * ld], y = b[j];
+                        sx_ += u.x * y.x - u.y * y.y;
+                        sy_ += u.x * y.y + u.y * y.x;
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        sx_ += __shfl_xor(sx_, off);
+                        sy_ += __shfl_xor(sy_, off);
+                    }
+                    if (lane == 0) b[i] = make_double2(b[i].x - sx_, b[i].y - sy_);
                 }
-                for (int off = 32; off > 0; off >>= 1) {
-                    sx += __shfl_xor(sx, off);
-                    sy += __shfl_xor(sy, off);
+                for (int q = tid; q < kSB * kSB; q += 256) {
+                    const int r = q % kSB, c = q / kSB;
+                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : make_double2(1.0, 0.0);
                 }
-                if (lane == 0) sb[i] = make_double2(sb[i].x - sx, sb[i].y - sy);
+                __syncthreads();
+                if (wave == 0) {
+                    double2 v = (lane < in) ? b[i0 + lane] : make_double2(0.0, 0.0);
+                    for (int c = 0; c < in; ++c) {
+                        if (lane == c) v = cdiv(v, sD[c][c]);
+                        const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        if (lane > c && lane < in) v = cfms(v, sD[c][lane], yc);
+                    }
+                    if (lane < in) b[i0 + lane] = v;
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (wave == 0) {
-                double2 v = (lane < in) ? sb[i0 + lane] : make_double2(0.0, 0.0);
-                for (int c = 0; c < in; ++c) {
-                    if (lane == c) v = cdiv(v, M[(i0 + c) + (i0 + c) * ld]);
-                    const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
-                    if (lane > c && lane < in) v = cfms(v, M[(i0 + c) + (i0 + lane) * ld], yc);
-                }
-                if (lane < in) sb[i0 + lane] = v;
-            }
-            __syncthreads();
         }
-        // L^T z = y (upper, unit): from the bottom, dot over columns of L
-        for (int64_t i1 = n; i1 > 0; i1 -= 64) {
-            const int64_t i0 = (i1 - 64) > 0 ? (i1 - 64) : 0;
-            const int64_t in = i1 - i0;
-            for (int64_t r = wave; r < in; r += 4) {
-                const int64_t i = i0 + r;
-                double sx = 0.0, sy = 0.0;
-                for (int64_t j = i1 + lane; j < n; j += 64) {
-                    const double2 l = M[j + i * ld], z = sb[j];
-                    sx += l.x * z.x - l.y * z.y;
-                    sy += l.x * z.y + l.y * z.x;
+        if (lower) {
+            // L^T z = y (upper, unit): blocks from the bottom, dots over columns of L
+            for (int64_t i1 = n; i1 > 0; i1 -= kSB) {
+                const int64_t i0 = (i1 - kSB) > 0 ? (i1 - kSB) : 0;
+                const int in = (int)(i1 - i0);
+                for (int r = wave; r < in; r += 4) {
+                    const int64_t i = i0 + r;
+                    double sx_ = 0.0, sy_ = 0.0;
+                    for (int64_t j = i1 + lane; j < n; j += 64) {
+                        const double2 lv = M[j + i * ld], z = b[j];
+                        sx_ += lv.x * z.x - lv.y * z.y;
+                        sy_ += lv.x * z.y + lv.y * z.x;
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        sx_ += __shfl_xor(sx_, off);
+                        sy_ += __shfl_xor(sy_, off);
+                    }
+                    if (lane == 0) b[i] = make_double2(b[i].x - sx_, b[i].y - sy_);
                 }
-                for (int off = 32; off > 0; off >>= 1) {
-                    sx += __shfl_xor(sx, off);
-                    sy += __shfl_xor(sy, off);
+                for (int q = tid; q < kSB * kSB; q += 256) {
+                    const int r = q % kSB, c = q / kSB;
+                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : make_double2(0.0, 0.0);
                 }
-                if (lane == 0) sb[i] = make_double2(sb[i].x - sx, sb[i].y - sy);
+                __syncthreads();
+                if (wave == 0) {
+                    double2 v = (lane < in) ? b[i0 + lane] : make_double2(0.0, 0.0);
+                    for (int c = in - 1; c >= 0; --c) {
+                        const double2 zc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        if (lane < c) v = cfms(v, sD[c][lane], zc);
+                    }
+                    if (lane < in) b[i0 + lane] = v;
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (wave == 0) {
-                double2 v = (lane < in) ? sb[i0 + lane] : make_double2(0.0, 0.0);
-                for (int c = (int)in - 1; c >= 0; --c) {
-                    const double2 zc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
-                    if (lane < c) v = cfms(v, M[(i0 + c) + (i0 + lane) * ld], zc);
-                }
-                if (lane < in) sb[i0 + lane] = v;
-            }
-            __syncthreads();
+            apply_pivots(n, piv, b, false, s_perm, s_inv);
         }
-        if (tid == 0)
-            for (int64_t k = n - 1; k >= 0; --k) {
-                const int64_t p = piv[k];
-                if (p != k) {
-                    const double2 t = sb[k];
-                    sb[k] = sb[p];
-                    sb[p] = t;
-                }
-            }
-        __syncthreads();
     }
-    for (int64_t i = tid; i < n; i += 256) b[i] = sb[i];
+}
+
+int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
+                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
+                  double2* B, int64_t ldb, int64_t bstride) {
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    MH_REQUIRE(batch <= 65535 && nrhs <= 65535, "lu_solve: batch or nrhs too large");
+    MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
+    const double tri = ((part & 1) ? 0.5 : 0.0) + ((part & 2) ? 0.5 : 0.0);
+    ProfScope ps(ctx, "lu_solve", 8.0 * tri * (double)n * n * nrhs * batch,
+                 16.0 * tri * (double)n * n * nrhs * batch);
+    hipLaunchKernelGGL(k_lu_solve, dim3((unsigned)nrhs, (unsigned)batch), dim3(256), 0, ctx->stream, n, LU,
+                       ld, stride, ipiv, trans, part, B, ldb, bstride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
 }
 
 int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
              const int64_t* ipiv, int trans, int64_t nrhs, double2* B, int64_t ldb,
              int64_t bstride) {
-    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
-    MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
-    MH_REQUIRE(batch <= 65535 && nrhs <= 65535, "lu_solve: batch or nrhs too large");
-    const size_t lds = sizeof(double2) * (size_t)n;
-    ProfScope ps(ctx, "lu_solve", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * nrhs * batch);
-    hipLaunchKernelGGL(k_lu_solve, dim3((unsigned)nrhs, (unsigned)batch), dim3(256), lds,
-                       ctx->stream, n, LU, ld, stride, ipiv, trans, B, ldb, bstride);
-    MH_HIP_CHECK(hipGetLastError());
-    return MH_OK;
+    return lu_solve_part(ctx, n, batch, LU, ld, stride, ipiv, trans, 3, nrhs, B, ldb, bstride);
 }
 
 // ---------------------------------------------------------------------------
-// Y = S X for small nrhs: one thread per row, S read once, coalesced
+// Y = S X (trans = 0) or Y = S^T X (trans = 1) for small nrhs
 // ---------------------------------------------------------------------------
 constexpr int kMaxRhs = 16;
 
@@ -457,8 +663,43 @@ __global__ __launch_bounds__(256) void k_zgemv_multi(int64_t n, int64_t nrhs,
             }
         }
     }
-    if (i < n)
-        for (int r = 0; r < nrhs; ++r) Ym[i + r * ldy] = acc[r];
+    if (i < n) {
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r)
+            if (r < nrhs) Ym[i + r * ldy] = acc[r];
+    }
+}
+
+// Y[:, r] = S^T X[:, r]: one wave per output row i (a column of S), lanes
+// along the column, coalesced.
+__global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
+                                                       const double2* __restrict__ S, int64_t ld,
+                                                       int64_t stride, const double2* __restrict__ X,
+                                                       int64_t ldx, int64_t xstride,
+                                                       double2* __restrict__ Y, int64_t ldy,
+                                                       int64_t ystride) {
+    const int64_t item = blockIdx.y;
+    const double2* Sm = S + item * stride;
+    const double2* Xm = X + item * xstride;
+    double2* Ym = Y + item * ystride;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double2* col = Sm + i * ld;
+    for (int64_t r = 0; r < nrhs; ++r) {
+        const double2* x = Xm + r * ldx;
+        double sr = 0.0, si = 0.0;
+        for (int64_t j = lane; j < n; j += 64) {
+            const double2 s = col[j], v = x[j];
+            sr += s.x * v.x - s.y * v.y;
+            si += s.x * v.y + s.y * v.x;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
+        }
+        if (lane == 0) Ym[i + r * ldy] = make_double2(sr, si);
+    }
 }
 
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
@@ -469,6 +710,17 @@ int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
     ProfScope ps(ctx, "zgemv", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * batch);
     hipLaunchKernelGGL(k_zgemv_multi, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256),
                        0, ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
+                     int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
+                     double2* Y, int64_t ldy, int64_t ystride) {
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    ProfScope ps(ctx, "zgemv_t", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * nrhs * batch);
+    hipLaunchKernelGGL(k_zgemv_t_multi, dim3((unsigned)((n + 3) / 4), (unsigned)batch), dim3(256), 0,
+                       ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
