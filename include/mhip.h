@@ -192,6 +192,38 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
                   const double* dof_fields, int64_t nbeta, const double* betas,
                   double* A, double* B, double* F);
 
+/* ---- gradients ----------------------------------------------------------
+ * mh_freq_sweep plus the derivatives of A, B and F along nparam parameter
+ * directions theta_m (dimensions, positions, frequency), for the Zygote /
+ * ForwardDiff uses of the reference (test/outputs_differentiation.jl:59-153,
+ * paper/MeshGradients_singlebody.jl, paper/meshGradients_pair.jl).  The
+ * linear solve is differentiated by its adjoint (one transposed solve with
+ * the LU factors per influenced dof, as ImplicitAD.implicit_linear does for
+ * the reference, src/matrix_assembly.jl:120); the influence matrices'
+ * derivatives are recomputed pair by pair and contracted on the fly, never
+ * stored.  Tangents (device, may be NULL = zero):
+ *   dvertices  : nparam x (nvertices x 3)  d vertices / d theta_m
+ *   dcenters   : nparam x (nfaces x 3)     d centers  / d theta_m
+ *   dnormals   : nparam x (nfaces x 3)
+ *   dareas     : nparam x nfaces
+ *   ddof_fields: nparam x ndof x (nfaces x 3)
+ *   domegas    : HOST, nparam x nomega     d omega_w / d theta_m
+ * Radii only select the near/far branch and carry no derivative.
+ * Outputs (device): A, B, F as mh_freq_sweep; dA, dB: nomega x nparam x
+ * ndof x ndof (same inner layout as A); dF: nomega x nparam x nbeta x ndof
+ * complex128.  Synchronises.  Limits: ndof >= 1, ndof + nbeta <= 4, FP64,
+ * Rankine/image/Wu terms. */
+int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
+                       int direct, int precision, int64_t nomega,
+                       const double* omegas, double g, double rho, int64_t ndof,
+                       const double* dof_fields, int64_t nbeta,
+                       const double* betas, int64_t nparam,
+                       const double* dvertices, const double* dcenters,
+                       const double* dnormals, const double* dareas,
+                       const double* ddof_fields, const double* domegas,
+                       double* A, double* B, double* F, double* dA, double* dB,
+                       double* dF);
+
 /* ---- host-array entry points (one call, synchronous) -------------------
  * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
  * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.

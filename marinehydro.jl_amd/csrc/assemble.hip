@@ -44,75 +44,6 @@ __global__ void k_pack_panels(int64_t nf, int64_t nv, const double* __restrict__
     out[j] = p;
 }
 
-// Rankine + image Rankine: value and gradients w.r.t. the source point (gxi)
-// and the collocation point (gx) for one pair.
-__device__ __forceinline__ void rankine_terms(int gf, double x0, double x1, double x2,
-                                              const PanelRec& p, double& I, double gxi[3],
-                                              double gx[3]) {
-    I = 0.0;
-    gxi[0] = gxi[1] = gxi[2] = 0.0;
-    gx[0] = gx[1] = gx[2] = 0.0;
-    double v, g[3];
-    if (gf & GF_RANKINE) {  // rankine.jl:24-128
-        rankine_panel(x0, x1, x2, p, v, g);
-        I += v;
-        gxi[0] += g[0];
-        gxi[1] += g[1];
-        gxi[2] += g[2];
-        gx[0] -= g[0];
-        gx[1] -= g[1];
-        gx[2] -= g[2];
-    }
-    if (gf & (GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) {
-        // collocation point reflected across z = 0 (rankine_reflected.jl:19-35)
-        rankine_panel(x0, x1, -x2, p, v, g);
-        const double s = ((gf & GF_RANKINE_REFLECTED) ? 1.0 : 0.0) -
-                         ((gf & GF_RANKINE_REFLECTED_NEG) ? 1.0 : 0.0);
-        I += s * v;
-        gxi[0] += s * g[0];
-        gxi[1] += s * g[1];
-        gxi[2] += s * g[2];
-        // w.r.t. x: vertical reflection of minus the source gradient
-        gx[0] -= s * g[0];
-        gx[1] -= s * g[1];
-        gx[2] += s * g[2];
-    }
-}
-
-// Wave term for one pair at wavenumber k, already multiplied by the source
-// area (one-point rule, wu.jl:33-47): value, gradient w.r.t. the source
-// point (z component shared with the collocation gradient) and the
-// horizontal factors.  Reduced coordinates per
-// abstract_greens_function.jl:88-155.
-template <int WAVE>
-__device__ __forceinline__ void wave_term(double k, double dx, double dy, double zsum,
-                                          double area, cplx& Sv, cplx gxi[3], cplx gx[3]) {
-    const double r = k * horiz_dist(dx, dy);
-    const double z = k * zsum;
-    cplx G, dGdr, dGdz;
-    if (WAVE == 1) wu_dimless<true>(r, z, G, dGdr, dGdz);
-    double d1 = 0.0, d2 = 0.0;  // dr/dxi_1, dr/dxi_2
-    if (fabs(r) > 1e-6) {
-        const double kk = k * k / r;
-        d1 = kk * (-dx);
-        d2 = kk * (-dy);
-    }
-    const double ka = k * area;
-    Sv.re = ka * G.re;
-    Sv.im = ka * G.im;
-    gxi[0].re = ka * (d1 * dGdr.re);
-    gxi[0].im = ka * (d1 * dGdr.im);
-    gxi[1].re = ka * (d2 * dGdr.re);
-    gxi[1].im = ka * (d2 * dGdr.im);
-    gxi[2].re = ka * (k * dGdz.re);
-    gxi[2].im = ka * (k * dGdz.im);
-    gx[0].re = -gxi[0].re;
-    gx[0].im = -gxi[0].im;
-    gx[1].re = -gxi[1].re;
-    gx[1].im = -gxi[1].im;
-    gx[2] = gxi[2];
-}
-
 constexpr int kFG = 4;  // wavenumbers per thread (blockIdx.z group)
 
 template <int WAVE, int FORM>

@@ -339,42 +339,32 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
     return check_info(c, reinterpret_cast<int*>(ws + o_info), batch);
 }
 
-int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
-                  int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
-                  const double* dof_fields, int64_t nbeta, const double* betas, double* A,
-                  double* B, double* F) {
-    TRY(check_ctx(ctx));
-    TRY(check_mesh(mesh));
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "freq_sweep: bad Green-function mask");
-    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
-               "freq_sweep: choose one wave term");
-    MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0, "freq_sweep: negative size");
-    MH_REQUIRE(g > 0.0 && rho > 0.0, "freq_sweep: g and rho must be positive");
-    if (precision != MH_FP64) {
-        set_error("freq_sweep: only MH_FP64 is available in this build");
-        return MH_ERR_NOT_SUPPORTED;
-    }
+// Optional gradient request of sweep_impl (see mh_freq_sweep_grad).
+struct GradArgs {
+    int64_t nparam;
+    const double *dV, *dC, *dN, *dA, *ddofs;  // device tangents (may be NULL)
+    const double* domega;                      // host, nparam x nomega (may be NULL)
+    double *dAout, *dBout, *dFout;             // device outputs
+};
+
+static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int64_t nomega,
+                      const double* omegas, double g, double rho, int64_t ndof,
+                      const double* dof_fields, int64_t nbeta, const double* betas, double* A,
+                      double* B, double* F, const GradArgs* ga) {
     const int64_t n = mesh->nfaces;
     const int64_t nr = ndof + nbeta;
-    if (n == 0 || nomega == 0 || nr == 0) return MH_OK;
-    MH_REQUIRE(n <= 4096, "freq_sweep: at most 4096 panels per body");
-    MH_REQUIRE(nr <= 16, "freq_sweep: at most 16 dofs + headings");
-    MH_REQUIRE(omegas, "freq_sweep: null omegas");
-    MH_REQUIRE(ndof == 0 || (dof_fields && A && B), "freq_sweep: null dof/output array");
-    MH_REQUIRE(nbeta == 0 || (betas && F), "freq_sweep: null heading/output array");
-    for (int64_t w = 0; w < nomega; ++w)
-        MH_REQUIRE(omegas[w] > 0.0, "freq_sweep: frequencies must be positive");
-    Context* c = CTX(ctx);
-    // frequencies are processed in chunks that keep S and D/K under 32 GiB
     // Per frequency: S and [D | S bc] (direct) or [K | bc] (indirect), n x
     // (n + nr) column-major; the right-hand sides ride along as extra columns
-    // of the factorisation, which leaves them forward-substituted.
+    // of the factorisation, which leaves them forward-substituted.  Frequencies
+    // are processed in chunks that keep the matrices under 32 GiB.
     const int64_t ncols = n + nr;
     const int64_t ld = n, stride = n * ncols, ldb = n, bstride = n * nr;
     const size_t mat_aug = sizeof(double2) * (size_t)n * (size_t)ncols;
     int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_aug));
     if (chunk < 1) chunk = 1;
     if (chunk > nomega) chunk = nomega;
+    const int Pg = (ga && ga->nparam > 1) ? 4 : 1;  // tangent directions per gradient pass
+    const int64_t wa = n * (ndof > 0 ? ndof : 1);  // per-frequency stride of n x ndof blocks
     Bump bump;
     const size_t o_S = bump.take(mat_aug * chunk);
     const size_t o_D = bump.take(mat_aug * chunk);
@@ -386,6 +376,20 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
     const size_t o_k = bump.take(sizeof(double) * (size_t)nomega);
     const size_t o_be = bump.take(sizeof(double) * (size_t)(nbeta > 0 ? nbeta : 1));
     const size_t o_P = bump.take(sizeof(PanelRec) * (size_t)n);
+    size_t o_cw = 0, o_lam = 0, o_mu = 0, o_tan = 0, o_dk = 0, o_dom = 0, o_part = 0, o_T1 = 0, o_T2 = 0;
+    const int64_t tiles = (n + 63) / 64;
+    const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
+    if (ga) {
+        o_cw = bump.take(sizeof(double2) * (size_t)(wa * chunk));
+        o_lam = bump.take(sizeof(double2) * (size_t)(wa * chunk));
+        o_mu = bump.take(sizeof(double2) * (size_t)(wa * chunk));
+        o_tan = bump.take(sizeof(double) * (size_t)(n * 19 * Pg));
+        o_dk = bump.take(sizeof(double) * (size_t)(chunk * Pg));
+        o_dom = bump.take(sizeof(double) * (size_t)(chunk * Pg));
+        o_part = bump.take(sizeof(double2) * (size_t)(chunk * tiles * tiles * nout));
+        o_T1 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
+        o_T2 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
+    }
     if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
     char* ws = static_cast<char*>(c->ws);
     double2* dS = reinterpret_cast<double2*>(ws + o_S);
@@ -429,12 +433,159 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
         } else {
             TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, aug, ld, stride, phi, ldb, bstride));
         }
-        TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, phi, ldb, bstride,
-                          A ? A + w0 * ndof * ndof : nullptr, B ? B + w0 * ndof * ndof : nullptr,
-                          F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr));
+        double* Aw = A ? A + w0 * ndof * ndof : nullptr;
+        double* Bw = B ? B + w0 * ndof * ndof : nullptr;
+        double2* Fw = F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr;
+        TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, phi, ldb, bstride, Aw,
+                          Bw, Fw));
         TRY(check_info(c, info, nw));
+        if (!ga || ndof == 0) continue;
+        // ---- gradients (grad.hip) ----
+        double2* cw = reinterpret_cast<double2*>(ws + o_cw);
+        double2* lam = reinterpret_cast<double2*>(ws + o_lam);
+        double2* mu = reinterpret_cast<double2*>(ws + o_mu);
+        double* tan = reinterpret_cast<double*>(ws + o_tan);
+        double* d_dk = reinterpret_cast<double*>(ws + o_dk);
+        double* d_dom = reinterpret_cast<double*>(ws + o_dom);
+        double2* part = reinterpret_cast<double2*>(ws + o_part);
+        double2* T1 = reinterpret_cast<double2*>(ws + o_T1);
+        double2* T2 = reinterpret_cast<double2*>(ws + o_T2);
+        TRY(launch_force_weights(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, cw, wa));
+        const double2 *aS, *aDw, *sS, *sD, *rhoW;
+        int64_t wss;
+        if (direct) {  // lambda = D^-T c; mu = S^T lambda
+            MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
+            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+            TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
+            aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
+        } else {       // lambda = K^-T S^T c
+            TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
+            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+            aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
+        }
+        // outputs of this chunk, host side: F of every (dof, rhs) pair
+        std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof), hF(2 * nw * nbeta * ndof);
+        if (ndof) {
+            MH_HIP_CHECK(hipMemcpyAsync(hA.data(), Aw, sizeof(double) * hA.size(), hipMemcpyDeviceToHost, c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(hB.data(), Bw, sizeof(double) * hB.size(), hipMemcpyDeviceToHost, c->stream));
+        }
+        if (nbeta)
+            MH_HIP_CHECK(hipMemcpyAsync(hF.data(), Fw, sizeof(double2) * nw * nbeta * ndof, hipMemcpyDeviceToHost, c->stream));
+        const int64_t np = ga->nparam;
+        std::vector<double> gA(nw * np * ndof * ndof), gB(nw * np * ndof * ndof), gF(2 * nw * np * nbeta * ndof);
+        std::vector<double> hdk(nw * Pg), hdom(nw * Pg);
+        std::vector<double2> hT1(nw * nout), hT2(nw * nout);
+        for (int64_t m0 = 0; m0 < np; m0 += Pg) {
+            for (int64_t w = 0; w < nw; ++w)
+                for (int m = 0; m < Pg; ++m) {
+                    const double dw = (ga->domega && m0 + m < np) ? ga->domega[(m0 + m) * nomega + w0 + w] : 0.0;
+                    hdom[w * Pg + m] = dw;
+                    hdk[w * Pg + m] = 2.0 * omegas[w0 + w] / g * dw;
+                }
+            MH_HIP_CHECK(hipMemcpyAsync(d_dk, hdk.data(), sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(d_dom, hdom.data(), sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
+            TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
+            int64_t nblk = 0;
+            TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, gf_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
+                                     aDw, sS, sD, wa, wss, part, &nblk));
+            TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 0));
+            TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
+                                  ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
+            MH_HIP_CHECK(hipMemcpyAsync(hT1.data(), T1, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(hT2.data(), T2, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));
+            MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+            for (int64_t w = 0; w < nw; ++w) {
+                const double om = omegas[w0 + w];
+                for (int m = 0; m < Pg && m0 + m < np; ++m) {
+                    const double dw = hdom[w * Pg + m];
+                    for (int64_t d = 0; d < ndof; ++d)
+                        for (int64_t r = 0; r < nr; ++r) {
+                            const int64_t o = ((w * ndof + d) * nr + r) * Pg + m;
+                            const double fr = hT1[o].x + hT2[o].x, fi = hT1[o].y + hT2[o].y;
+                            if (r < ndof) {
+                                // A = Re F / omega^2, B = Im F / omega  (waves.jl:36)
+                                const int64_t ia = (w * ndof + r) * ndof + d;
+                                const double Fre = hA[ia] * om * om, Fim = hB[ia] * om;
+                                const int64_t og = (((w * np) + m0 + m) * ndof + r) * ndof + d;
+                                gA[og] = fr / (om * om) - 2.0 * Fre * dw / (om * om * om);
+                                gB[og] = fi / om - Fim * dw / (om * om);
+                            } else {
+                                const int64_t og = (((w * np) + m0 + m) * nbeta + (r - ndof)) * ndof + d;
+                                gF[2 * og] = fr;
+                                gF[2 * og + 1] = fi;
+                            }
+                        }
+                }
+            }
+        }
+        (void)hF;
+        if (ga->dAout)
+            MH_HIP_CHECK(hipMemcpy(ga->dAout + w0 * np * ndof * ndof, gA.data(), sizeof(double) * gA.size(), hipMemcpyHostToDevice));
+        if (ga->dBout)
+            MH_HIP_CHECK(hipMemcpy(ga->dBout + w0 * np * ndof * ndof, gB.data(), sizeof(double) * gB.size(), hipMemcpyHostToDevice));
+        if (ga->dFout && nbeta)
+            MH_HIP_CHECK(hipMemcpy(ga->dFout + 2 * w0 * np * nbeta * ndof, gF.data(), sizeof(double) * gF.size(), hipMemcpyHostToDevice));
     }
     return MH_OK;
+}
+
+static int sweep_checks(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int precision,
+                        int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
+                        const double* dof_fields, int64_t nbeta, const double* betas, double* A,
+                        double* B, double* F) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "freq_sweep: bad Green-function mask");
+    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
+               "freq_sweep: choose one wave term");
+    MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0, "freq_sweep: negative size");
+    MH_REQUIRE(g > 0.0 && rho > 0.0, "freq_sweep: g and rho must be positive");
+    if (precision != MH_FP64) {
+        set_error("freq_sweep: only MH_FP64 is available in this build");
+        return MH_ERR_NOT_SUPPORTED;
+    }
+    const int64_t n = mesh->nfaces;
+    const int64_t nr = ndof + nbeta;
+    if (n == 0 || nomega == 0 || nr == 0) return MH_OK;
+    MH_REQUIRE(n <= 4096, "freq_sweep: at most 4096 panels per body");
+    MH_REQUIRE(nr <= 16, "freq_sweep: at most 16 dofs + headings");
+    MH_REQUIRE(omegas, "freq_sweep: null omegas");
+    MH_REQUIRE(ndof == 0 || (dof_fields && A && B), "freq_sweep: null dof/output array");
+    MH_REQUIRE(nbeta == 0 || (betas && F), "freq_sweep: null heading/output array");
+    for (int64_t w = 0; w < nomega; ++w)
+        MH_REQUIRE(omegas[w] > 0.0, "freq_sweep: frequencies must be positive");
+    return MH_OK;
+}
+
+int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
+                  int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
+                  const double* dof_fields, int64_t nbeta, const double* betas, double* A,
+                  double* B, double* F) {
+    TRY(sweep_checks(ctx, mesh, gf_mask, precision, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
+                     betas, A, B, F));
+    if (mesh->nfaces == 0 || nomega == 0 || ndof + nbeta == 0) return MH_OK;
+    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
+                      betas, A, B, F, nullptr);
+}
+
+int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct,
+                       int precision, int64_t nomega, const double* omegas, double g, double rho,
+                       int64_t ndof, const double* dof_fields, int64_t nbeta, const double* betas,
+                       int64_t nparam, const double* dvertices, const double* dcenters,
+                       const double* dnormals, const double* dareas, const double* ddof_fields,
+                       const double* domegas, double* A, double* B, double* F, double* dA,
+                       double* dB, double* dF) {
+    TRY(sweep_checks(ctx, mesh, gf_mask, precision, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
+                     betas, A, B, F));
+    MH_REQUIRE(nparam >= 0, "freq_sweep_grad: negative nparam");
+    if (mesh->nfaces == 0 || nomega == 0 || ndof + nbeta == 0) return MH_OK;
+    MH_REQUIRE(ndof >= 1, "freq_sweep_grad: at least one dof (the influenced force)");
+    MH_REQUIRE(ndof + nbeta <= 4, "freq_sweep_grad: at most 4 dofs + headings");
+    MH_REQUIRE(!(gf_mask & MH_GF_DELHOMMEAU), "freq_sweep_grad: Delhommeau gradients not available");
+    MH_REQUIRE(nparam == 0 || (dA && dB && (nbeta == 0 || dF)), "freq_sweep_grad: null gradient output");
+    GradArgs ga{nparam, dvertices, dcenters, dnormals, dareas, ddof_fields, domegas, dA, dB, dF};
+    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
+                      betas, A, B, F, nparam > 0 ? &ga : nullptr);
 }
 
 // ---- host-array entry points -----------------------------------------------

@@ -1,0 +1,449 @@
+// Gradients of A, B and F with respect to P shape/frequency parameters.
+//
+// For one frequency, direct formulation (src/matrix_assembly.jl:125-126,
+// src/waves.jl:21-37): D phi^r = S b^r and F^{i,r} = c^i . phi^r with
+// c^i_p = i rho omega (-n_p . dof^i_p) a_p.  With the adjoint vector
+// lambda^i = D^{-T} c^i (one transposed solve with the LU factors),
+//   dF^{i,r} = (dc^i) . phi^r + (S^T lambda^i) . db^r
+//            + sum_pq lambda^i_p (dS_pq b^r_q - dD_pq phi^r_q).
+// Indirect (src/matrix_assembly.jl:127-131): K sigma^r = b^r, phi = S sigma,
+// lambda^i = K^{-T} S^T c^i and
+//   dF^{i,r} = (dc^i) . phi^r + lambda^i . db^r
+//            + sum_pq (c^i_p dS_pq - lambda^i_p dK_pq) sigma^r_q.
+// The influence matrices' derivatives dS, dD, dK are never stored: the pair
+// kernel recomputes every panel pair with Dual<P> arithmetic (the FP64 pair
+// code of mh_device.hpp run with P tangent directions) and contracts them on
+// the fly with the adjoint weights; the O(N) boundary-condition and force
+// terms are handled by a second dual kernel.  The derivative is that of the
+// code as written (the Wu fits, the branch thresholds), i.e. what automatic
+// differentiation of the reference gives and what central finite
+// differences of the forward path converge to.
+#include <vector>
+
+#include "mh_internal.hpp"
+
+namespace mh {
+
+constexpr int kTanFields = 19;  // dc[3], dn[3], da, dV[4][3]
+
+// Tangent record of panel q for parameters [m0, m0+P): tan[(q*19 + f)*P + m].
+template <int P>
+__global__ void k_pack_tangents(int64_t nf, int64_t nv, int64_t nparam, int64_t m0,
+                                const int64_t* __restrict__ F, const double* __restrict__ dV,
+                                const double* __restrict__ dC, const double* __restrict__ dN,
+                                const double* __restrict__ dA, double* __restrict__ tan) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nf) return;
+    double* t = tan + q * kTanFields * P;
+    for (int m = 0; m < P; ++m) {
+        const int64_t pm = m0 + m;
+        const bool ok = pm < nparam;
+        for (int c = 0; c < 3; ++c) {
+            t[c * P + m] = ok && dC ? dC[pm * nf * 3 + q + c * nf] : 0.0;
+            t[(3 + c) * P + m] = ok && dN ? dN[pm * nf * 3 + q + c * nf] : 0.0;
+        }
+        t[6 * P + m] = ok && dA ? dA[pm * nf + q] : 0.0;
+        for (int v = 0; v < 4; ++v) {
+            int64_t iv = F[q + v * nf];
+            iv = (iv < 0 || iv >= nv) ? 0 : iv;
+            for (int c = 0; c < 3; ++c)
+                t[(7 + 3 * v + c) * P + m] = ok && dV ? dV[pm * nv * 3 + iv + c * nv] : 0.0;
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ Dual<P> tangent_of(double v, const double* t, int field) {
+    Dual<P> r;
+    r.v = v;
+#pragma unroll
+    for (int m = 0; m < P; ++m) r.d[m] = t[field * P + m];
+    return r;
+}
+
+__device__ __forceinline__ double2 cmac(double2 acc, double ar, double ai, double2 b) {
+    // acc + (ar + i ai) * b
+    return make_double2(acc.x + ar * b.x - ai * b.y, acc.y + ar * b.y + ai * b.x);
+}
+
+// ---------------------------------------------------------------------------
+// pair tangent-contraction kernel
+//   part[w][blk][i][r][m] = sum_{p in tile rows, q in tile cols}
+//       aS^i_p dS_pq,m sS^r_q - aD^i_p dDK_pq,m sD^r_q
+// FORM 1: DK = D (source normal, gradient w.r.t. the source point);
+// FORM 2: DK = K (collocation normal, gradient w.r.t. the collocation point).
+// ---------------------------------------------------------------------------
+template <int WAVE, int FORM, int P, int NR>
+__global__ __launch_bounds__(256) void k_tangent_pairs(
+    int64_t n, const double* __restrict__ C, const double* __restrict__ N,
+    const PanelRec* __restrict__ panels, const double* __restrict__ tan, int gf,
+    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr,
+    const double2* __restrict__ aS, const double2* __restrict__ aD, const double2* __restrict__ sS,
+    const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s,
+    double2* __restrict__ part) {
+    typedef Dual<P> D_;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t w = blockIdx.z;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double* ti = tan + ic * kTanFields * P;
+    const D_ x0 = tangent_of<P>(C[ic], ti, 0);
+    const D_ x1 = tangent_of<P>(C[ic + n], ti, 1);
+    const D_ x2 = tangent_of<P>(C[ic + 2 * n], ti, 2);
+    D_ ni[3];
+    if (FORM == 2) {
+        ni[0] = tangent_of<P>(N[ic], ti, 3);
+        ni[1] = tangent_of<P>(N[ic + n], ti, 4);
+        ni[2] = tangent_of<P>(N[ic + 2 * n], ti, 5);
+    }
+    D_ k;
+    k.v = ks[w];
+#pragma unroll
+    for (int m = 0; m < P; ++m) k.d[m] = dks[w * P + m];
+    const double2* sSw = sS + w * wstride_s;
+    const double2* sDw = sD + w * wstride_s;
+    double2 RS[NR][P], RD[NR][P];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            RS[r][m] = make_double2(0.0, 0.0);
+            RD[r][m] = make_double2(0.0, 0.0);
+        }
+    const int64_t jb = (int64_t)blockIdx.y * 64;
+    for (int jj = wave; jj < 64; jj += 4) {
+        const int64_t j = jb + jj;
+        if (j >= n) break;
+        const PanelRec& pr = panels[j];
+        const double* tj = tan + j * kTanFields * P;
+        PanelT<D_> p;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p.c[c] = tangent_of<P>(pr.c[c], tj, c);
+            p.n[c] = tangent_of<P>(pr.n[c], tj, 3 + c);
+        }
+        p.area = tangent_of<P>(pr.area, tj, 6);
+        p.radius = pr.radius;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p.v[v][c] = tangent_of<P>(pr.v[v][c], tj, 7 + 3 * v + c);
+        D_ IR, gxiR[3], gxR[3];
+        rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        cplxT<D_> Sv, gxi[3], gx[3];
+        if (WAVE) {
+            wave_term<WAVE>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], p.area, Sv, gxi, gx);
+        } else {
+            Sv.re = Sv.im = dconst<P>(0.0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gxi[c].re = gxi[c].im = gx[c].re = gx[c].im = dconst<P>(0.0);
+        }
+        // S = -1/(4 pi) (IR + Sv);  DK = delta/2 - 1/(4 pi) n . g
+        const D_ s_re = IR + Sv.re;
+        const D_& s_im = Sv.im;
+        D_ k_re, k_im;
+        if (FORM == 1) {
+            k_re = p.n[0] * (gxiR[0] + gxi[0].re) + p.n[1] * (gxiR[1] + gxi[1].re) +
+                   p.n[2] * (gxiR[2] + gxi[2].re);
+            k_im = p.n[0] * gxi[0].im + p.n[1] * gxi[1].im + p.n[2] * gxi[2].im;
+        } else {
+            k_re = ni[0] * (gxR[0] + gx[0].re) + ni[1] * (gxR[1] + gx[1].re) +
+                   ni[2] * (gxR[2] + gx[2].re);
+            k_im = ni[0] * gx[0].im + ni[1] * gx[1].im + ni[2] * gx[2].im;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r < nr) {
+                const double2 ss = sSw[j + r * n];
+                const double2 sd = sDw[j + r * n];
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    RS[r][m] = cmac(RS[r][m], -kInv4Pi * s_re.d[m], -kInv4Pi * s_im.d[m], ss);
+                    RD[r][m] = cmac(RD[r][m], -kInv4Pi * k_re.d[m], -kInv4Pi * k_im.d[m], sd);
+                }
+            }
+        }
+    }
+    // contract with the left weights and reduce over the workgroup
+    __shared__ double2 red[4][NR * P];
+    const int64_t nblk = (int64_t)gridDim.x * gridDim.y;
+    const int64_t blk = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const double2* aSw = aS + w * wstride_a;
+    const double2* aDw = aD + w * wstride_a;
+    for (int64_t d = 0; d < ndof; ++d) {
+        const double2 as = row_ok ? aSw[i + d * n] : make_double2(0.0, 0.0);
+        const double2 ad = row_ok ? aDw[i + d * n] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r >= nr) continue;
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                double vx = as.x * RS[r][m].x - as.y * RS[r][m].y - (ad.x * RD[r][m].x - ad.y * RD[r][m].y);
+                double vy = as.x * RS[r][m].y + as.y * RS[r][m].x - (ad.x * RD[r][m].y + ad.y * RD[r][m].x);
+                for (int off = 32; off > 0; off >>= 1) {
+                    vx += __shfl_xor(vx, off);
+                    vy += __shfl_xor(vy, off);
+                }
+                if (lane == 0) red[wave][r * P + m] = make_double2(vx, vy);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    const double2 a = red[0][r * P + m], b = red[1][r * P + m],
+                                  c2 = red[2][r * P + m], e = red[3][r * P + m];
+                    part[(((w * nblk + blk) * ndof + d) * nr + r) * P + m] =
+                        make_double2(a.x + b.x + c2.x + e.x, a.y + b.y + c2.y + e.y);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// out[w][d][r][m] (+)= sum over blocks of part
+__global__ void k_reduce_partials(int64_t nw, int64_t nblk, int64_t nout, const double2* __restrict__ part,
+                                  double2* __restrict__ out, int accumulate) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw * nout) return;
+    const int64_t w = t / nout, o = t % nout;
+    double sx = 0.0, sy = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) {
+        const double2 v = part[(w * nblk + b) * nout + o];
+        sx += v.x;
+        sy += v.y;
+    }
+    if (accumulate) {
+        out[t].x += sx;
+        out[t].y += sy;
+    } else {
+        out[t] = make_double2(sx, sy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// force weights c^i_p = i rho omega (-n_p . dof^i_p) a_p  (src/waves.jl:21-25, :34)
+// ---------------------------------------------------------------------------
+__global__ void k_force_weights(int64_t n, const double* __restrict__ N, const double* __restrict__ areas,
+                                const double* __restrict__ omega, double rho, int64_t ndof,
+                                const double* __restrict__ dofs, double2* __restrict__ cw,
+                                int64_t wstride) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t w = blockIdx.y;
+    if (p >= n) return;
+    const double om = omega[w];
+    for (int64_t d = 0; d < ndof; ++d) {
+        const double* f = dofs + d * n * 3;
+        const double amp = -(f[p] * N[p] + f[p + n] * N[p + n] + f[p + 2 * n] * N[p + 2 * n]);
+        cw[w * wstride + p + d * n] = make_double2(0.0, rho * om * amp * areas[p]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// O(N) terms: T1[w][d][r][m] = sum_p dc^d_p,m phi^r_p + rhoW^d_p db^r_p,m
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ void dual_sincos(const Dual<P>& x, Dual<P>& s, Dual<P>& c) {
+    double sv, cv;
+    ::sincos(x.v, &sv, &cv);
+    s = chain(x, sv, cv);
+    c = chain(x, cv, -sv);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_bc_tangent(
+    int64_t n, const double* __restrict__ C, const double* __restrict__ N,
+    const double* __restrict__ areas, const double* __restrict__ tan,
+    const double* __restrict__ omega, const double* __restrict__ domega, double g, double rho,
+    int64_t ndof, const double* __restrict__ dofs, const double* __restrict__ ddofs, int64_t nparam,
+    int64_t m0, int64_t nbeta, const double* __restrict__ beta, const double2* __restrict__ phi,
+    int64_t bstride, const double2* __restrict__ rhoW, int64_t wstride,
+    double2* __restrict__ out /* [w][d][r][m] */) {
+    typedef Dual<P> D_;
+    const int64_t w = blockIdx.x;
+    const int64_t nr = ndof + nbeta;
+    D_ om;
+    om.v = omega[w];
+#pragma unroll
+    for (int m = 0; m < P; ++m) om.d[m] = domega[w * P + m];  // d omega_w / d theta_(m0+m)
+    const D_ k = om * om / g;
+    __shared__ double2 red[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t d = 0; d < ndof; ++d)
+        for (int64_t r = 0; r < nr; ++r)
+            for (int m = 0; m < P; ++m) {
+                double ax = 0.0, ay = 0.0;
+                for (int64_t p = threadIdx.x; p < n; p += 256) {
+                    const double* t = tan + p * kTanFields * P;
+                    D_ nn[3], x[3];
+                    for (int c = 0; c < 3; ++c) {
+                        x[c] = tangent_of<P>(C[p + c * n], t, c);
+                        nn[c] = tangent_of<P>(N[p + c * n], t, 3 + c);
+                    }
+                    const D_ a = tangent_of<P>(areas[p], t, 6);
+                    // dof fields (tangents given per parameter, or zero)
+                    D_ fd[3];
+                    for (int c = 0; c < 3; ++c) {
+                        fd[c].v = dofs[d * n * 3 + p + c * n];
+                        for (int q = 0; q < P; ++q)
+                            fd[c].d[q] = (ddofs && m0 + q < nparam)
+                                             ? ddofs[((m0 + q) * ndof + d) * n * 3 + p + c * n]
+                                             : 0.0;
+                    }
+                    // c^d_p = i rho omega (-n . dof^d) a : imaginary part only
+                    const D_ cim = (rho * om) * (-(nn[0] * fd[0] + nn[1] * fd[1] + nn[2] * fd[2])) * a;
+                    // b^r_p
+                    D_ bre, bim;
+                    if (r < ndof) {
+                        D_ fr[3];
+                        for (int c = 0; c < 3; ++c) {
+                            fr[c].v = dofs[r * n * 3 + p + c * n];
+                            for (int q = 0; q < P; ++q)
+                                fr[c].d[q] = (ddofs && m0 + q < nparam)
+                                                 ? ddofs[((m0 + q) * ndof + r) * n * 3 + p + c * n]
+                                                 : 0.0;
+                        }
+                        const D_ nd = nn[0] * fr[0] + nn[1] * fr[1] + nn[2] * fr[2];
+                        bre = dconst<P>(0.0);
+                        bim = -(om * nd);
+                    } else {
+                        const double bb = beta[r - ndof];
+                        const double cb = ::cos(bb), sb = ::sin(bb);
+                        const D_ wbar = x[0] * cb + x[1] * sb;
+                        const D_ cih = exp(k * x[2]);
+                        const D_ amp = (g * k) / om;
+                        D_ sn, cs;
+                        dual_sincos(k * wbar, sn, cs);
+                        const D_ pr = amp * cs, pi = amp * sn;
+                        const D_ vre = pr * (cb * cih) * nn[0] + pr * (sb * cih) * nn[1] + pi * cih * nn[2];
+                        const D_ vim = pi * (cb * cih) * nn[0] + pi * (sb * cih) * nn[1] - pr * cih * nn[2];
+                        bre = -vre;
+                        bim = -vim;
+                    }
+                    const double2 ph = phi[w * bstride + p + r * n];
+                    const double2 rw = rhoW[w * wstride + p + d * n];
+                    // dc . phi  (dc = i dcim)
+                    ax += -cim.d[m] * ph.y;
+                    ay += cim.d[m] * ph.x;
+                    // rhoW . db
+                    ax += rw.x * bre.d[m] - rw.y * bim.d[m];
+                    ay += rw.x * bim.d[m] + rw.y * bre.d[m];
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    ax += __shfl_xor(ax, off);
+                    ay += __shfl_xor(ay, off);
+                }
+                if (lane == 0) red[wave] = make_double2(ax, ay);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    out[((w * ndof + d) * nr + r) * P + m] =
+                        make_double2(red[0].x + red[1].x + red[2].x + red[3].x,
+                                     red[0].y + red[1].y + red[2].y + red[3].y);
+                }
+                __syncthreads();
+            }
+}
+
+// ---------------------------------------------------------------------------
+// host driver pieces
+// ---------------------------------------------------------------------------
+template <int WAVE, int FORM, int P, int NR>
+static void launch_tp(dim3 grid, hipStream_t st, int64_t n, const double* C, const double* N,
+                      const PanelRec* P_, const double* tan, int gf, const double* ks, const double* dks,
+                      int64_t ndof, int64_t nr, const double2* aS, const double2* aD, const double2* sS,
+                      const double2* sD, int64_t wa, int64_t ws, double2* part) {
+    hipLaunchKernelGGL((k_tangent_pairs<WAVE, FORM, P, NR>), grid, dim3(256), 0, st, n, C, N, P_, tan, gf,
+                       ks, dks, ndof, nr, aS, aD, sS, sD, wa, ws, part);
+}
+
+int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, const double* tan,
+                         int P, int gf, int form, int64_t nw, const double* d_k, const double* d_dk,
+                         int64_t ndof, int64_t nr, const double2* aS, const double2* aD,
+                         const double2* sS, const double2* sD, int64_t wa, int64_t ws, double2* part,
+                         int64_t* nblk_out) {
+    const int64_t n = m->nfaces;
+    const int64_t tiles = (n + 63) / 64;
+    *nblk_out = tiles * tiles;
+    dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)nw);
+    MH_REQUIRE(nr <= 4, "gradients: at most 4 dofs + headings per pass");
+    ProfScope ps(ctx, "tangent_pairs", 0.0, 0.0);
+    const bool wu = gf & GF_WU;
+    const bool nr2 = nr <= 2;
+#define TP(WV, FM, PP, NRR)                                                                        \
+    launch_tp<WV, FM, PP, NRR>(grid, ctx->stream, n, m->centers, m->normals, panels, tan, gf, d_k, \
+                               d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)
+    if (P == 1) {
+        if (wu) {
+            if (form == 1) { if (nr2) TP(1, 1, 1, 2); else TP(1, 1, 1, 4); }
+            else { if (nr2) TP(1, 2, 1, 2); else TP(1, 2, 1, 4); }
+        } else {
+            if (form == 1) { if (nr2) TP(0, 1, 1, 2); else TP(0, 1, 1, 4); }
+            else { if (nr2) TP(0, 2, 1, 2); else TP(0, 2, 1, 4); }
+        }
+    } else {
+        if (wu) {
+            if (form == 1) { if (nr2) TP(1, 1, 4, 2); else TP(1, 1, 4, 4); }
+            else { if (nr2) TP(1, 2, 4, 2); else TP(1, 2, 4, 4); }
+        } else {
+            if (form == 1) { if (nr2) TP(0, 1, 4, 2); else TP(0, 1, 4, 4); }
+            else { if (nr2) TP(0, 2, 4, 2); else TP(0, 2, 4, 4); }
+        }
+    }
+#undef TP
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_pack_tangents(Context* ctx, const mh_mesh_t* m, int P, int64_t nparam, int64_t m0,
+                         const double* dV, const double* dC, const double* dN, const double* dA,
+                         double* tan) {
+    const int64_t nf = m->nfaces;
+    dim3 grid((unsigned)((nf + 127) / 128)), blk(128);
+    if (P == 1)
+        hipLaunchKernelGGL(k_pack_tangents<1>, grid, blk, 0, ctx->stream, nf, m->nvertices, nparam, m0,
+                           m->faces, dV, dC, dN, dA, tan);
+    else
+        hipLaunchKernelGGL(k_pack_tangents<4>, grid, blk, 0, ctx->stream, nf, m->nvertices, nparam, m0,
+                           m->faces, dV, dC, dN, dA, tan);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_force_weights(Context* ctx, const mh_mesh_t* m, int64_t nw, const double* d_om, double rho,
+                         int64_t ndof, const double* dofs, double2* cw, int64_t wstride) {
+    const int64_t n = m->nfaces;
+    hipLaunchKernelGGL(k_force_weights, dim3((unsigned)((n + 255) / 256), (unsigned)nw), dim3(256), 0,
+                       ctx->stream, n, m->normals, m->areas, d_om, rho, ndof, dofs, cw, wstride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan, int64_t nw,
+                      const double* d_om, const double* d_dom, double g, double rho, int64_t ndof,
+                      const double* dofs, const double* ddofs, int64_t nparam, int64_t m0,
+                      int64_t nbeta, const double* d_beta, const double2* phi, int64_t bstride,
+                      const double2* rhoW, int64_t wstride, double2* out) {
+    const int64_t n = m->nfaces;
+    ProfScope ps(ctx, "bc_tangent", 0.0, 0.0);
+    if (P == 1)
+        hipLaunchKernelGGL(k_bc_tangent<1>, dim3((unsigned)nw), dim3(256), 0, ctx->stream, n, m->centers,
+                           m->normals, m->areas, tan, d_om, d_dom, g, rho, ndof, dofs, ddofs, nparam, m0,
+                           nbeta, d_beta, phi, bstride, rhoW, wstride, out);
+    else
+        hipLaunchKernelGGL(k_bc_tangent<4>, dim3((unsigned)nw), dim3(256), 0, ctx->stream, n, m->centers,
+                           m->normals, m->areas, tan, d_om, d_dom, g, rho, ndof, dofs, ddofs, nparam, m0,
+                           nbeta, d_beta, phi, bstride, rhoW, wstride, out);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout, const double2* part,
+                           double2* out, int accumulate) {
+    const int64_t tot = nw * nout;
+    if (tot == 0) return MH_OK;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, nw,
+                       nblk, nout, part, out, accumulate);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+}  // namespace mh

@@ -119,4 +119,23 @@ int launch_forces(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, const dou
                   const double2* phi, int64_t ldb, int64_t bstride, double* A, double* B,
                   double2* F);
 
+// ---- launchers (grad.hip) ---------------------------------------------------
+int launch_pack_tangents(Context* ctx, const mh_mesh_t* m, int P, int64_t nparam, int64_t m0,
+                         const double* dV, const double* dC, const double* dN, const double* dA,
+                         double* tan);
+int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, const double* tan,
+                         int P, int gf, int form, int64_t nw, const double* d_k, const double* d_dk,
+                         int64_t ndof, int64_t nr, const double2* aS, const double2* aD,
+                         const double2* sS, const double2* sD, int64_t wa, int64_t ws, double2* part,
+                         int64_t* nblk_out);
+int launch_force_weights(Context* ctx, const mh_mesh_t* m, int64_t nw, const double* d_om, double rho,
+                         int64_t ndof, const double* dofs, double2* cw, int64_t wstride);
+int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan, int64_t nw,
+                      const double* d_om, const double* d_dom, double g, double rho, int64_t ndof,
+                      const double* dofs, const double* ddofs, int64_t nparam, int64_t m0,
+                      int64_t nbeta, const double* d_beta, const double2* phi, int64_t bstride,
+                      const double2* rhoW, int64_t wstride, double2* out);
+int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout, const double2* part,
+                           double2* out, int accumulate);
+
 }  // namespace mh

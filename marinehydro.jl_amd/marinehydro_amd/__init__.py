@@ -12,7 +12,8 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   assemble_matrices_batched, assemble_matrix_wu, solve, radiation_bc,
                   integrate_pressure, airy_waves_potential, airy_waves_velocity,
                   airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep,
-                  calculate_radiation_forces, DiffractionForce)
+                  calculate_radiation_forces, DiffractionForce, Tangents, shape_tangents,
+                  freq_sweep_grad)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
                      mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
                      translational_dofs, rotational_dofs, rigid_dofs)

@@ -55,6 +55,9 @@ _SIGS = {
     "mh_solve": (_int, [_vp, _i64, _i64, _int, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
     "mh_freq_sweep": (_int, [_vp, _vp, _int, _int, _int, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp,
                              _vp, _vp, _vp]),
+    "mh_freq_sweep_grad": (_int, [_vp, _vp, _int, _int, _int, _i64, _vp, _d, _d, _i64, _vp, _i64,
+                                  _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp, _vp]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   _d, _d, _d, _vp]),
     "mh_diffraction_force_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,

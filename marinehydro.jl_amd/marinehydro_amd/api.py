@@ -453,3 +453,114 @@ def DiffractionForce(mesh, omega, dof):
     """src/waves.jl:40-52: complex diffraction force, heading 0 (Wu, direct)."""
     _, _, F = freq_sweep(mesh, [omega], [dof], betas=[0.0])
     return complex(F[0, 0, 0])
+
+
+# ---------------------------------------------------------------------------
+# gradients (adjoint of the solve, tangents of the geometry)
+# ---------------------------------------------------------------------------
+@dataclass
+class Tangents:
+    """Derivatives of the mesh fields (and frequencies) along nparam
+    parameter directions.  Arrays are (nparam, n, 3) / (nparam, n); any may
+    be None (zero).  domegas: (nparam, nomega) or None."""
+    nparam: int
+    dvertices: np.ndarray = None
+    dcenters: np.ndarray = None
+    dnormals: np.ndarray = None
+    dareas: np.ndarray = None
+    ddof_fields: np.ndarray = None  # (nparam, ndof, nfaces, 3)
+    domegas: np.ndarray = None
+
+
+def shape_tangents(mesh, dvertices_list, nomega=0, omega_param=None, dof_axes=None):
+    """Tangents of a mesh whose face quantities are functions of its vertices
+    (the synthetic bodies): dvertices_list[m] = d vertices / d theta_m (None
+    for a parameter that is not geometric).  omega_param: index m of the
+    frequency parameter, if any.  dof_axes: per dof, None for a translation or
+    the rotation axis of a rotational dof (its field depends on the centres).
+    """
+    from .meshes import face_geometry_jvp
+
+    npar = len(dvertices_list)
+    nv, nf = mesh.nvertices, mesh.nfaces
+    dV = np.zeros((npar, nv, 3))
+    dC = np.zeros((npar, nf, 3))
+    dN = np.zeros((npar, nf, 3))
+    dA = np.zeros((npar, nf))
+    for m, dv in enumerate(dvertices_list):
+        if dv is None:
+            continue
+        dV[m] = dv
+        dC[m], dN[m], dA[m] = face_geometry_jvp(mesh.vertices, mesh.faces, dv)
+    ddof = None
+    if dof_axes is not None and any(a is not None for a in dof_axes):
+        ddof = np.zeros((npar, len(dof_axes), nf, 3))
+        for d, ax in enumerate(dof_axes):
+            if ax is not None:
+                for m in range(npar):
+                    ddof[m, d] = np.cross(np.asarray(ax, dtype=np.float64), dC[m])
+    dom = None
+    if omega_param is not None:
+        dom = np.zeros((npar, nomega))
+        dom[omega_param, :] = 1.0
+    return Tangents(npar, dV, dC, dN, dA, ddof, dom)
+
+
+def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = None,
+                    green_functions=None, direct=True, precision="fp64", device=0, g=None, rho=None):
+    """freq_sweep plus derivatives along the tangent directions.
+
+    Returns (A, B, F, dA, dB, dF): A, B, F as freq_sweep; dA[w, m, i, j],
+    dB[w, m, i, j], dF[w, m, b, i] = derivatives along parameter m.
+    """
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    n, nv = dm.nfaces, dm.mesh.nvertices
+    gfs = green_functions if green_functions is not None else (Rankine(), RankineReflected(), GFWu())
+    om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
+    be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
+    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+    ndof, nbeta, nw = len(fields), be.shape[0], om.shape[0]
+    T = tangents
+    npar = T.nparam
+    dev = f"cuda:{dm.device}"
+
+    def up(a, shape):  # (nparam, n, c) -> column-major per parameter
+        if a is None:
+            return None
+        a = np.asarray(a, dtype=np.float64).reshape(shape)
+        if a.ndim == 3:
+            a = np.transpose(a, (0, 2, 1))
+        elif a.ndim == 4:
+            a = np.transpose(a, (0, 1, 3, 2))
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    dV = up(T.dvertices, (npar, nv, 3))
+    dC = up(T.dcenters, (npar, n, 3))
+    dN = up(T.dnormals, (npar, n, 3))
+    dA_ = up(T.dareas, (npar, n))
+    ddof = up(T.ddof_fields, (npar, ndof, n, 3))
+    dom = None if T.domegas is None else np.ascontiguousarray(np.asarray(T.domegas, dtype=np.float64).reshape(npar, nw))
+    dofs = torch.from_numpy(np.ascontiguousarray(np.stack([f.T for f in fields]))).to(dev)
+    A = torch.zeros((nw, ndof, ndof), dtype=torch.float64, device=dev)
+    B = torch.zeros_like(A)
+    F = torch.zeros((nw, max(nbeta, 1), ndof), dtype=torch.complex128, device=dev)
+    gA = torch.zeros((nw, max(npar, 1), ndof, ndof), dtype=torch.float64, device=dev)
+    gB = torch.zeros_like(gA)
+    gF = torch.zeros((nw, max(npar, 1), max(nbeta, 1), ndof), dtype=torch.complex128, device=dev)
+    ptr = lambda t: t.data_ptr() if t is not None else None
+    prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
+    ctx = L.context(dm.device)
+    L.check(L.lib().mh_freq_sweep_grad(
+        ctx, C.byref(dm.struct), gf_mask(gfs), int(bool(direct)), prec, nw, om.ctypes.data,
+        float(g or SETTINGS.g), float(rho or SETTINGS.rho), ndof, dofs.data_ptr(), nbeta,
+        be.ctypes.data if nbeta else None, npar, ptr(dV), ptr(dC), ptr(dN), ptr(dA_), ptr(ddof),
+        dom.ctypes.data if dom is not None else None, A.data_ptr(), B.data_ptr(), F.data_ptr(),
+        gA.data_ptr(), gB.data_ptr(), gF.data_ptr()), "freq_sweep_grad")
+    A_ = A.transpose(1, 2).cpu().numpy()
+    B_ = B.transpose(1, 2).cpu().numpy()
+    F_ = F.cpu().numpy()[:, :nbeta, :]
+    gA_ = gA.transpose(2, 3).cpu().numpy()[:, :npar]
+    gB_ = gB.transpose(2, 3).cpu().numpy()[:, :npar]
+    gF_ = gF.cpu().numpy()[:, :npar, :nbeta, :]
+    return A_, B_, F_, gA_, gB_, gF_

@@ -1,0 +1,123 @@
+"""GPU gradients of A, B, F (adjoint solve + geometry tangents) against
+central finite differences of the FP64 CPU oracle.
+
+The reference's own criterion for its gradients is agreement with central
+finite differences at atol = rtol = 1e-6 (test/outputs_differentiation.jl:
+64-67, 139-150).  Here the step is h = 1e-5 (relative for lengths), the
+finite differences come from the oracle on regenerated meshes, and the
+tolerance is 1e-6 relative to the largest derivative of each quantity.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GF_WU = 1 | 2 | 8
+
+
+def pair_body(mh, r1, r2, d, res):
+    """Two floating hemispheres, centres (0,0,0) and (d,0,0)
+    (the adjoint-gradient configuration); d vertices / d (r1, r2, d)."""
+    b1 = mh.hemisphere(r1, center=(0.0, 0.0, 0.0), ntheta=res, nphi=res)
+    b2 = mh.hemisphere(r2, center=(d, 0.0, 0.0), ntheta=res, nphi=res)
+    mesh = mh.combine_meshes([b1.mesh, b2.mesh])
+    nv1, nv2 = b1.mesh.nvertices, b2.mesh.nvertices
+    z1, z2 = np.zeros((nv1, 3)), np.zeros((nv2, 3))
+    dV = [np.vstack([b1.dvertices["radius"], z2]), np.vstack([z1, b2.dvertices["radius"]]),
+          np.vstack([z1, b2.dvertices["x"]])]
+    return mesh, dV
+
+
+def fd(oracle, make, theta, m, omega_of, dofs_of, betas, direct, h, rho=1025.0):
+    """central difference of the oracle's (A, B, F) along parameter m"""
+    out = []
+    for s in (+1, -1):
+        t = list(theta)
+        t[m] += s * h
+        mesh = make(t)
+        A, B, F = oracle.freq_solve(GF_WU, mesh, omega_of(t), dofs_of(mesh), betas, direct=direct,
+                                    rho=rho)
+        out.append((A, B, F))
+    return [(out[0][q] - out[1][q]) / (2 * h) for q in range(3)]
+
+
+def check(got, ref, tol, what, value=None):
+    """|got - ref| <= tol * max|ref|, or, for derivatives that vanish (a
+    translation of a single body leaves A and B unchanged), below the
+    finite-difference noise floor 1e-9 * max|value| per unit step."""
+    err = np.max(np.abs(got - ref))
+    scale = np.max(np.abs(ref))
+    floor = 1e-9 * np.max(np.abs(value)) if value is not None else 0.0
+    assert err <= tol * scale or err <= floor, (what, err, scale, floor, got, ref)
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_single_hemisphere_radius_position_omega(mh, oracle, gpu, direct):
+    res, omega = 10, 1.03
+    theta0 = [1.0, 0.0, omega]  # radius, x, omega
+
+    def make(t):
+        return mh.hemisphere(t[0], center=(t[1], 0.0, 0.0), ntheta=res, nphi=res).mesh
+
+    body = mh.hemisphere(1.0, ntheta=res, nphi=res)
+    mesh = body.mesh
+    dofs_of = lambda m: mh.rigid_dofs(m, ["Heave", "Surge"])
+    betas = [0.0]
+    T = mh.shape_tangents(mesh, [body.dvertices["radius"], body.dvertices["x"], None], nomega=1,
+                          omega_param=2)
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(mesh, [omega], dofs_of(mesh), betas, T, direct=direct,
+                                             rho=1025.0)
+    Ao, Bo, Fo = oracle.freq_solve(GF_WU, mesh, omega, dofs_of(mesh), betas, direct=direct, rho=1025.0)
+    check(A[0], Ao, 1e-10, "A")
+    check(F[0].T, Fo, 1e-10, "F")
+    for m in range(3):
+        gA, gB, gF = fd(oracle, make, theta0, m, lambda t: t[2], dofs_of, betas, direct, 1e-5)
+        check(dA[0, m], gA, 1e-6, ("dA", m), A)
+        check(dB[0, m], gB, 1e-6, ("dB", m), B)
+        check(dF[0, m].T, gF, 1e-6, ("dF", m), F)
+
+
+def test_hemisphere_pair_seeded(mh, oracle, gpu):
+    """pair of heave hemispheres, theta = (r1, r2, d, omega) drawn from a seed
+    (r ~ U[0.5, 2], d ~ U[3, 10] redrawn until the bodies are 0.5 m apart,
+    omega ~ U[0.5, 2])"""
+    rng = np.random.default_rng(2024)
+    while True:
+        r1, r2 = rng.uniform(0.5, 2.0, 2)
+        d = rng.uniform(3.0, 10.0)
+        if d > r1 + r2 + 0.5:
+            break
+    omega = rng.uniform(0.5, 2.0)
+    res = 8
+    theta0 = [r1, r2, d, omega]
+    make = lambda t: pair_body(mh, t[0], t[1], t[2], res)[0]
+    mesh, dV = pair_body(mh, r1, r2, d, res)
+    heave = np.zeros((mesh.nfaces, 3))
+    heave[:, 2] = 1.0
+    dofs_of = lambda m: [np.tile([0, 0, 1.0], (m.nfaces, 1))]
+    T = mh.shape_tangents(mesh, dV + [None], nomega=1, omega_param=3)
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(mesh, [omega], [heave], [0.0], T, rho=1025.0)
+    for m in range(4):
+        gA, gB, gF = fd(oracle, make, theta0, m, lambda t: t[3], dofs_of, [0.0], True,
+                        1e-5 * max(1.0, abs(theta0[m])))
+        check(dA[0, m], gA, 1e-6, ("dA", m), A)
+        check(dB[0, m], gB, 1e-6, ("dB", m), B)
+        check(dF[0, m].T, gF, 1e-6, ("dF", m), F)
+
+
+def test_rotational_dof_and_several_frequencies(mh, oracle, gpu):
+    # pitch about an offset centre: the dof field moves with the centres
+    res = 8
+    rc = np.array([0.3, 0.2, 0.0])
+    body = mh.hemisphere(1.2, ntheta=res, nphi=res)
+    mesh = body.mesh
+    omegas = [0.6, 1.4]
+    dofs_of = lambda m: mh.rigid_dofs(m, ["Heave", "Pitch"], rotation_center=rc)
+    T = mh.shape_tangents(mesh, [body.dvertices["radius"]], dof_axes=[None, [0, 1.0, 0]])
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(mesh, omegas, dofs_of(mesh), [0.4], T, rho=1025.0)
+    make = lambda t: mh.hemisphere(t[0], ntheta=res, nphi=res).mesh
+    for w, om in enumerate(omegas):
+        gA, gB, gF = fd(oracle, make, [1.2], 0, lambda t: om, dofs_of, [0.4], True, 1e-5)
+        check(dA[w, 0], gA, 1e-6, ("dA", w), A)
+        check(dB[w, 0], gB, 1e-6, ("dB", w), B)
+        check(dF[w, 0].T, gF, 1e-6, ("dF", w), F)
