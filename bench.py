@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--nomega", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=2,
                     help="frequencies timed on the host oracle (0 = skip)")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="forward sweep only (no adjoint gradients)")
     ap.add_argument("--profile-only", action="store_true",
                     help="one sweep, no timing loop (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -66,10 +68,20 @@ def main():
     dofs = [np.array([0.0, 0.0, 1.0])]
     betas = [0.0]
     dm = mh.DeviceMesh(mesh)
-    log(f"bench: N={n} panels, {args.nomega} frequencies, precision={args.precision}")
+    grad = not args.no_grad
+    # gradient parameters: theta = (radius, omega)
+    T = mh.shape_tangents(mesh, [body.dvertices["radius"], None], nomega=args.nomega, omega_param=1)
+    log(f"bench: N={n} panels, {args.nomega} frequencies, precision={args.precision}, "
+        f"adjoint gradients={'on, theta=(radius, omega)' if grad else 'off'}")
+
+    def forward(prec=None):
+        return mh.freq_sweep(dm, omegas, dofs, betas, precision=prec or args.precision, g=g, rho=rho)
 
     def step():
-        return mh.freq_sweep(dm, omegas, dofs, betas, precision=args.precision, g=g, rho=rho)
+        if grad:
+            return mh.freq_sweep_grad(dm, omegas, dofs, betas, T, precision=args.precision, g=g,
+                                      rho=rho)
+        return forward()
 
     if args.profile_only:
         step()
@@ -83,11 +95,24 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        A, B, F = step()
+        res = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     value = args.nomega / dt
+    A, B, F = res[:3]
     log(f"bench: {dt * 1e3:.1f} ms per sweep, {value:.1f} freq-solves/s")
+    # the FP64 forward + central finite-difference sweep (2 |theta| + 1
+    # forward sweeps) that the adjoint path replaces
+    fd_ms = None
+    if grad:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(2 * T.nparam + 1):
+            forward("fp64")
+        torch.cuda.synchronize()
+        fd_ms = (time.perf_counter() - t0) * 1e3
+        log(f"bench: FP64 forward + central-FD sweep {fd_ms:.1f} ms "
+            f"({fd_ms / (dt * 1e3):.2f}x the timed step)")
 
     # kernel timer over one more sweep: dominant kernel and its roofline
     mh._lib.profile_enable(True)
@@ -113,6 +138,7 @@ def main():
 
     cpu = None
     max_err = None
+    max_gerr = None
     if args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -120,21 +146,44 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         idx = np.linspace(0, args.nomega - 1, args.cpu_sample).round().astype(int)
         errs = []
+        gerrs = []
+        gfm = O.RANKINE | O.RANKINE_REFLECTED | O.WU
         t0 = time.perf_counter()
         for q in idx:
-            Ao, Bo, Fo = O.freq_solve(O.RANKINE | O.RANKINE_REFLECTED | O.WU, mesh, omegas[q], dofs,
-                                      betas, direct=True, g=g, rho=rho, nthreads=threads)
+            Ao, Bo, Fo = O.freq_solve(gfm, mesh, omegas[q], dofs, betas, direct=True, g=g, rho=rho,
+                                      nthreads=threads)
             errs += [abs(A[q, 0, 0] - Ao[0, 0]) / abs(Ao[0, 0]),
                      abs(B[q, 0, 0] - Bo[0, 0]) / abs(Bo[0, 0]),
                      abs(F[q, 0, 0] - Fo[0, 0]) / abs(Fo[0, 0])]
+            if grad:  # oracle gradients by central differences, h = 1e-6
+                h = 1e-6
+                fdv = []
+                for m in range(2):
+                    vals = []
+                    for sgn in (1, -1):
+                        if m == 0:
+                            mm = mh.hemisphere(1.0 + sgn * h, ntheta=args.ntheta, nphi=args.nphi).mesh
+                            om = omegas[q]
+                        else:
+                            mm, om = mesh, omegas[q] + sgn * h
+                        vals.append(O.freq_solve(gfm, mm, om, dofs, betas, direct=True, g=g, rho=rho,
+                                                 nthreads=threads))
+                    fdv.append([(vals[0][c] - vals[1][c]) / (2 * h) for c in range(3)])
+                dA, dB, dF = res[3], res[4], res[5]
+                for m in range(2):
+                    gerrs += [abs(dA[q, m, 0, 0] - fdv[m][0][0, 0]) / abs(fdv[m][0][0, 0]),
+                              abs(dB[q, m, 0, 0] - fdv[m][1][0, 0]) / abs(fdv[m][1][0, 0]),
+                              abs(dF[q, m, 0, 0] - fdv[m][2][0, 0]) / abs(fdv[m][2][0, 0])]
         tc = time.perf_counter() - t0
         max_err = float(max(errs))
+        max_gerr = float(max(gerrs)) if gerrs else None
+        what = "forward + central-FD gradients (5 solves)" if grad else "forward"
         cpu = {"value": round(len(idx) / tc, 4), "unit": "freq-solves/s", "cores": threads,
                "kind": "port",
                "sample": f"{len(idx)} of the {args.nomega} frequencies (indices {idx.tolist()}), "
-                         f"same mesh, FP64 C oracle with OpenMP ({tc:.1f} s)"}
+                         f"same mesh, FP64 C oracle with OpenMP, {what} ({tc:.1f} s)"}
         log(f"bench: cpu oracle {cpu['value']} freq-solves/s on {threads} threads; "
-            f"max rel err GPU vs oracle {max_err:.2e}")
+            f"max rel err GPU vs oracle {max_err:.2e}, gradients vs oracle FD {max_gerr}")
 
     out = {
         "metric": METRIC,
@@ -153,10 +202,13 @@ def main():
                                f"({n} panels), {args.nomega} frequencies 0.3-3.0 rad/s, "
                                "Wu GF, direct, heave + 1 heading",
                    "panels": n, "frequencies": args.nomega, "precision": args.precision,
-                   "adjoint_gradients": False},
+                   "adjoint_gradients": "theta=(radius, omega)" if grad else None},
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_rel_err_vs_oracle": max_err,
+        "max_rel_err_grad_vs_oracle_fd": max_gerr,
+        "fp64_forward_fd_sweep_ms": round(fd_ms, 3) if fd_ms else None,
+        "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,
         "kernel_ms_per_step": kernel_ms,
     }
     print(json.dumps(out), flush=True)

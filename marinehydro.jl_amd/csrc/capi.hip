@@ -363,7 +363,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_aug));
     if (chunk < 1) chunk = 1;
     if (chunk > nomega) chunk = nomega;
-    const int Pg = (ga && ga->nparam > 1) ? 4 : 1;  // tangent directions per gradient pass
+    // tangent directions per gradient pass
+    const int Pg = !ga ? 1 : (ga->nparam <= 1 ? 1 : (ga->nparam == 2 ? 2 : 4));
     const int64_t wa = n * (ndof > 0 ? ndof : 1);  // per-frequency stride of n x ndof blocks
     Bump bump;
     const size_t o_S = bump.take(mat_aug * chunk);
@@ -464,13 +465,11 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
         }
         // outputs of this chunk, host side: F of every (dof, rhs) pair
-        std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof), hF(2 * nw * nbeta * ndof);
+        std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof);
         if (ndof) {
             MH_HIP_CHECK(hipMemcpyAsync(hA.data(), Aw, sizeof(double) * hA.size(), hipMemcpyDeviceToHost, c->stream));
             MH_HIP_CHECK(hipMemcpyAsync(hB.data(), Bw, sizeof(double) * hB.size(), hipMemcpyDeviceToHost, c->stream));
         }
-        if (nbeta)
-            MH_HIP_CHECK(hipMemcpyAsync(hF.data(), Fw, sizeof(double2) * nw * nbeta * ndof, hipMemcpyDeviceToHost, c->stream));
         const int64_t np = ga->nparam;
         std::vector<double> gA(nw * np * ndof * ndof), gB(nw * np * ndof * ndof), gF(2 * nw * np * nbeta * ndof);
         std::vector<double> hdk(nw * Pg), hdom(nw * Pg);
@@ -518,7 +517,6 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 }
             }
         }
-        (void)hF;
         if (ga->dAout)
             MH_HIP_CHECK(hipMemcpy(ga->dAout + w0 * np * ndof * ndof, gA.data(), sizeof(double) * gA.size(), hipMemcpyHostToDevice));
         if (ga->dBout)

@@ -379,6 +379,14 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
             if (form == 1) { if (nr2) TP(0, 1, 1, 2); else TP(0, 1, 1, 4); }
             else { if (nr2) TP(0, 2, 1, 2); else TP(0, 2, 1, 4); }
         }
+    } else if (P == 2) {
+        if (wu) {
+            if (form == 1) { if (nr2) TP(1, 1, 2, 2); else TP(1, 1, 2, 4); }
+            else { if (nr2) TP(1, 2, 2, 2); else TP(1, 2, 2, 4); }
+        } else {
+            if (form == 1) { if (nr2) TP(0, 1, 2, 2); else TP(0, 1, 2, 4); }
+            else { if (nr2) TP(0, 2, 2, 2); else TP(0, 2, 2, 4); }
+        }
     } else {
         if (wu) {
             if (form == 1) { if (nr2) TP(1, 1, 4, 2); else TP(1, 1, 4, 4); }
@@ -400,6 +408,9 @@ int launch_pack_tangents(Context* ctx, const mh_mesh_t* m, int P, int64_t nparam
     dim3 grid((unsigned)((nf + 127) / 128)), blk(128);
     if (P == 1)
         hipLaunchKernelGGL(k_pack_tangents<1>, grid, blk, 0, ctx->stream, nf, m->nvertices, nparam, m0,
+                           m->faces, dV, dC, dN, dA, tan);
+    else if (P == 2)
+        hipLaunchKernelGGL(k_pack_tangents<2>, grid, blk, 0, ctx->stream, nf, m->nvertices, nparam, m0,
                            m->faces, dV, dC, dN, dA, tan);
     else
         hipLaunchKernelGGL(k_pack_tangents<4>, grid, blk, 0, ctx->stream, nf, m->nvertices, nparam, m0,
@@ -426,6 +437,10 @@ int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan
     ProfScope ps(ctx, "bc_tangent", 0.0, 0.0);
     if (P == 1)
         hipLaunchKernelGGL(k_bc_tangent<1>, dim3((unsigned)nw), dim3(256), 0, ctx->stream, n, m->centers,
+                           m->normals, m->areas, tan, d_om, d_dom, g, rho, ndof, dofs, ddofs, nparam, m0,
+                           nbeta, d_beta, phi, bstride, rhoW, wstride, out);
+    else if (P == 2)
+        hipLaunchKernelGGL(k_bc_tangent<2>, dim3((unsigned)nw), dim3(256), 0, ctx->stream, n, m->centers,
                            m->normals, m->areas, tan, d_om, d_dom, g, rho, ndof, dofs, ddofs, nparam, m0,
                            nbeta, d_beta, phi, bstride, rhoW, wstride, out);
     else

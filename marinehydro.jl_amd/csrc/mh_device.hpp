@@ -347,4 +347,64 @@ __device__ __forceinline__ void wave_term(T k, T dx, T dy, T zsum, T area, cplxT
     gx[2] = gxi[2];
 }
 
+
+// Wave term with P tangent directions.  G depends on the parameters only
+// through the reduced coordinates (r, z), so the Wu functions run with two
+// local tangents (d/dr, d/dz) and the result is chained to the P parameter
+// directions: X_m = X_r dr_m + X_z dz_m.  Same values and derivatives as
+// running the generic version in Dual<P>, at a cost independent of P.
+template <int P>
+__device__ __forceinline__ Dual<P> lift_rz(const Dual<2>& X, const Dual<P>& r, const Dual<P>& z) {
+    Dual<P> o;
+    o.v = X.v;
+#pragma unroll
+    for (int m = 0; m < P; ++m) o.d[m] = X.d[0] * r.d[m] + X.d[1] * z.d[m];
+    return o;
+}
+
+template <int WAVE, int P>
+__device__ __forceinline__ void wave_term(Dual<P> k, Dual<P> dx, Dual<P> dy, Dual<P> zsum,
+                                          Dual<P> area, cplxT<Dual<P>>& Sv, cplxT<Dual<P>> gxi[3],
+                                          cplxT<Dual<P>> gx[3]) {
+    typedef Dual<P> T;
+    const T r = k * horiz_dist(dx, dy);
+    const T z = k * zsum;
+    Dual<2> rl, zl;
+    rl.v = r.v;
+    rl.d[0] = 1.0;
+    rl.d[1] = 0.0;
+    zl.v = z.v;
+    zl.d[0] = 0.0;
+    zl.d[1] = 1.0;
+    cplxT<Dual<2>> Gl, Grl, Gzl;
+    if (WAVE == 1) wu_dimless<true>(rl, zl, Gl, Grl, Gzl);
+    cplxT<T> G, dGdr, dGdz;
+    G.re = lift_rz(Gl.re, r, z);
+    G.im = lift_rz(Gl.im, r, z);
+    dGdr.re = lift_rz(Grl.re, r, z);
+    dGdr.im = lift_rz(Grl.im, r, z);
+    dGdz.re = lift_rz(Gzl.re, r, z);
+    dGdz.im = lift_rz(Gzl.im, r, z);
+    T d1 = Cst<T>::make(0.0), d2 = Cst<T>::make(0.0);
+    if (::fabs(val(r)) > 1e-6) {
+        const T kk = k * k / r;
+        d1 = kk * (-dx);
+        d2 = kk * (-dy);
+    }
+    const T ka = k * area;
+    Sv.re = ka * G.re;
+    Sv.im = ka * G.im;
+    gxi[0].re = ka * (d1 * dGdr.re);
+    gxi[0].im = ka * (d1 * dGdr.im);
+    gxi[1].re = ka * (d2 * dGdr.re);
+    gxi[1].im = ka * (d2 * dGdr.im);
+    gxi[2].re = ka * (k * dGdz.re);
+    gxi[2].im = ka * (k * dGdz.im);
+    gx[0].re = -gxi[0].re;
+    gx[0].im = -gxi[0].im;
+    gx[1].re = -gxi[1].re;
+    gx[1].im = -gxi[1].im;
+    gx[2] = gxi[2];
+}
+
 }  // namespace mh

@@ -528,6 +528,8 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
     def up(a, shape):  # (nparam, n, c) -> column-major per parameter
         if a is None:
             return None
+        if torch.is_tensor(a):  # already device-resident, column-major per parameter
+            return a
         a = np.asarray(a, dtype=np.float64).reshape(shape)
         if a.ndim == 3:
             a = np.transpose(a, (0, 2, 1))
