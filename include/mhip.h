@@ -211,8 +211,7 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
  * Radii only select the near/far branch and carry no derivative.
  * Outputs (device): A, B, F as mh_freq_sweep; dA, dB: nomega x nparam x
  * ndof x ndof (same inner layout as A); dF: nomega x nparam x nbeta x ndof
- * complex128.  Synchronises.  Limits: ndof >= 1, ndof + nbeta <= 4, FP64,
- * Rankine/image/Wu terms. */
+ * complex128.  Synchronises.  Limits: ndof >= 1, ndof + nbeta <= 4. */
 int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
                        int direct, int precision, int64_t nomega,
                        const double* omegas, double g, double rho, int64_t ndof,

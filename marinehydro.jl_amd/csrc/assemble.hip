@@ -142,10 +142,6 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
                     int64_t stride) {
     const int64_t n = m->nfaces;
     if (n == 0 || nk == 0) return MH_OK;
-    if (gf & GF_DELHOMMEAU) {
-        set_error("assemble: the Delhommeau wave term is not available in this build");
-        return MH_ERR_NOT_SUPPORTED;
-    }
     const int64_t tiles = (n + 63) / 64;
     const int64_t groups = (nk + kFG - 1) / kFG;
     if (tiles > 0x7fffffff || groups > 65535) {
@@ -157,6 +153,9 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
     ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
     if (gf & GF_WU)
         dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
+                         D, K, ld, stride);
+    else if (gf & GF_DELHOMMEAU)
+        dispatch_form<2>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
                          D, K, ld, stride);
     else
         dispatch_form<0>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
@@ -226,10 +225,13 @@ __global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
             }
         }
     }
-    if (gf & GF_WU) {
+    if (gf & (GF_WU | GF_DELHOMMEAU)) {
         cplx Sv, gxi[3], gx[3];
         const double area = panel_int ? p.area : 1.0;
-        wave_term<1>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
+        if (gf & GF_WU)
+            wave_term<1>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
+        else
+            wave_term<2>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
         val_re += Sv.re;
         val_im += Sv.im;
         for (int c = 0; c < 3; ++c) {
@@ -247,10 +249,6 @@ __global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
 int launch_gf_eval(Context* ctx, int gf, int op, int wrt_first, int64_t np, const double* pts,
                    const mh_panel_t* pan, double k, double2* out) {
     if (np == 0) return MH_OK;
-    if (gf & GF_DELHOMMEAU) {
-        set_error("gf_eval: the Delhommeau wave term is not available in this build");
-        return MH_ERR_NOT_SUPPORTED;
-    }
     hipLaunchKernelGGL(k_gf_eval, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, ctx->stream, gf,
                        op, wrt_first, np, pts, pan, k, out);
     MH_HIP_CHECK(hipGetLastError());
@@ -262,7 +260,10 @@ __global__ void k_wave_term(int term, int64_t n, const double* __restrict__ r,
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     cplx G, dr, dz;
-    wu_dimless<true>(r[t], z[t], G, dr, dz);
+    if (term == GF_WU)
+        wu_dimless<true>(r[t], z[t], G, dr, dz);
+    else
+        delhommeau_dimless(r[t], z[t], G, dr, dz);
     out[t] = make_double2(G.re, G.im);
     out[t + n] = make_double2(dr.re, dr.im);
     out[t + 2 * n] = make_double2(dz.re, dz.im);
@@ -270,9 +271,9 @@ __global__ void k_wave_term(int term, int64_t n, const double* __restrict__ r,
 
 int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const double* z,
                      double2* out) {
-    if (term != GF_WU) {
-        set_error("wave_term: only MH_GF_WU is available in this build");
-        return MH_ERR_NOT_SUPPORTED;
+    if (term != GF_WU && term != GF_DELHOMMEAU) {
+        set_error("wave_term: term must be MH_GF_WU or MH_GF_DELHOMMEAU");
+        return MH_ERR_ARGUMENT;
     }
     if (n == 0) return MH_OK;
     hipLaunchKernelGGL(k_wave_term, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream,

@@ -366,36 +366,43 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)nw);
     MH_REQUIRE(nr <= 4, "gradients: at most 4 dofs + headings per pass");
     ProfScope ps(ctx, "tangent_pairs", 0.0, 0.0);
-    const bool wu = gf & GF_WU;
+    const int wave = (gf & GF_WU) ? 1 : ((gf & GF_DELHOMMEAU) ? 2 : 0);
     const bool nr2 = nr <= 2;
 #define TP(WV, FM, PP, NRR)                                                                        \
     launch_tp<WV, FM, PP, NRR>(grid, ctx->stream, n, m->centers, m->normals, panels, tan, gf, d_k, \
                                d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)
-    if (P == 1) {
-        if (wu) {
-            if (form == 1) { if (nr2) TP(1, 1, 1, 2); else TP(1, 1, 1, 4); }
-            else { if (nr2) TP(1, 2, 1, 2); else TP(1, 2, 1, 4); }
-        } else {
-            if (form == 1) { if (nr2) TP(0, 1, 1, 2); else TP(0, 1, 1, 4); }
-            else { if (nr2) TP(0, 2, 1, 2); else TP(0, 2, 1, 4); }
-        }
-    } else if (P == 2) {
-        if (wu) {
-            if (form == 1) { if (nr2) TP(1, 1, 2, 2); else TP(1, 1, 2, 4); }
-            else { if (nr2) TP(1, 2, 2, 2); else TP(1, 2, 2, 4); }
-        } else {
-            if (form == 1) { if (nr2) TP(0, 1, 2, 2); else TP(0, 1, 2, 4); }
-            else { if (nr2) TP(0, 2, 2, 2); else TP(0, 2, 2, 4); }
-        }
-    } else {
-        if (wu) {
-            if (form == 1) { if (nr2) TP(1, 1, 4, 2); else TP(1, 1, 4, 4); }
-            else { if (nr2) TP(1, 2, 4, 2); else TP(1, 2, 4, 4); }
-        } else {
-            if (form == 1) { if (nr2) TP(0, 1, 4, 2); else TP(0, 1, 4, 4); }
-            else { if (nr2) TP(0, 2, 4, 2); else TP(0, 2, 4, 4); }
-        }
-    }
+#define TP_NR(WV, FM, PP) \
+    do {                  \
+        if (nr2)          \
+            TP(WV, FM, PP, 2); \
+        else              \
+            TP(WV, FM, PP, 4); \
+    } while (0)
+#define TP_FORM(WV, PP)        \
+    do {                       \
+        if (form == 1)         \
+            TP_NR(WV, 1, PP);  \
+        else                   \
+            TP_NR(WV, 2, PP);  \
+    } while (0)
+#define TP_WAVE(PP)            \
+    do {                       \
+        if (wave == 1)         \
+            TP_FORM(1, PP);    \
+        else if (wave == 2)    \
+            TP_FORM(2, PP);    \
+        else                   \
+            TP_FORM(0, PP);    \
+    } while (0)
+    if (P == 1)
+        TP_WAVE(1);
+    else if (P == 2)
+        TP_WAVE(2);
+    else
+        TP_WAVE(4);
+#undef TP_WAVE
+#undef TP_FORM
+#undef TP_NR
 #undef TP
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;

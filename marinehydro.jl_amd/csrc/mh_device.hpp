@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "delhommeau.hpp"
 #include "mh_dual.hpp"
 
 namespace mh {
@@ -274,6 +275,46 @@ __device__ __forceinline__ T horiz_dist(T dx, T dy) {
 }
 
 // ---------------------------------------------------------------------------
+// Exact Guevel-Delhommeau wave term (delhommeau.hpp) in the same interface:
+// double values, or Dual<2> with local tangents (d/dr, d/dz) seeded by the
+// caller; dG/dr is the exact r-derivative of G and dG/dz = G + 2/hypot(r, z)
+// (exact_Guevel_Delhommeau.jl:52-59), so the local second derivatives are
+// two more theta-integrals and closed forms.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void delhommeau_dimless(double r, double z, cplx& G, cplx& dGdr,
+                                                   cplx& dGdz) {
+    double g[2], gr[2], gz[2];
+    delh::delhommeau_exact<false>(r, z, g, gr, gz);
+    G = {g[0], g[1]};
+    dGdr = {gr[0], gr[1]};
+    dGdz = {gz[0], gz[1]};
+}
+
+__device__ __forceinline__ void delhommeau_dimless(const Dual<2>& rl, const Dual<2>& zl,
+                                                   cplxT<Dual<2>>& G, cplxT<Dual<2>>& dGdr,
+                                                   cplxT<Dual<2>>& dGdz) {
+    // rl, zl carry the local seeds (1, 0) and (0, 1)
+    const double r = rl.v, z = zl.v;
+    double g[2], gr[2], gz[2], grr[2], grz[2];
+    delh::delhommeau_exact<true>(r, z, g, gr, gz, grr, grz);
+    const double d = ::hypot(r, z), d3 = d * d * d;
+    auto mk = [](double v, double dr, double dz) {
+        Dual<2> o;
+        o.v = v;
+        o.d[0] = dr;
+        o.d[1] = dz;
+        return o;
+    };
+    // G_r = Gr, G_z = Gz;  Gr_r = Grr, Gr_z = Grz;  Gz = G + 2/d
+    G.re = mk(g[0], gr[0], gz[0]);
+    G.im = mk(g[1], gr[1], gz[1]);
+    dGdr.re = mk(gr[0], grr[0], grz[0]);
+    dGdr.im = mk(gr[1], grr[1], grz[1]);
+    dGdz.re = mk(gz[0], gr[0] - 2.0 * r / d3, gz[0] - 2.0 * z / d3);
+    dGdz.im = mk(gz[1], gr[1], gz[1]);
+}
+
+// ---------------------------------------------------------------------------
 // Pair terms shared by the assembly and gradient kernels
 // ---------------------------------------------------------------------------
 // Rankine + image Rankine: value and gradients w.r.t. the source point (gxi)
@@ -325,6 +366,7 @@ __device__ __forceinline__ void wave_term(T k, T dx, T dy, T zsum, T area, cplxT
     const T z = k * zsum;
     cplxT<T> G, dGdr, dGdz;
     if (WAVE == 1) wu_dimless<true>(r, z, G, dGdr, dGdz);
+    if (WAVE == 2) delhommeau_dimless(r, z, G, dGdr, dGdz);
     T d1 = Cst<T>::make(0.0), d2 = Cst<T>::make(0.0);  // dr/dxi_1, dr/dxi_2
     if (::fabs(val(r)) > 1e-6) {
         const T kk = k * k / r;
@@ -378,6 +420,7 @@ __device__ __forceinline__ void wave_term(Dual<P> k, Dual<P> dx, Dual<P> dy, Dua
     zl.d[1] = 1.0;
     cplxT<Dual<2>> Gl, Grl, Gzl;
     if (WAVE == 1) wu_dimless<true>(rl, zl, Gl, Grl, Gzl);
+    if (WAVE == 2) delhommeau_dimless(rl, zl, Gl, Grl, Gzl);
     cplxT<T> G, dGdr, dGdz;
     G.re = lift_rz(Gl.re, r, z);
     G.im = lift_rz(Gl.im, r, z);
