@@ -339,6 +339,39 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
     return check_info(c, reinterpret_cast<int*>(ws + o_info), batch);
 }
 
+// Solves op(A) X = Bm (op = A or A^T) for `batch` systems with the FP32 LU
+// factors of A and iterative refinement against the FP64 A: X <- X + LU32^{-1}
+// (Bm - op(A) X), until every column's relative residual max|r| / max|b| is
+// below 1e-12 (at most 12 sweeps).  Bm, X, R share one layout (ldb, bstride);
+// R32 is n x nrhs per item.
+static int ir_solve(Context* c, int64_t n, int64_t batch, int trans, const double2* A64, int64_t ld,
+                    int64_t stride, const float2* LU32, int64_t stride32, const int64_t* piv,
+                    const double2* Bm, int64_t nrhs, int64_t ldb, int64_t bstride, double2* X,
+                    double2* R, float2* R32, double* d_nrm) {
+    const int64_t s32 = n * nrhs;
+    MH_HIP_CHECK(hipMemsetAsync(X, 0, sizeof(double2) * (size_t)(bstride * batch), c->stream));
+    MH_HIP_CHECK(hipMemcpyAsync(R, Bm, sizeof(double2) * (size_t)(bstride * batch), hipMemcpyDeviceToDevice,
+                                c->stream));
+    std::vector<double> h(batch * nrhs);
+    for (int it = 0; it < 12; ++it) {
+        TRY(convert_to_f32(c, n, nrhs, batch, R, ldb, bstride, R32, n, s32));
+        TRY(lu_solve_part_f32(c, n, batch, LU32, n, stride32, piv, trans, 3, nrhs, R32, n, s32));
+        TRY(add_from_f32(c, n, nrhs, batch, R32, n, s32, X, ldb, bstride));
+        if (trans)
+            TRY(zgemm_tn_batched(c, n, nrhs, batch, A64, ld, stride, X, ldb, bstride, R, ldb, bstride, Bm));
+        else
+            TRY(zgemm_nn_batched(c, n, nrhs, batch, A64, ld, stride, X, ldb, bstride, R, ldb, bstride, Bm));
+        TRY(rel_residual(c, n, nrhs, batch, R, Bm, ldb, bstride, d_nrm));
+        MH_HIP_CHECK(hipMemcpyAsync(h.data(), d_nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+        double worst = 0.0;
+        for (double v : h) worst = v > worst ? v : worst;
+        if (worst < 1e-12) return MH_OK;
+    }
+    set_error("mixed precision: iterative refinement did not reach a 1e-12 relative residual");
+    return MH_ERR_NOT_CONVERGED;
+}
+
 // Optional gradient request of sweep_impl (see mh_freq_sweep_grad).
 struct GradArgs {
     int64_t nparam;
@@ -347,7 +380,8 @@ struct GradArgs {
     double *dAout, *dBout, *dFout;             // device outputs
 };
 
-static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int64_t nomega,
+static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
+                      int64_t nomega,
                       const double* omegas, double g, double rho, int64_t ndof,
                       const double* dof_fields, int64_t nbeta, const double* betas, double* A,
                       double* B, double* F, const GradArgs* ga) {
@@ -377,6 +411,16 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     const size_t o_k = bump.take(sizeof(double) * (size_t)nomega);
     const size_t o_be = bump.take(sizeof(double) * (size_t)(nbeta > 0 ? nbeta : 1));
     const size_t o_P = bump.take(sizeof(PanelRec) * (size_t)n);
+    const bool mixed = precision == MH_MIXED;
+    const int64_t ncol_ir = nr > ndof ? nr : ndof;  // columns of the refinement buffers
+    size_t o_D32 = 0, o_y = 0, o_R = 0, o_R32 = 0, o_nrm = 0;
+    if (mixed) {
+        o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * chunk);
+        o_y = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
+        o_R = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
+        o_R32 = bump.take(sizeof(float2) * (size_t)(n * ncol_ir * chunk));
+        o_nrm = bump.take(sizeof(double) * (size_t)(ncol_ir * chunk));
+    }
     size_t o_cw = 0, o_lam = 0, o_mu = 0, o_tan = 0, o_dk = 0, o_dom = 0, o_part = 0, o_T1 = 0, o_T2 = 0;
     const int64_t tiles = (n + 63) / 64;
     const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
@@ -417,22 +461,43 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         TRY(launch_assemble(c, mesh, P, gf_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
                             direct ? nullptr : dD, ld, stride));
         TRY(launch_bcs(c, mesh, nw, d_om + w0, g, ndof, dof_fields, nbeta, d_be, rhs, ldb, bstride));
-        if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
+        float2* D32 = reinterpret_cast<float2*>(ws + o_D32);
+        double2* ybuf = reinterpret_cast<double2*>(ws + o_y);
+        double2* Rb = reinterpret_cast<double2*>(ws + o_R);
+        float2* R32 = reinterpret_cast<float2*>(ws + o_R32);
+        double* d_nrm = reinterpret_cast<double*>(ws + o_nrm);
+        if (mixed) {
+            // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64
+            TRY(convert_to_f32(c, n, n, nw, dD, ld, stride, D32, n, n * n));
+            MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
+            TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
+            if (direct) {  // phi = D \ (S bc)
+                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
+                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, piv, ybuf, nr, ldb, bstride, phi, Rb,
+                             R32, d_nrm));
+            } else {  // sigma = K \ bc (kept in ybuf); phi = S sigma
+                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, piv, rhs, nr, ldb, bstride, ybuf, Rb,
+                             R32, d_nrm));
+                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
+            }
+        } else if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
             TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
-        } else {       // sigma = K \ bc; phi = S sigma   (:127-131)
+        } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
             MH_HIP_CHECK(hipMemcpy2DAsync(aug, sizeof(double2) * stride, rhs, sizeof(double2) * bstride,
                                           sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
                                           c->stream));
         }
-        MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
-        TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
-        TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 0, 2, nr, aug, ld, stride));
-        if (direct) {
-            MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
-                                          sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
-                                          c->stream));
-        } else {
-            TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, aug, ld, stride, phi, ldb, bstride));
+        if (!mixed) {
+            MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
+            TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
+            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 0, 2, nr, aug, ld, stride));
+            if (direct) {
+                MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
+                                              sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
+                                              c->stream));
+            } else {
+                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, aug, ld, stride, phi, ldb, bstride));
+            }
         }
         double* Aw = A ? A + w0 * ndof * ndof : nullptr;
         double* Bw = B ? B + w0 * ndof * ndof : nullptr;
@@ -455,14 +520,24 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         const double2 *aS, *aDw, *sS, *sD, *rhoW;
         int64_t wss;
         if (direct) {  // lambda = D^-T c; mu = S^T lambda
-            MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
-            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+            if (mixed) {
+                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, piv, cw, ndof, n, wa, lam, Rb, R32, d_nrm));
+            } else {
+                MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
+                TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+            }
             TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
             aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
         } else {       // lambda = K^-T S^T c
-            TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
-            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
-            aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
+            if (mixed) {
+                TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, mu, n, wa));
+                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, piv, mu, ndof, n, wa, lam, Rb, R32, d_nrm));
+                aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
+            } else {
+                TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
+                TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+                aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
+            }
         }
         // outputs of this chunk, host side: F of every (dof, rhs) pair
         std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof);
@@ -538,10 +613,7 @@ static int sweep_checks(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, in
                "freq_sweep: choose one wave term");
     MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0, "freq_sweep: negative size");
     MH_REQUIRE(g > 0.0 && rho > 0.0, "freq_sweep: g and rho must be positive");
-    if (precision != MH_FP64) {
-        set_error("freq_sweep: only MH_FP64 is available in this build");
-        return MH_ERR_NOT_SUPPORTED;
-    }
+    MH_REQUIRE(precision == MH_FP64 || precision == MH_MIXED, "freq_sweep: unknown precision");
     const int64_t n = mesh->nfaces;
     const int64_t nr = ndof + nbeta;
     if (n == 0 || nomega == 0 || nr == 0) return MH_OK;
@@ -562,8 +634,8 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int dire
     TRY(sweep_checks(ctx, mesh, gf_mask, precision, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
                      betas, A, B, F));
     if (mesh->nfaces == 0 || nomega == 0 || ndof + nbeta == 0) return MH_OK;
-    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
-                      betas, A, B, F, nullptr);
+    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, precision, nomega, omegas, g, rho, ndof,
+                      dof_fields, nbeta, betas, A, B, F, nullptr);
 }
 
 int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct,
@@ -581,8 +653,8 @@ int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int
     MH_REQUIRE(ndof + nbeta <= 4, "freq_sweep_grad: at most 4 dofs + headings");
     MH_REQUIRE(nparam == 0 || (dA && dB && (nbeta == 0 || dF)), "freq_sweep_grad: null gradient output");
     GradArgs ga{nparam, dvertices, dcenters, dnormals, dareas, ddof_fields, domegas, dA, dB, dF};
-    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, nomega, omegas, g, rho, ndof, dof_fields, nbeta,
-                      betas, A, B, F, nparam > 0 ? &ga : nullptr);
+    return sweep_impl(CTX(ctx), mesh, gf_mask, direct, precision, nomega, omegas, g, rho, ndof,
+                      dof_fields, nbeta, betas, A, B, F, nparam > 0 ? &ga : nullptr);
 }
 
 // ---- host-array entry points -----------------------------------------------

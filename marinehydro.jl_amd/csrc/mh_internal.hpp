@@ -103,11 +103,25 @@ int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int
 // Y[b] = S[b]^T * X[b]
 int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
-                     double2* Y, int64_t ldy, int64_t ystride);
+                     double2* Y, int64_t ldy, int64_t ystride, const double2* Z = nullptr);
 // Y[b] = S[b] * X[b]  (n x n times n x nrhs, complex)
+// (with Z: Y[b] = Z[b] - S[b] * X[b], Z laid out like Y)
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
-                     double2* Y, int64_t ldy, int64_t ystride);
+                     double2* Y, int64_t ldy, int64_t ystride, const double2* Z = nullptr);
+// FP32 factorisation and solves (mixed precision)
+int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
+                  int64_t stride, int64_t* ipiv, int* d_info);
+int lu_solve_part_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld,
+                      int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
+                      float2* B, int64_t ldb, int64_t bstride);
+int convert_to_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* src,
+                   int64_t lds, int64_t sstride, float2* dst, int64_t ldd, int64_t dstride);
+int add_from_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const float2* d,
+                 int64_t ldd, int64_t dstride, double2* x, int64_t ldx, int64_t xstride);
+// out[b*cols + j] = max|r[:, j]| / max|b[:, j]| (|.| = |re| + |im|)
+int rel_residual(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* r,
+                 const double2* b, int64_t ld, int64_t stride, double* out);
 
 // ---- launchers (physics.hip) ------------------------------------------------
 // rhs[w] columns: ndof radiation BCs then nbeta Airy BCs, for each omega.

@@ -20,64 +20,102 @@
 namespace mh {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// element traits: T = double (FP64 LU) or float (FP32 LU of the mixed path)
+template <class T>
+struct Cx;
+template <>
+struct Cx<double> {
+    typedef double2 t;
+    typedef d4 acc;
+    // v_mfma_f64_16x16x4f64: result register r of lane l is row 4r + l/16
+    __device__ __forceinline__ static int row(int r, int l) { return 4 * r + (l >> 4); }
+    __device__ __forceinline__ static acc mfma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Cx<float> {
+    typedef float2 t;
+    typedef f4 acc;
+    // v_mfma_f32_16x16x4f32: result register r of lane l is row 4 (l/16) + r
+    __device__ __forceinline__ static int row(int r, int l) { return 4 * (l >> 4) + r; }
+    __device__ __forceinline__ static acc mfma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <class C2>
+__device__ __forceinline__ C2 mkc(decltype(C2().x) a, decltype(C2().x) b) {
+    C2 o;
+    o.x = a;
+    o.y = b;
+    return o;
+}
 
 constexpr int kNBOuter = 128;  // block-column width of the trailing updates
 constexpr int kLeaf = 8;       // register-resident leaf width
 constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <class C2>
+__device__ __forceinline__ C2 cmul(C2 a, C2 b) {
+    return mkc<C2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // a - b*c
-__device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
-    return make_double2(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
+template <class C2>
+__device__ __forceinline__ C2 cfms(C2 a, C2 b, C2 c) {
+    return mkc<C2>(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
 }
-__device__ __forceinline__ double2 crecip(double2 a) {
+template <class C2>
+__device__ __forceinline__ C2 crecip(C2 a) {
     // Smith's algorithm for 1/a
+    typedef decltype(C2().x) T;
     if (fabs(a.x) >= fabs(a.y)) {
-        const double r = a.y / a.x, d = a.x + a.y * r;
-        return make_double2(1.0 / d, -r / d);
+        const T r = a.y / a.x, d = a.x + a.y * r;
+        return mkc<C2>(T(1) / d, -r / d);
     }
-    const double r = a.x / a.y, d = a.y + a.x * r;
-    return make_double2(r / d, -1.0 / d);
+    const T r = a.x / a.y, d = a.y + a.x * r;
+    return mkc<C2>(r / d, T(-1) / d);
 }
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) { return cmul(a, crecip(b)); }
+template <class C2>
+__device__ __forceinline__ C2 cdiv(C2 a, C2 b) { return cmul(a, crecip(b)); }
 
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= 8, held in registers.
 // Thread t owns local rows t + 1024 q, q < R.
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict__ A, int64_t ld,
+template <class T, int R>
+__global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
                                                   int64_t stride, int64_t k0, int w,
                                                   int64_t* __restrict__ ipiv,
                                                   int* __restrict__ info) {
-    double2* M = A + (int64_t)blockIdx.x * stride;
+    typedef typename Cx<T>::t C2;
+    C2* M = A + (int64_t)blockIdx.x * stride;
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = (int)(n - k0);
-    __shared__ double s_v[16];
+    __shared__ T s_v[16];
     __shared__ int s_i[16];
-    __shared__ double2 s_crow[kLeaf], s_prow[kLeaf];
+    __shared__ C2 s_crow[kLeaf], s_prow[kLeaf];
     __shared__ int s_p, s_zero;
-    double2 a[R][kLeaf];
+    C2 a[R][kLeaf];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const int r = tid + 1024 * q;
 #pragma unroll
         for (int c = 0; c < kLeaf; ++c)
-            a[q][c] = (r < m && c < w) ? M[(k0 + r) + (k0 + c) * ld] : make_double2(0.0, 0.0);
+            a[q][c] = (r < m && c < w) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
     }
 #pragma unroll
     for (int c = 0; c < kLeaf; ++c) {
         if (c < w) {  // uniform
-        double best = -1.0;
+        T best = T(-1);
         int bi = 0x7fffffff;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int r = tid + 1024 * q;
             if (r >= c && r < m) {
-                const double v = fabs(a[q][c].x) + fabs(a[q][c].y);
+                const T v = fabs(a[q][c].x) + fabs(a[q][c].y);
                 if (v > best) {
                     best = v;
                     bi = r;
@@ -85,7 +123,7 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off);
+            const T ov = __shfl_xor(best, off);
             const int oi = __shfl_xor(bi, off);
             if (ov > best || (ov == best && oi < bi)) {
                 best = ov;
@@ -98,10 +136,10 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict
         }
         __syncthreads();
         if (wave == 0) {
-            double b = (lane < 16) ? s_v[lane] : -1.0;
+            T b = (lane < 16) ? s_v[lane] : T(-1);
             int p = (lane < 16) ? s_i[lane] : 0x7fffffff;
             for (int off = 8; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(b, off);
+                const T ov = __shfl_xor(b, off);
                 const int oi = __shfl_xor(p, off);
                 if (ov > b || (ov == b && oi < p)) {
                     b = ov;
@@ -109,7 +147,7 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict
                 }
             }
             if (lane == 0) {
-                if (b <= 0.0) {  // exact zero pivot: record, no interchange, no update
+                if (b <= T(0)) {  // exact zero pivot: record, no interchange, no update
                     p = c;
                     s_zero = 1;
                     atomicOr(info + blockIdx.x, 1);
@@ -146,12 +184,12 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict
                 }
         }
         if (!s_zero) {
-            const double2 inv = crecip(s_prow[c]);
+            const C2 inv = crecip(s_prow[c]);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int r = tid + 1024 * q;
                 if (r > c && r < m) {
-                    const double2 l = cmul(a[q][c], inv);
+                    const C2 l = cmul(a[q][c], inv);
                     a[q][c] = l;
 #pragma unroll
                     for (int j = c + 1; j < kLeaf; ++j) a[q][j] = cfms(a[q][j], l, s_prow[j]);
@@ -173,21 +211,22 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, double2* __restrict
 // ---------------------------------------------------------------------------
 // row interchanges of pivots [k0, k0+npiv) on columns [c_lo, c_hi)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_swap(double2* __restrict__ A, int64_t ld, int64_t stride,
+template <class C2>
+__global__ __launch_bounds__(256) void k_lu_swap(C2* __restrict__ A, int64_t ld, int64_t stride,
                                                  int64_t n, const int64_t* __restrict__ ipiv,
                                                  int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
-    double2* M = A + (int64_t)blockIdx.y * stride;
+    C2* M = A + (int64_t)blockIdx.y * stride;
     const int64_t* piv = ipiv + (int64_t)blockIdx.y * n;
     __shared__ int64_t sp[kNBOuter];
     for (int t = threadIdx.x; t < npiv; t += 256) sp[t] = piv[k0 + t];
     __syncthreads();
     const int64_t j = c_lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= c_hi) return;
-    double2* col = M + j * ld;
+    C2* col = M + j * ld;
     for (int t = 0; t < npiv; ++t) {
         const int64_t p = sp[t];
         if (p != k0 + t) {
-            const double2 x = col[k0 + t];
+            const C2 x = col[k0 + t];
             col[k0 + t] = col[p];
             col[p] = x;
         }
@@ -200,22 +239,24 @@ __global__ __launch_bounds__(256) void k_lu_swap(double2* __restrict__ A, int64_
 // holds row r, coalesced 512-byte column reads); forward substitution by
 // broadcasting x_c within the 32-lane group.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lu_trsm(double2* __restrict__ A, int64_t ld, int64_t stride,
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_trsm(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
                                                  int64_t k0, int nb, int64_t c_lo, int64_t c_hi) {
-    double2* M = A + (int64_t)blockIdx.y * stride;
-    __shared__ double2 sL[kTrsmNB][kTrsmNB + 1];  // sL[c][r] = L11(r, c)
+    typedef typename Cx<T>::t C2;
+    C2* M = A + (int64_t)blockIdx.y * stride;
+    __shared__ C2 sL[kTrsmNB][kTrsmNB + 1];  // sL[c][r] = L11(r, c)
     for (int q = threadIdx.x; q < kTrsmNB * kTrsmNB; q += 256) {
         const int r = q % kTrsmNB, c = q / kTrsmNB;
-        sL[c][r] = (r < nb && c < nb) ? M[(k0 + r) + (k0 + c) * ld] : make_double2(0.0, 0.0);
+        sL[c][r] = (r < nb && c < nb) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
     }
     __syncthreads();
     const int r = threadIdx.x & 31;
     const int64_t j = c_lo + (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
     const bool ok = j < c_hi;
-    double2* col = M + (ok ? j : c_lo) * ld;
-    double2 x = (ok && r < nb) ? col[k0 + r] : make_double2(0.0, 0.0);
+    C2* col = M + (ok ? j : c_lo) * ld;
+    C2 x = (ok && r < nb) ? col[k0 + r] : mkc<C2>(T(0), T(0));
     for (int c = 0; c < nb; ++c) {
-        const double2 xc = make_double2(__shfl(x.x, c, 32), __shfl(x.y, c, 32));
+        const C2 xc = mkc<C2>(__shfl(x.x, c, 32), __shfl(x.y, c, 32));
         if (r > c) x = cfms(x, sL[c][r], xc);
     }
     if (ok && r < nb) col[k0 + r] = x;
@@ -230,23 +271,26 @@ __global__ __launch_bounds__(256) void k_lu_trsm(double2* __restrict__ A, int64_
 // ---------------------------------------------------------------------------
 constexpr int kKC = 16;  // depth per LDS stage
 
-__global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_t ld, int64_t stride,
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
                                                  int64_t r_lo, int64_t r_hi, int64_t c_lo,
                                                  int64_t c_hi, int64_t k_lo, int kw) {
-    double2* M = A + (int64_t)blockIdx.z * stride;
+    typedef typename Cx<T>::t C2;
+    typedef typename Cx<T>::acc acc_t;
+    C2* M = A + (int64_t)blockIdx.z * stride;
     const int64_t tr = r_lo + (int64_t)blockIdx.x * 64;
     const int64_t tc = c_lo + (int64_t)blockIdx.y * 64;
-    __shared__ double2 sA[kKC][64];  // sA[k][row]
-    __shared__ double2 sB[kKC][64];  // sB[k][col]
+    __shared__ C2 sA[kKC][64];  // sA[k][row]
+    __shared__ C2 sB[kKC][64];  // sB[k][col]
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int wr = wave & 1, wc = wave >> 1;
-    d4 accr[2][2], acci[2][2];
+    acc_t accr[2][2], acci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            accr[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-            acci[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+            accr[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
+            acci[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
         }
     for (int kc = 0; kc < kw; kc += kKC) {
 #pragma unroll
@@ -256,19 +300,19 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_
                 const int r = q & 63, k = q >> 6;
                 const int64_t gr = tr + r;
                 sA[k][r] = (gr < r_hi && kc + k < kw) ? M[gr + (k_lo + kc + k) * ld]
-                                                      : make_double2(0.0, 0.0);
+                                                      : mkc<C2>(T(0), T(0));
             }
             {
                 const int k = q & 15, c = q >> 4;
                 const int64_t gc = tc + c;
                 sB[k][c] = (gc < c_hi && kc + k < kw) ? M[(k_lo + kc + k) + gc * ld]
-                                                      : make_double2(0.0, 0.0);
+                                                      : mkc<C2>(T(0), T(0));
             }
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kKC; kk += 4) {
-            double2 av[2], bv[2];
+            C2 av[2], bv[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) av[a] = sA[kk + (l >> 4)][32 * wr + 16 * a + (l & 15)];
 #pragma unroll
@@ -277,10 +321,10 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, accr[a][b], 0, 0, 0);
-                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[a].y, bv[b].y, accr[a][b], 0, 0, 0);
-                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].y, acci[a][b], 0, 0, 0);
-                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].x, acci[a][b], 0, 0, 0);
+                    accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
+                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
                 }
         }
         __syncthreads();
@@ -293,11 +337,11 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_
             if (gc >= c_hi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = tr + 32 * wr + 16 * a + 4 * r + (l >> 4);
+                const int64_t gr = tr + 32 * wr + 16 * a + Cx<T>::row(r, l);
                 if (gr >= r_hi) continue;
-                double2* o = M + gr + gc * ld;
-                const double2 v = *o;
-                *o = make_double2(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+                C2* o = M + gr + gc * ld;
+                const C2 v = *o;
+                *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
             }
         }
 }
@@ -305,43 +349,48 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double2* __restrict__ A, int64_
 // ---------------------------------------------------------------------------
 // host-side recursion
 // ---------------------------------------------------------------------------
+template <class T>
 struct LuPlan {
+    typedef typename Cx<T>::t C2;
     Context* ctx;
-    double2* A;
+    C2* A;
     int64_t n, ncols, ld, stride, batch;
     int64_t* ipiv;
     int* info;
 };
 
-static int lu_swap(const LuPlan& P, int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
+template <class T>
+static int lu_swap(const LuPlan<T>& P, int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
     if (c_hi <= c_lo || npiv <= 0) return MH_OK;
     const int64_t nc = c_hi - c_lo;
     ProfScope ps(P.ctx, "lu_swap", 0.0, 64.0 * npiv * (double)nc * P.batch);
-    hipLaunchKernelGGL(k_lu_swap, dim3((unsigned)((nc + 255) / 256), (unsigned)P.batch), dim3(256), 0,
+    hipLaunchKernelGGL(k_lu_swap<typename Cx<T>::t>, dim3((unsigned)((nc + 255) / 256), (unsigned)P.batch), dim3(256), 0,
                        P.ctx->stream, P.A, P.ld, P.stride, P.n, P.ipiv, k0, npiv, c_lo, c_hi);
     return MH_OK;
 }
 
-static int lu_gemm(const LuPlan& P, int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
+template <class T>
+static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
                    int64_t k_lo, int kw) {
     if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
-    ProfScope ps(P.ctx, "lu_gemm", 8.0 * (double)mr * (double)mc * kw * P.batch,
-                 16.0 * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
-    hipLaunchKernelGGL(k_lu_gemm, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
+    ProfScope ps(P.ctx, sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32", 8.0 * (double)mr * (double)mc * kw * P.batch,
+                 2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
+    hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
                        dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
     return MH_OK;
 }
 
 // X = L[k0:k0+w, k0:k0+w]^{-1} X on columns [c_lo, c_hi), blocked by 32 rows
-static int lu_trsm(const LuPlan& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+template <class T>
+static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
     if (c_hi <= c_lo) return MH_OK;
     const int64_t nc = c_hi - c_lo;
     for (int s = 0; s < w; s += kTrsmNB) {
         const int t = (w - s) < kTrsmNB ? (w - s) : kTrsmNB;
         {
             ProfScope ps(P.ctx, "lu_trsm", 4.0 * t * t * (double)nc * P.batch, 32.0 * t * (double)nc * P.batch);
-            hipLaunchKernelGGL(k_lu_trsm, dim3((unsigned)((nc + 7) / 8), (unsigned)P.batch), dim3(256), 0,
+            hipLaunchKernelGGL(k_lu_trsm<T>, dim3((unsigned)((nc + 7) / 8), (unsigned)P.batch), dim3(256), 0,
                                P.ctx->stream, P.A, P.ld, P.stride, k0 + s, t, c_lo, c_hi);
         }
         if (s + t < w) TRY_RC(lu_gemm(P, k0 + s + t, k0 + w, c_lo, c_hi, k0 + s, t));
@@ -349,24 +398,26 @@ static int lu_trsm(const LuPlan& P, int64_t k0, int w, int64_t c_lo, int64_t c_h
     return MH_OK;
 }
 
-static int lu_leaf(const LuPlan& P, int64_t k0, int w) {
+template <class T>
+static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
     const int64_t m = P.n - k0;
     ProfScope ps(P.ctx, "lu_leaf", 8.0 * (double)m * w * w / 2.0 * P.batch, 32.0 * (double)m * w * P.batch);
     if (m <= 1024)
-        hipLaunchKernelGGL((k_lu_leaf<1>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+        hipLaunchKernelGGL((k_lu_leaf<T, 1>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
                            P.ld, P.stride, k0, w, P.ipiv, P.info);
     else if (m <= 2048)
-        hipLaunchKernelGGL((k_lu_leaf<2>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+        hipLaunchKernelGGL((k_lu_leaf<T, 2>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
                            P.ld, P.stride, k0, w, P.ipiv, P.info);
     else
-        hipLaunchKernelGGL((k_lu_leaf<4>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
+        hipLaunchKernelGGL((k_lu_leaf<T, 4>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
                            P.ld, P.stride, k0, w, P.ipiv, P.info);
     return MH_OK;
 }
 
 // recursive factorisation of the panel [k0, k0+w) (rows k0..n), touching only
 // the panel's columns (zgetrf2 order)
-static int lu_panel(const LuPlan& P, int64_t k0, int w) {
+template <class T>
+static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
     if (w <= kLeaf) return lu_leaf(P, k0, w);
     int h = ((w / 2) + kLeaf - 1) / kLeaf * kLeaf;
     if (h >= w) h = w - kLeaf;
@@ -379,12 +430,13 @@ static int lu_panel(const LuPlan& P, int64_t k0, int w) {
     return MH_OK;
 }
 
-int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
-              int64_t stride, int64_t* ipiv, int* d_info) {
+template <class T>
+static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, typename Cx<T>::t* A,
+                       int64_t ld, int64_t stride, int64_t* ipiv, int* d_info) {
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
-    LuPlan P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info};
+    LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info};
     for (int64_t k0 = 0; k0 < n; k0 += kNBOuter) {
         const int w = (int)((n - k0) < kNBOuter ? (n - k0) : kNBOuter);
         TRY_RC(lu_panel(P, k0, w));
@@ -395,6 +447,16 @@ int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A,
     }
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
+}
+
+int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
+              int64_t stride, int64_t* ipiv, int* d_info) {
+    return lu_factor_t<double>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info);
+}
+
+int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
+                  int64_t stride, int64_t* ipiv, int* d_info) {
+    return lu_factor_t<float>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,7 +474,8 @@ int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A,
 constexpr int kMaxSolveN = 4096;
 constexpr int kPerThread = kMaxSolveN / 256;
 
-__device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, double2* __restrict__ b,
+template <class C2>
+__device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, C2* __restrict__ b,
                              bool forward, int* s_perm, int* s_inv) {
     const int tid = threadIdx.x;
     for (int i = tid; i < n; i += 256) s_perm[i] = i;
@@ -434,7 +497,7 @@ __device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, double2
         __syncthreads();
     }
     const int* dst = forward ? s_inv : s_perm;
-    double2 v[kPerThread];
+    C2 v[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const int j = tid + 256 * q;
@@ -451,18 +514,20 @@ __device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, double2
 
 constexpr int kSB = 32;
 
-__global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __restrict__ A,
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const typename Cx<T>::t* __restrict__ A,
                                                   int64_t ld, int64_t stride,
                                                   const int64_t* __restrict__ ipiv, int trans, int part,
-                                                  double2* __restrict__ B, int64_t ldb,
+                                                  typename Cx<T>::t* __restrict__ B, int64_t ldb,
                                                   int64_t bstride) {
+    typedef typename Cx<T>::t C2;
     const int64_t item = blockIdx.y;
-    const double2* M = A + item * stride;
+    const C2* M = A + item * stride;
     const int64_t* piv = ipiv + item * n;
-    double2* b = B + item * bstride + (int64_t)blockIdx.x * ldb;
+    C2* b = B + item * bstride + (int64_t)blockIdx.x * ldb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ double2 sD[kSB][kSB + 1];  // diagonal block, sD[r][c]
-    __shared__ double2 sx[kSB];
+    __shared__ C2 sD[kSB][kSB + 1];  // diagonal block, sD[r][c]
+    __shared__ C2 sx[kSB];
     __shared__ int s_perm[kMaxSolveN], s_inv[kMaxSolveN];
     const bool lower = part & 1, upper = part & 2;
     if (!trans) {
@@ -472,13 +537,13 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 const int jn = (int)((n - j0) < kSB ? (n - j0) : kSB);
                 for (int q = tid; q < kSB * kSB; q += 256) {
                     const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : make_double2(0.0, 0.0);
+                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : mkc<C2>(T(0), T(0));
                 }
                 __syncthreads();
                 if (wave == 0) {
-                    double2 v = (lane < jn) ? b[j0 + lane] : make_double2(0.0, 0.0);
+                    C2 v = (lane < jn) ? b[j0 + lane] : mkc<C2>(T(0), T(0));
                     for (int c = 0; c < jn; ++c) {
-                        const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        const C2 yc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
                         if (lane > c && lane < jn) v = cfms(v, sD[lane][c], yc);
                     }
                     if (lane < jn) {
@@ -488,7 +553,7 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 }
                 __syncthreads();
                 for (int64_t i = j0 + jn + tid; i < n; i += 256) {
-                    double2 v = b[i];
+                    C2 v = b[i];
                     for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
                     b[i] = v;
                 }
@@ -501,14 +566,14 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 const int jn = (int)(j1 - j0);
                 for (int q = tid; q < kSB * kSB; q += 256) {
                     const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : make_double2(1.0, 0.0);
+                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : mkc<C2>(T(1), T(0));
                 }
                 __syncthreads();
                 if (wave == 0) {
-                    double2 v = (lane < jn) ? b[j0 + lane] : make_double2(0.0, 0.0);
+                    C2 v = (lane < jn) ? b[j0 + lane] : mkc<C2>(T(0), T(0));
                     for (int c = jn - 1; c >= 0; --c) {
                         if (lane == c) v = cdiv(v, sD[c][c]);
-                        const double2 xc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        const C2 xc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
                         if (lane < c) v = cfms(v, sD[lane][c], xc);
                     }
                     if (lane < jn) {
@@ -518,7 +583,7 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 }
                 __syncthreads();
                 for (int64_t i = tid; i < j0; i += 256) {
-                    double2 v = b[i];
+                    C2 v = b[i];
                     for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
                     b[i] = v;
                 }
@@ -532,9 +597,9 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 const int in = (int)((n - i0) < kSB ? (n - i0) : kSB);
                 for (int r = wave; r < in; r += 4) {
                     const int64_t i = i0 + r;
-                    double sx_ = 0.0, sy_ = 0.0;
+                    T sx_ = T(0), sy_ = T(0);
                     for (int64_t j = lane; j < i0; j += 64) {
-                        const double2 u = M[j + i * ld], y = b[j];
+                        const C2 u = M[j + i * ld], y = b[j];
                         sx_ += u.x * y.x - u.y * y.y;
                         sy_ += u.x * y.y + u.y * y.x;
                     }
@@ -542,18 +607,18 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                         sx_ += __shfl_xor(sx_, off);
                         sy_ += __shfl_xor(sy_, off);
                     }
-                    if (lane == 0) b[i] = make_double2(b[i].x - sx_, b[i].y - sy_);
+                    if (lane == 0) b[i] = mkc<C2>(b[i].x - sx_, b[i].y - sy_);
                 }
                 for (int q = tid; q < kSB * kSB; q += 256) {
                     const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : make_double2(1.0, 0.0);
+                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : mkc<C2>(T(1), T(0));
                 }
                 __syncthreads();
                 if (wave == 0) {
-                    double2 v = (lane < in) ? b[i0 + lane] : make_double2(0.0, 0.0);
+                    C2 v = (lane < in) ? b[i0 + lane] : mkc<C2>(T(0), T(0));
                     for (int c = 0; c < in; ++c) {
                         if (lane == c) v = cdiv(v, sD[c][c]);
-                        const double2 yc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        const C2 yc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
                         if (lane > c && lane < in) v = cfms(v, sD[c][lane], yc);
                     }
                     if (lane < in) b[i0 + lane] = v;
@@ -568,9 +633,9 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                 const int in = (int)(i1 - i0);
                 for (int r = wave; r < in; r += 4) {
                     const int64_t i = i0 + r;
-                    double sx_ = 0.0, sy_ = 0.0;
+                    T sx_ = T(0), sy_ = T(0);
                     for (int64_t j = i1 + lane; j < n; j += 64) {
-                        const double2 lv = M[j + i * ld], z = b[j];
+                        const C2 lv = M[j + i * ld], z = b[j];
                         sx_ += lv.x * z.x - lv.y * z.y;
                         sy_ += lv.x * z.y + lv.y * z.x;
                     }
@@ -578,17 +643,17 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
                         sx_ += __shfl_xor(sx_, off);
                         sy_ += __shfl_xor(sy_, off);
                     }
-                    if (lane == 0) b[i] = make_double2(b[i].x - sx_, b[i].y - sy_);
+                    if (lane == 0) b[i] = mkc<C2>(b[i].x - sx_, b[i].y - sy_);
                 }
                 for (int q = tid; q < kSB * kSB; q += 256) {
                     const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : make_double2(0.0, 0.0);
+                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : mkc<C2>(T(0), T(0));
                 }
                 __syncthreads();
                 if (wave == 0) {
-                    double2 v = (lane < in) ? b[i0 + lane] : make_double2(0.0, 0.0);
+                    C2 v = (lane < in) ? b[i0 + lane] : mkc<C2>(T(0), T(0));
                     for (int c = in - 1; c >= 0; --c) {
-                        const double2 zc = make_double2(__shfl(v.x, c), __shfl(v.y, c));
+                        const C2 zc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
                         if (lane < c) v = cfms(v, sD[c][lane], zc);
                     }
                     if (lane < in) b[i0 + lane] = v;
@@ -600,19 +665,34 @@ __global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const double2* __re
     }
 }
 
-int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
-                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
-                  double2* B, int64_t ldb, int64_t bstride) {
+template <class T>
+static int lu_solve_part_t(Context* ctx, int64_t n, int64_t batch, const typename Cx<T>::t* LU,
+                           int64_t ld, int64_t stride, const int64_t* ipiv, int trans, int part,
+                           int64_t nrhs, typename Cx<T>::t* B, int64_t ldb, int64_t bstride) {
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535 && nrhs <= 65535, "lu_solve: batch or nrhs too large");
     MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
     const double tri = ((part & 1) ? 0.5 : 0.0) + ((part & 2) ? 0.5 : 0.0);
-    ProfScope ps(ctx, "lu_solve", 8.0 * tri * (double)n * n * nrhs * batch,
-                 16.0 * tri * (double)n * n * nrhs * batch);
-    hipLaunchKernelGGL(k_lu_solve, dim3((unsigned)nrhs, (unsigned)batch), dim3(256), 0, ctx->stream, n, LU,
+    ProfScope ps(ctx, sizeof(T) == 8 ? "lu_solve" : "lu_solve_f32", 8.0 * tri * (double)n * n * nrhs * batch,
+                 2.0 * sizeof(T) * tri * (double)n * n * nrhs * batch);
+    hipLaunchKernelGGL(k_lu_solve<T>, dim3((unsigned)nrhs, (unsigned)batch), dim3(256), 0, ctx->stream, n, LU,
                        ld, stride, ipiv, trans, part, B, ldb, bstride);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
+}
+
+int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
+                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
+                  double2* B, int64_t ldb, int64_t bstride) {
+    return lu_solve_part_t<double>(ctx, n, batch, LU, ld, stride, ipiv, trans, part, nrhs, B, ldb,
+                                   bstride);
+}
+
+int lu_solve_part_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld,
+                      int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
+                      float2* B, int64_t ldb, int64_t bstride) {
+    return lu_solve_part_t<float>(ctx, n, batch, LU, ld, stride, ipiv, trans, part, nrhs, B, ldb,
+                                  bstride);
 }
 
 int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
@@ -631,7 +711,7 @@ __global__ __launch_bounds__(256) void k_zgemv_multi(int64_t n, int64_t nrhs,
                                                      int64_t stride, const double2* __restrict__ X,
                                                      int64_t ldx, int64_t xstride,
                                                      double2* __restrict__ Y, int64_t ldy,
-                                                     int64_t ystride) {
+                                                     int64_t ystride, const double2* __restrict__ Z) {
     const int64_t item = blockIdx.y;
     const double2* Sm = S + item * stride;
     const double2* Xm = X + item * xstride;
@@ -664,9 +744,17 @@ __global__ __launch_bounds__(256) void k_zgemv_multi(int64_t n, int64_t nrhs,
         }
     }
     if (i < n) {
+        const double2* Zm = Z ? Z + item * ystride : nullptr;
 #pragma unroll
         for (int r = 0; r < kMaxRhs; ++r)
-            if (r < nrhs) Ym[i + r * ldy] = acc[r];
+            if (r < nrhs) {
+                if (Zm) {
+                    const double2 zv = Zm[i + r * ldy];
+                    Ym[i + r * ldy] = make_double2(zv.x - acc[r].x, zv.y - acc[r].y);
+                } else {
+                    Ym[i + r * ldy] = acc[r];
+                }
+            }
     }
 }
 
@@ -677,7 +765,7 @@ __global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
                                                        int64_t stride, const double2* __restrict__ X,
                                                        int64_t ldx, int64_t xstride,
                                                        double2* __restrict__ Y, int64_t ldy,
-                                                       int64_t ystride) {
+                                                       int64_t ystride, const double2* __restrict__ Z) {
     const int64_t item = blockIdx.y;
     const double2* Sm = S + item * stride;
     const double2* Xm = X + item * xstride;
@@ -698,29 +786,123 @@ __global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
             sr += __shfl_xor(sr, off);
             si += __shfl_xor(si, off);
         }
-        if (lane == 0) Ym[i + r * ldy] = make_double2(sr, si);
+        if (lane == 0) {
+            if (Z) {
+                const double2 zv = Z[item * ystride + i + r * ldy];
+                Ym[i + r * ldy] = make_double2(zv.x - sr, zv.y - si);
+            } else {
+                Ym[i + r * ldy] = make_double2(sr, si);
+            }
+        }
     }
 }
 
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
-                     double2* Y, int64_t ldy, int64_t ystride) {
+                     double2* Y, int64_t ldy, int64_t ystride, const double2* Z) {
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
     MH_REQUIRE(nrhs <= kMaxRhs, "matvec: at most 16 right-hand sides per call");
     ProfScope ps(ctx, "zgemv", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * batch);
     hipLaunchKernelGGL(k_zgemv_multi, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256),
-                       0, ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride);
+                       0, ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride, Z);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
 
 int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
-                     double2* Y, int64_t ldy, int64_t ystride) {
+                     double2* Y, int64_t ldy, int64_t ystride, const double2* Z) {
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
     ProfScope ps(ctx, "zgemv_t", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * nrhs * batch);
     hipLaunchKernelGGL(k_zgemv_t_multi, dim3((unsigned)((n + 3) / 4), (unsigned)batch), dim3(256), 0,
-                       ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride);
+                       ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride, Z);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP32 copies, updates and residual norms for iterative
+// refinement of an FP32 LU against the FP64 matrix
+// ---------------------------------------------------------------------------
+__global__ void k_to_f32(int64_t rows, int64_t cols, const double2* __restrict__ src, int64_t lds,
+                         int64_t sstride, float2* __restrict__ dst, int64_t ldd, int64_t dstride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = blockIdx.y;
+    if (t >= rows * cols) return;
+    const int64_t i = t % rows, j = t / rows;
+    const double2 v = src[item * sstride + i + j * lds];
+    dst[item * dstride + i + j * ldd] = make_float2((float)v.x, (float)v.y);
+}
+
+// x += upcast(d)
+__global__ void k_add_f32(int64_t rows, int64_t cols, const float2* __restrict__ d, int64_t ldd,
+                          int64_t dstride, double2* __restrict__ x, int64_t ldx, int64_t xstride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = blockIdx.y;
+    if (t >= rows * cols) return;
+    const int64_t i = t % rows, j = t / rows;
+    const float2 v = d[item * dstride + i + j * ldd];
+    double2& o = x[item * xstride + i + j * ldx];
+    o.x += (double)v.x;
+    o.y += (double)v.y;
+}
+
+// out[item*cols + j] = max_i |r_ij|_1 / max_i |b_ij|_1   (one workgroup per column)
+__global__ __launch_bounds__(256) void k_rel_residual(int64_t rows, const double2* __restrict__ r,
+                                                      const double2* __restrict__ b, int64_t ld,
+                                                      int64_t stride, double* __restrict__ out) {
+    const int64_t item = blockIdx.y, j = blockIdx.x;
+    const double2* rc = r + item * stride + j * ld;
+    const double2* bc = b + item * stride + j * ld;
+    double mr = 0.0, mb = 0.0;
+    for (int64_t i = threadIdx.x; i < rows; i += 256) {
+        mr = fmax(mr, fabs(rc[i].x) + fabs(rc[i].y));
+        mb = fmax(mb, fabs(bc[i].x) + fabs(bc[i].y));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = fmax(mr, __shfl_xor(mr, off));
+        mb = fmax(mb, __shfl_xor(mb, off));
+    }
+    __shared__ double s[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s[0][wave] = mr;
+        s[1][wave] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double R = fmax(fmax(s[0][0], s[0][1]), fmax(s[0][2], s[0][3]));
+        const double B = fmax(fmax(s[1][0], s[1][1]), fmax(s[1][2], s[1][3]));
+        out[item * gridDim.x + j] = (B > 0.0) ? R / B : R;
+    }
+}
+
+int convert_to_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* src,
+                   int64_t lds, int64_t sstride, float2* dst, int64_t ldd, int64_t dstride) {
+    const int64_t tot = rows * cols;
+    if (tot == 0 || batch == 0) return MH_OK;
+    ProfScope ps(ctx, "to_f32", 0.0, 24.0 * (double)tot * batch);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((tot + 255) / 256), (unsigned)batch), dim3(256), 0,
+                       ctx->stream, rows, cols, src, lds, sstride, dst, ldd, dstride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int add_from_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const float2* d,
+                 int64_t ldd, int64_t dstride, double2* x, int64_t ldx, int64_t xstride) {
+    const int64_t tot = rows * cols;
+    if (tot == 0 || batch == 0) return MH_OK;
+    hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((tot + 255) / 256), (unsigned)batch), dim3(256), 0,
+                       ctx->stream, rows, cols, d, ldd, dstride, x, ldx, xstride);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int rel_residual(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* r,
+                 const double2* b, int64_t ld, int64_t stride, double* out) {
+    if (rows == 0 || cols == 0 || batch == 0) return MH_OK;
+    hipLaunchKernelGGL(k_rel_residual, dim3((unsigned)cols, (unsigned)batch), dim3(256), 0, ctx->stream,
+                       rows, r, b, ld, stride, out);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
