@@ -119,6 +119,9 @@ int mh_context_destroy(mh_context_t ctx) {
     (void)hipStreamSynchronize(c->stream);
     if (c->ws) (void)hipFree(c->ws);
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->tri_sync) (void)hipFree(c->tri_sync);
+    if (c->tri_err) (void)hipFree(c->tri_err);
+    if (c->tri_ws) (void)hipFree(c->tri_ws);
     delete c;
     return MH_OK;
 }
@@ -275,24 +278,36 @@ int mh_lu_solve(mh_context_t ctx, int64_t n, int64_t batch, const double* LU, in
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
     MH_REQUIRE(LU && ipiv && B, "lu_solve: null array");
     MH_REQUIRE(ld >= n && ldb >= n, "lu_solve: leading dimension too small");
-    return lu_solve(CTX(ctx), n, batch, reinterpret_cast<const double2*>(LU), ld, stride, ipiv,
-                    trans ? 1 : 0, nrhs, reinterpret_cast<double2*>(B), ldb, bstride);
+    Context* c = CTX(ctx);
+    Bump bump;
+    const size_t o_perm = bump.take(sizeof(int) * (size_t)(n * batch));
+    const size_t o_inv = bump.take(sizeof(double2) * (size_t)tri_inv_elems(n, batch));
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    int* perm = reinterpret_cast<int*>(static_cast<char*>(c->ws) + o_perm);
+    double2* inv = reinterpret_cast<double2*>(static_cast<char*>(c->ws) + o_inv);
+    const double2* LU2 = reinterpret_cast<const double2*>(LU);
+    TRY(tri_prepare(c, n, batch, LU2, ld, stride, ipiv, perm, inv));
+    TRY(tri_solve(c, n, batch, LU2, ld, stride, perm, inv, trans ? 1 : 0, 3, nrhs,
+                  reinterpret_cast<double2*>(B), ldb, bstride));
+    return tri_check(c);
 }
 
 // shared by mh_solve and mh_freq_sweep: LU + solve for `batch` systems whose
 // right-hand sides are already in rhs (overwritten by phi)
 static int solve_impl(Context* c, int64_t n, int64_t batch, int direct, double2* DK,
                       const double2* S, int64_t ld, int64_t stride, double2* rhs, int64_t nrhs,
-                      int64_t ldb, int64_t bstride, double2* tmp, int64_t* ipiv, int* info) {
+                      int64_t ldb, int64_t bstride, double2* tmp, int64_t* ipiv, int* perm, double2* inv,
+                      int* info) {
     MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * batch, c->stream));
     TRY(lu_factor(c, n, n, batch, DK, ld, stride, ipiv, info));
+    TRY(tri_prepare(c, n, batch, DK, ld, stride, ipiv, perm, inv));
     if (direct) {  // phi = D \ (S bc)
         TRY(zgemm_nn_batched(c, n, nrhs, batch, S, ld, stride, rhs, ldb, bstride, tmp, ldb, bstride));
-        TRY(lu_solve(c, n, batch, DK, ld, stride, ipiv, 0, nrhs, tmp, ldb, bstride));
+        TRY(tri_solve(c, n, batch, DK, ld, stride, perm, inv, 0, 3, nrhs, tmp, ldb, bstride));
         MH_HIP_CHECK(hipMemcpyAsync(rhs, tmp, sizeof(double2) * (size_t)(bstride * batch),
                                     hipMemcpyDeviceToDevice, c->stream));
     } else {  // sigma = K \ bc; phi = S sigma
-        TRY(lu_solve(c, n, batch, DK, ld, stride, ipiv, 0, nrhs, rhs, ldb, bstride));
+        TRY(tri_solve(c, n, batch, DK, ld, stride, perm, inv, 0, 3, nrhs, rhs, ldb, bstride));
         TRY(zgemm_nn_batched(c, n, nrhs, batch, S, ld, stride, rhs, ldb, bstride, tmp, ldb, bstride));
         MH_HIP_CHECK(hipMemcpyAsync(rhs, tmp, sizeof(double2) * (size_t)(bstride * batch),
                                     hipMemcpyDeviceToDevice, c->stream));
@@ -327,6 +342,8 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
     const size_t o_tmp = bump.take(sizeof(double2) * (size_t)(bs * batch));
     const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * batch));
     const size_t o_info = bump.take(sizeof(int) * (size_t)batch);
+    const size_t o_perm = bump.take(sizeof(int) * (size_t)(n * batch));
+    const size_t o_inv = bump.take(sizeof(double2) * (size_t)tri_inv_elems(n, batch));
     if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
     char* ws = static_cast<char*>(c->ws);
     if (phi != bc)
@@ -335,7 +352,9 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
     TRY(solve_impl(c, n, batch, direct, reinterpret_cast<double2*>(DK),
                    reinterpret_cast<const double2*>(S), ld, stride, reinterpret_cast<double2*>(phi),
                    nrhs, ldb, bs, reinterpret_cast<double2*>(ws + o_tmp),
-                   reinterpret_cast<int64_t*>(ws + o_piv), reinterpret_cast<int*>(ws + o_info)));
+                   reinterpret_cast<int64_t*>(ws + o_piv), reinterpret_cast<int*>(ws + o_perm),
+                   reinterpret_cast<double2*>(ws + o_inv), reinterpret_cast<int*>(ws + o_info)));
+    TRY(tri_check(c));
     return check_info(c, reinterpret_cast<int*>(ws + o_info), batch);
 }
 
@@ -345,7 +364,7 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
 // below 1e-12 (at most 12 sweeps).  Bm, X, R share one layout (ldb, bstride);
 // R32 is n x nrhs per item.
 static int ir_solve(Context* c, int64_t n, int64_t batch, int trans, const double2* A64, int64_t ld,
-                    int64_t stride, const float2* LU32, int64_t stride32, const int64_t* piv,
+                    int64_t stride, const float2* LU32, int64_t stride32, const int* perm, const float2* inv32,
                     const double2* Bm, int64_t nrhs, int64_t ldb, int64_t bstride, double2* X,
                     double2* R, float2* R32, double* d_nrm) {
     const int64_t s32 = n * nrhs;
@@ -355,7 +374,7 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, int trans, const doubl
     std::vector<double> h(batch * nrhs);
     for (int it = 0; it < 12; ++it) {
         TRY(convert_to_f32(c, n, nrhs, batch, R, ldb, bstride, R32, n, s32));
-        TRY(lu_solve_part_f32(c, n, batch, LU32, n, stride32, piv, trans, 3, nrhs, R32, n, s32));
+        TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, trans, 3, nrhs, R32, n, s32));
         TRY(add_from_f32(c, n, nrhs, batch, R32, n, s32, X, ldb, bstride));
         if (trans)
             TRY(zgemm_tn_batched(c, n, nrhs, batch, A64, ld, stride, X, ldb, bstride, R, ldb, bstride, Bm));
@@ -407,6 +426,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     const size_t o_phi = bump.take(sizeof(double2) * (size_t)(bstride * chunk));
     const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * chunk));
     const size_t o_info = bump.take(sizeof(int) * (size_t)chunk);
+    const size_t o_perm = bump.take(sizeof(int) * (size_t)(n * chunk));
+    const size_t o_inv = bump.take(sizeof(double2) * (size_t)tri_inv_elems(n, chunk));
     const size_t o_om = bump.take(sizeof(double) * (size_t)nomega);
     const size_t o_k = bump.take(sizeof(double) * (size_t)nomega);
     const size_t o_be = bump.take(sizeof(double) * (size_t)(nbeta > 0 ? nbeta : 1));
@@ -443,6 +464,9 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     double2* phi = reinterpret_cast<double2*>(ws + o_phi);
     int64_t* piv = reinterpret_cast<int64_t*>(ws + o_piv);
     int* info = reinterpret_cast<int*>(ws + o_info);
+    int* perm = reinterpret_cast<int*>(ws + o_perm);
+    double2* inv = reinterpret_cast<double2*>(ws + o_inv);
+    float2* inv32 = reinterpret_cast<float2*>(ws + o_inv);
     double* d_om = reinterpret_cast<double*>(ws + o_om);
     double* d_k = reinterpret_cast<double*>(ws + o_k);
     double* d_be = reinterpret_cast<double*>(ws + o_be);
@@ -471,12 +495,13 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             TRY(convert_to_f32(c, n, n, nw, dD, ld, stride, D32, n, n * n));
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
+            TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
             if (direct) {  // phi = D \ (S bc)
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
-                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, piv, ybuf, nr, ldb, bstride, phi, Rb,
+                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, perm, inv32, ybuf, nr, ldb, bstride, phi, Rb,
                              R32, d_nrm));
             } else {  // sigma = K \ bc (kept in ybuf); phi = S sigma
-                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, piv, rhs, nr, ldb, bstride, ybuf, Rb,
+                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, perm, inv32, rhs, nr, ldb, bstride, ybuf, Rb,
                              R32, d_nrm));
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
             }
@@ -490,7 +515,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         if (!mixed) {
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
-            TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 0, 2, nr, aug, ld, stride));
+            TRY(tri_prepare(c, n, nw, dD, ld, stride, piv, (ga && ndof) ? perm : nullptr, inv));
+            TRY(tri_solve(c, n, nw, dD, ld, stride, nullptr, inv, 0, 2, nr, aug, ld, stride));
             if (direct) {
                 MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
                                               sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
@@ -504,6 +530,7 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         double2* Fw = F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr;
         TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, phi, ldb, bstride, Aw,
                           Bw, Fw));
+        TRY(tri_check(c));
         TRY(check_info(c, info, nw));
         if (!ga || ndof == 0) continue;
         // ---- gradients (grad.hip) ----
@@ -521,21 +548,21 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         int64_t wss;
         if (direct) {  // lambda = D^-T c; mu = S^T lambda
             if (mixed) {
-                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, piv, cw, ndof, n, wa, lam, Rb, R32, d_nrm));
+                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, perm, inv32, cw, ndof, n, wa, lam, Rb, R32, d_nrm));
             } else {
                 MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
-                TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
             }
             TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
             aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
         } else {       // lambda = K^-T S^T c
             if (mixed) {
                 TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, mu, n, wa));
-                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, piv, mu, ndof, n, wa, lam, Rb, R32, d_nrm));
+                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, perm, inv32, mu, ndof, n, wa, lam, Rb, R32, d_nrm));
                 aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
             } else {
                 TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
-                TRY(lu_solve_part(c, n, nw, dD, ld, stride, piv, 1, 3, ndof, lam, n, wa));
+                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
                 aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
             }
         }

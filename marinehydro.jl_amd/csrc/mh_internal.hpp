@@ -30,6 +30,14 @@ struct Context {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     int* d_flag = nullptr;  // device error / info flag (one int)
+    // triangular solves: [ticket u64 | pad | per-block flags u32], epoch of
+    // the last launch, error word, out-of-place solution workspace
+    void* tri_sync = nullptr;
+    size_t tri_sync_bytes = 0;
+    unsigned tri_epoch = 0;
+    int* tri_err = nullptr;
+    void* tri_ws = nullptr;
+    size_t tri_ws_bytes = 0;
     bool profiling = false;
     std::vector<ProfRec> prof;
 };
@@ -93,13 +101,29 @@ int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const d
 // ---- launchers (solve.hip) --------------------------------------------------
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
               int64_t stride, int64_t* ipiv, int* d_info);
-int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
-             int64_t stride, const int64_t* ipiv, int trans, int64_t nrhs, double2* B,
-             int64_t ldb, int64_t bstride);
-// part: 1 = forward (pivots + unit L), 2 = back (U), 3 = both
-int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
-                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
-                  double2* B, int64_t ldb, int64_t bstride);
+// ---- launchers (trsv.hip) ---------------------------------------------------
+constexpr int kMaxSolveN = 4096;
+// After an LU factorisation: perm[b*n + i], the net row permutation of the
+// interchanges ipiv ((P b)[i] = b[perm[i]]; skipped when perm is null), and
+// the inverses of the 32 x 32 diagonal blocks of L and U that the solves use
+// (inv holds tri_inv_elems(n, batch) elements).
+int64_t tri_inv_elems(int64_t n, int64_t batch);
+int tri_prepare(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+                const int64_t* ipiv, int* perm, double2* inv);
+int tri_prepare_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                    const int64_t* ipiv, int* perm, float2* inv);
+// Solves with the factors of A = P L U, in place on B (n x nrhs per item):
+// trans 0: A X = B, 1: A^T X = B; part 1 = L (with P), 2 = U, 3 = both.
+int tri_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+              const int* perm, const double2* inv, int trans, int part, int64_t nrhs, double2* B,
+              int64_t ldb, int64_t bstride);
+int tri_solve_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                  const int* perm, const float2* inv, int trans, int part, int64_t nrhs, float2* B,
+                  int64_t ldb, int64_t bstride);
+// Reports (and clears) a failed inter-workgroup wait or a bad pivot index of
+// the solves issued so far; synchronises the context stream.
+int tri_check(Context* ctx);
+
 // Y[b] = S[b]^T * X[b]
 int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
@@ -112,9 +136,6 @@ int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
 // FP32 factorisation and solves (mixed precision)
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
                   int64_t stride, int64_t* ipiv, int* d_info);
-int lu_solve_part_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld,
-                      int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
-                      float2* B, int64_t ldb, int64_t bstride);
 int convert_to_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* src,
                    int64_t lds, int64_t sstride, float2* dst, int64_t ldd, int64_t dstride);
 int add_from_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const float2* d,

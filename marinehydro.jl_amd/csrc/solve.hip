@@ -15,70 +15,14 @@
 // right-hand side appended to the matrix leaves the factorisation already
 // forward-substituted.  Reference: the LAPACK factorisation behind `\` /
 // implicit_linear (src/matrix_assembly.jl:108-126).
+#include "mh_cx.hpp"
 #include "mh_internal.hpp"
 
 namespace mh {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// element traits: T = double (FP64 LU) or float (FP32 LU of the mixed path)
-template <class T>
-struct Cx;
-template <>
-struct Cx<double> {
-    typedef double2 t;
-    typedef d4 acc;
-    // v_mfma_f64_16x16x4f64: result register r of lane l is row 4r + l/16
-    __device__ __forceinline__ static int row(int r, int l) { return 4 * r + (l >> 4); }
-    __device__ __forceinline__ static acc mfma(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <>
-struct Cx<float> {
-    typedef float2 t;
-    typedef f4 acc;
-    // v_mfma_f32_16x16x4f32: result register r of lane l is row 4 (l/16) + r
-    __device__ __forceinline__ static int row(int r, int l) { return 4 * (l >> 4) + r; }
-    __device__ __forceinline__ static acc mfma(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-template <class C2>
-__device__ __forceinline__ C2 mkc(decltype(C2().x) a, decltype(C2().x) b) {
-    C2 o;
-    o.x = a;
-    o.y = b;
-    return o;
-}
-
 constexpr int kNBOuter = 128;  // block-column width of the trailing updates
 constexpr int kLeaf = 8;       // register-resident leaf width
 constexpr int kTrsmNB = 32;    // rows per triangular-solve block
-
-template <class C2>
-__device__ __forceinline__ C2 cmul(C2 a, C2 b) {
-    return mkc<C2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// a - b*c
-template <class C2>
-__device__ __forceinline__ C2 cfms(C2 a, C2 b, C2 c) {
-    return mkc<C2>(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
-}
-template <class C2>
-__device__ __forceinline__ C2 crecip(C2 a) {
-    // Smith's algorithm for 1/a
-    typedef decltype(C2().x) T;
-    if (fabs(a.x) >= fabs(a.y)) {
-        const T r = a.y / a.x, d = a.x + a.y * r;
-        return mkc<C2>(T(1) / d, -r / d);
-    }
-    const T r = a.x / a.y, d = a.y + a.x * r;
-    return mkc<C2>(r / d, T(-1) / d);
-}
-template <class C2>
-__device__ __forceinline__ C2 cdiv(C2 a, C2 b) { return cmul(a, crecip(b)); }
 
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= 8, held in registers.
@@ -467,240 +411,6 @@ int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2*
 // workgroup with coalesced column reads.
 //   part: 1 = lower (unit L, with the pivots), 2 = upper (U), 3 = both.
 // ---------------------------------------------------------------------------
-// Applies the row interchanges ipiv[0..n) to a vector b (global memory) with
-// the whole workgroup: the net permutation is built in LDS by one thread,
-// then every thread scatters its entries.  forward: b <- P b (interchanges in
-// order k = 0..n-1); otherwise the inverse (k = n-1..0).
-constexpr int kMaxSolveN = 4096;
-constexpr int kPerThread = kMaxSolveN / 256;
-
-template <class C2>
-__device__ void apply_pivots(int64_t n, const int64_t* __restrict__ piv, C2* __restrict__ b,
-                             bool forward, int* s_perm, int* s_inv) {
-    const int tid = threadIdx.x;
-    for (int i = tid; i < n; i += 256) s_perm[i] = i;
-    __syncthreads();
-    if (tid == 0)
-        for (int k = 0; k < n; ++k) {
-            const int p = (int)piv[k];
-            if (p != k) {
-                const int t = s_perm[k];
-                s_perm[k] = s_perm[p];
-                s_perm[p] = t;
-            }
-        }
-    __syncthreads();
-    // forward: b_new[i] = b_old[perm[i]]  <=>  b_new[inv[j]] = b_old[j]
-    // inverse: b_new[perm[i]] = b_old[i]
-    if (forward) {
-        for (int i = tid; i < n; i += 256) s_inv[s_perm[i]] = i;
-        __syncthreads();
-    }
-    const int* dst = forward ? s_inv : s_perm;
-    C2 v[kPerThread];
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-        const int j = tid + 256 * q;
-        if (j < n) v[q] = b[j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-        const int j = tid + 256 * q;
-        if (j < n) b[dst[j]] = v[q];
-    }
-    __syncthreads();
-}
-
-constexpr int kSB = 32;
-
-template <class T>
-__global__ __launch_bounds__(256) void k_lu_solve(int64_t n, const typename Cx<T>::t* __restrict__ A,
-                                                  int64_t ld, int64_t stride,
-                                                  const int64_t* __restrict__ ipiv, int trans, int part,
-                                                  typename Cx<T>::t* __restrict__ B, int64_t ldb,
-                                                  int64_t bstride) {
-    typedef typename Cx<T>::t C2;
-    const int64_t item = blockIdx.y;
-    const C2* M = A + item * stride;
-    const int64_t* piv = ipiv + item * n;
-    C2* b = B + item * bstride + (int64_t)blockIdx.x * ldb;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ C2 sD[kSB][kSB + 1];  // diagonal block, sD[r][c]
-    __shared__ C2 sx[kSB];
-    __shared__ int s_perm[kMaxSolveN], s_inv[kMaxSolveN];
-    const bool lower = part & 1, upper = part & 2;
-    if (!trans) {
-        if (lower) {
-            apply_pivots(n, piv, b, true, s_perm, s_inv);
-            for (int64_t j0 = 0; j0 < n; j0 += kSB) {  // L y = b
-                const int jn = (int)((n - j0) < kSB ? (n - j0) : kSB);
-                for (int q = tid; q < kSB * kSB; q += 256) {
-                    const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : mkc<C2>(T(0), T(0));
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    C2 v = (lane < jn) ? b[j0 + lane] : mkc<C2>(T(0), T(0));
-                    for (int c = 0; c < jn; ++c) {
-                        const C2 yc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
-                        if (lane > c && lane < jn) v = cfms(v, sD[lane][c], yc);
-                    }
-                    if (lane < jn) {
-                        b[j0 + lane] = v;
-                        sx[lane] = v;
-                    }
-                }
-                __syncthreads();
-                for (int64_t i = j0 + jn + tid; i < n; i += 256) {
-                    C2 v = b[i];
-                    for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
-                    b[i] = v;
-                }
-                __syncthreads();
-            }
-        }
-        if (upper) {
-            for (int64_t j1 = n; j1 > 0; j1 -= kSB) {  // U x = y
-                const int64_t j0 = (j1 - kSB) > 0 ? (j1 - kSB) : 0;
-                const int jn = (int)(j1 - j0);
-                for (int q = tid; q < kSB * kSB; q += 256) {
-                    const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < jn && c < jn) ? M[(j0 + r) + (j0 + c) * ld] : mkc<C2>(T(1), T(0));
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    C2 v = (lane < jn) ? b[j0 + lane] : mkc<C2>(T(0), T(0));
-                    for (int c = jn - 1; c >= 0; --c) {
-                        if (lane == c) v = cdiv(v, sD[c][c]);
-                        const C2 xc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
-                        if (lane < c) v = cfms(v, sD[lane][c], xc);
-                    }
-                    if (lane < jn) {
-                        b[j0 + lane] = v;
-                        sx[lane] = v;
-                    }
-                }
-                __syncthreads();
-                for (int64_t i = tid; i < j0; i += 256) {
-                    C2 v = b[i];
-                    for (int c = 0; c < jn; ++c) v = cfms(v, M[i + (j0 + c) * ld], sx[c]);
-                    b[i] = v;
-                }
-                __syncthreads();
-            }
-        }
-    } else {
-        if (upper) {
-            // U^T y = b (lower, non-unit): rows in blocks, dot products over columns of U
-            for (int64_t i0 = 0; i0 < n; i0 += kSB) {
-                const int in = (int)((n - i0) < kSB ? (n - i0) : kSB);
-                for (int r = wave; r < in; r += 4) {
-                    const int64_t i = i0 + r;
-                    T sx_ = T(0), sy_ = T(0);
-                    for (int64_t j = lane; j < i0; j += 64) {
-                        const C2 u = M[j + i * ld], y = b[j];
-                        sx_ += u.x * y.x - u.y * y.y;
-                        sy_ += u.x * y.y + u.y * y.x;
-                    }
-                    for (int off = 32; off > 0; off >>= 1) {
-                        sx_ += __shfl_xor(sx_, off);
-                        sy_ += __shfl_xor(sy_, off);
-                    }
-                    if (lane == 0) b[i] = mkc<C2>(b[i].x - sx_, b[i].y - sy_);
-                }
-                for (int q = tid; q < kSB * kSB; q += 256) {
-                    const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : mkc<C2>(T(1), T(0));
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    C2 v = (lane < in) ? b[i0 + lane] : mkc<C2>(T(0), T(0));
-                    for (int c = 0; c < in; ++c) {
-                        if (lane == c) v = cdiv(v, sD[c][c]);
-                        const C2 yc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
-                        if (lane > c && lane < in) v = cfms(v, sD[c][lane], yc);
-                    }
-                    if (lane < in) b[i0 + lane] = v;
-                }
-                __syncthreads();
-            }
-        }
-        if (lower) {
-            // L^T z = y (upper, unit): blocks from the bottom, dots over columns of L
-            for (int64_t i1 = n; i1 > 0; i1 -= kSB) {
-                const int64_t i0 = (i1 - kSB) > 0 ? (i1 - kSB) : 0;
-                const int in = (int)(i1 - i0);
-                for (int r = wave; r < in; r += 4) {
-                    const int64_t i = i0 + r;
-                    T sx_ = T(0), sy_ = T(0);
-                    for (int64_t j = i1 + lane; j < n; j += 64) {
-                        const C2 lv = M[j + i * ld], z = b[j];
-                        sx_ += lv.x * z.x - lv.y * z.y;
-                        sy_ += lv.x * z.y + lv.y * z.x;
-                    }
-                    for (int off = 32; off > 0; off >>= 1) {
-                        sx_ += __shfl_xor(sx_, off);
-                        sy_ += __shfl_xor(sy_, off);
-                    }
-                    if (lane == 0) b[i] = mkc<C2>(b[i].x - sx_, b[i].y - sy_);
-                }
-                for (int q = tid; q < kSB * kSB; q += 256) {
-                    const int r = q % kSB, c = q / kSB;
-                    sD[r][c] = (r < in && c < in) ? M[(i0 + r) + (i0 + c) * ld] : mkc<C2>(T(0), T(0));
-                }
-                __syncthreads();
-                if (wave == 0) {
-                    C2 v = (lane < in) ? b[i0 + lane] : mkc<C2>(T(0), T(0));
-                    for (int c = in - 1; c >= 0; --c) {
-                        const C2 zc = mkc<C2>(__shfl(v.x, c), __shfl(v.y, c));
-                        if (lane < c) v = cfms(v, sD[c][lane], zc);
-                    }
-                    if (lane < in) b[i0 + lane] = v;
-                }
-                __syncthreads();
-            }
-            apply_pivots(n, piv, b, false, s_perm, s_inv);
-        }
-    }
-}
-
-template <class T>
-static int lu_solve_part_t(Context* ctx, int64_t n, int64_t batch, const typename Cx<T>::t* LU,
-                           int64_t ld, int64_t stride, const int64_t* ipiv, int trans, int part,
-                           int64_t nrhs, typename Cx<T>::t* B, int64_t ldb, int64_t bstride) {
-    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
-    MH_REQUIRE(batch <= 65535 && nrhs <= 65535, "lu_solve: batch or nrhs too large");
-    MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
-    const double tri = ((part & 1) ? 0.5 : 0.0) + ((part & 2) ? 0.5 : 0.0);
-    ProfScope ps(ctx, sizeof(T) == 8 ? "lu_solve" : "lu_solve_f32", 8.0 * tri * (double)n * n * nrhs * batch,
-                 2.0 * sizeof(T) * tri * (double)n * n * nrhs * batch);
-    hipLaunchKernelGGL(k_lu_solve<T>, dim3((unsigned)nrhs, (unsigned)batch), dim3(256), 0, ctx->stream, n, LU,
-                       ld, stride, ipiv, trans, part, B, ldb, bstride);
-    MH_HIP_CHECK(hipGetLastError());
-    return MH_OK;
-}
-
-int lu_solve_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld,
-                  int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
-                  double2* B, int64_t ldb, int64_t bstride) {
-    return lu_solve_part_t<double>(ctx, n, batch, LU, ld, stride, ipiv, trans, part, nrhs, B, ldb,
-                                   bstride);
-}
-
-int lu_solve_part_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld,
-                      int64_t stride, const int64_t* ipiv, int trans, int part, int64_t nrhs,
-                      float2* B, int64_t ldb, int64_t bstride) {
-    return lu_solve_part_t<float>(ctx, n, batch, LU, ld, stride, ipiv, trans, part, nrhs, B, ldb,
-                                  bstride);
-}
-
-int lu_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
-             const int64_t* ipiv, int trans, int64_t nrhs, double2* B, int64_t ldb,
-             int64_t bstride) {
-    return lu_solve_part(ctx, n, batch, LU, ld, stride, ipiv, trans, 3, nrhs, B, ldb, bstride);
-}
-
 // ---------------------------------------------------------------------------
 // Y = S X (trans = 0) or Y = S^T X (trans = 1) for small nrhs
 // ---------------------------------------------------------------------------

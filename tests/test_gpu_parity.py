@@ -137,11 +137,14 @@ def test_assembly_ragged_sizes_and_matrix_api(mh, oracle, gpu):
         assert relmax(Kt.cpu().numpy(), Ko) < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 64, 100, 257])
-def test_lu_factor_and_solve(mh, gpu, n):
+@pytest.mark.parametrize("n,nrhs", [(1, 2), (5, 2), (31, 2), (32, 1), (33, 2), (64, 2), (100, 2),
+                                     (127, 3), (128, 4), (129, 5), (257, 2), (700, 6)])
+def test_lu_factor_and_solve(mh, gpu, n, nrhs):
+    # slabs of 128 rows and 32-column blocks: sizes around both, and more
+    # right-hand sides than one launch takes (4)
     torch = gpu
     rng = np.random.default_rng(n)
-    batch, nrhs = 3, 2
+    batch = 3
     A = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
     Bm = rng.standard_normal((batch, n, nrhs)) + 1j * rng.standard_normal((batch, n, nrhs))
     dA = torch.from_numpy(np.ascontiguousarray(A.transpose(0, 2, 1))).cuda()  # column-major
