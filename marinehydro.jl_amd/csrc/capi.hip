@@ -443,6 +443,9 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         o_nrm = bump.take(sizeof(double) * (size_t)(ncol_ir * chunk));
     }
     size_t o_cw = 0, o_lam = 0, o_mu = 0, o_tan = 0, o_dk = 0, o_dom = 0, o_part = 0, o_T1 = 0, o_T2 = 0;
+    size_t o_dR = 0, o_Y = 0;
+    const int rank_mask = gf_mask & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG);
+    const int wave_mask = gf_mask & (GF_WU | GF_DELHOMMEAU);
     const int64_t tiles = (n + 63) / 64;
     const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
     if (ga) {
@@ -455,6 +458,10 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         o_part = bump.take(sizeof(double2) * (size_t)(chunk * tiles * tiles * nout));
         o_T1 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
         o_T2 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
+        if (rank_mask) {
+            o_dR = bump.take(sizeof(double) * (size_t)(2 * Pg) * (size_t)n * (size_t)n);
+            o_Y = bump.take(sizeof(double2) * (size_t)(2 * Pg) * (size_t)n * (size_t)(chunk * nr));
+        }
     }
     if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
     char* ws = static_cast<char*>(c->ws);
@@ -587,9 +594,16 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             MH_HIP_CHECK(hipMemcpyAsync(d_dom, hdom.data(), sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
             TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
             int64_t nblk = 0;
-            TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, gf_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
-                                     aDw, sS, sD, wa, wss, part, &nblk));
+            if (wave_mask)
+                TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
+                                         aDw, sS, sD, wa, wss, part, &nblk));
             TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 0));
+            if (rank_mask) {
+                double* dR = reinterpret_cast<double*>(ws + o_dR);
+                double2* Yb = reinterpret_cast<double2*>(ws + o_Y);
+                TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, dR));
+                TRY(launch_rankine_contract(c, n, Pg, nw, ndof, nr, dR, aS, aDw, sS, sD, wa, wss, Yb, T2));
+            }
             TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
             MH_HIP_CHECK(hipMemcpyAsync(hT1.data(), T1, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));

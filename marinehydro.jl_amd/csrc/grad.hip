@@ -10,11 +10,14 @@
 // lambda^i = K^{-T} S^T c^i and
 //   dF^{i,r} = (dc^i) . phi^r + lambda^i . db^r
 //            + sum_pq (c^i_p dS_pq - lambda^i_p dK_pq) sigma^r_q.
-// The influence matrices' derivatives dS, dD, dK are never stored: the pair
-// kernel recomputes every panel pair with Dual<P> arithmetic (the FP64 pair
-// code of mh_device.hpp run with P tangent directions) and contracts them on
-// the fly with the adjoint weights; the O(N) boundary-condition and force
-// terms are handled by a second dual kernel.  The derivative is that of the
+// The derivatives are split by Green-function term.  The Rankine and image
+// parts of dS and dD/dK do not depend on the frequency: one pass over all
+// panel pairs (the FP64 pair code of mh_device.hpp run in Dual<P>) stores
+// them as real n x n matrices per parameter, and two GEMMs with all
+// frequencies' right vectors at once, then dot products with the left
+// vectors, contract them.  The wave-term parts are recomputed per frequency
+// and contracted on the fly with the adjoint weights, never stored.  The
+// O(N) boundary-condition and force terms are handled by a third kernel.  The derivative is that of the
 // code as written (the Wu fits, the branch thresholds), i.e. what automatic
 // differentiation of the reference gives and what central finite
 // differences of the forward path converge to.
@@ -126,12 +129,18 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
         }
         p.area = tangent_of<P>(pr.area, tj, 6);
         p.radius = pr.radius;
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) p.v[v][c] = tangent_of<P>(pr.v[v][c], tj, 7 + 3 * v + c);
         D_ IR, gxiR[3], gxR[3];
-        rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        if (gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p.v[v][c] = tangent_of<P>(pr.v[v][c], tj, 7 + 3 * v + c);
+            rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        } else {
+            IR = dconst<P>(0.0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gxiR[c] = gxR[c] = dconst<P>(0.0);
+        }
         cplxT<D_> Sv, gxi[3], gx[3];
         if (WAVE) {
             wave_term<WAVE>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], p.area, Sv, gxi, gx);
@@ -197,6 +206,161 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
                 __syncthreads();
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// frequency-independent Rankine + image tangents, stored:
+//   dR[(mat * P + m) * n * n + p + q * n], mat 0: dS_pq,m, mat 1: dDK_pq,m
+// (the real parts of S and D/K from the Rankine terms; FORM as above)
+// ---------------------------------------------------------------------------
+template <int FORM, int P>
+__global__ __launch_bounds__(256) void k_rankine_tangents(int64_t n, const double* __restrict__ C,
+                                                          const double* __restrict__ N,
+                                                          const PanelRec* __restrict__ panels,
+                                                          const double* __restrict__ tan, int gf,
+                                                          double* __restrict__ dR) {
+    typedef Dual<P> D_;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double* ti = tan + ic * kTanFields * P;
+    const D_ x0 = tangent_of<P>(C[ic], ti, 0);
+    const D_ x1 = tangent_of<P>(C[ic + n], ti, 1);
+    const D_ x2 = tangent_of<P>(C[ic + 2 * n], ti, 2);
+    D_ ni[3];
+    if (FORM == 2) {
+        ni[0] = tangent_of<P>(N[ic], ti, 3);
+        ni[1] = tangent_of<P>(N[ic + n], ti, 4);
+        ni[2] = tangent_of<P>(N[ic + 2 * n], ti, 5);
+    }
+    const int64_t nn = n * n;
+    const int64_t jb = (int64_t)blockIdx.y * 64;
+    for (int jj = wave; jj < 64; jj += 4) {
+        const int64_t j = jb + jj;
+        if (j >= n) break;
+        const PanelRec& pr = panels[j];
+        const double* tj = tan + j * kTanFields * P;
+        PanelT<D_> p;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p.c[c] = tangent_of<P>(pr.c[c], tj, c);
+            p.n[c] = tangent_of<P>(pr.n[c], tj, 3 + c);
+        }
+        p.area = tangent_of<P>(pr.area, tj, 6);
+        p.radius = pr.radius;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p.v[v][c] = tangent_of<P>(pr.v[v][c], tj, 7 + 3 * v + c);
+        D_ IR, gxiR[3], gxR[3];
+        rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        const D_ kr = (FORM == 1) ? p.n[0] * gxiR[0] + p.n[1] * gxiR[1] + p.n[2] * gxiR[2]
+                                  : ni[0] * gxR[0] + ni[1] * gxR[1] + ni[2] * gxR[2];
+        if (row_ok) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                dR[m * nn + i + j * n] = -kInv4Pi * IR.d[m];
+                dR[(P + m) * nn + i + j * n] = -kInv4Pi * kr.d[m];
+            }
+        }
+    }
+}
+
+// Y[z] (n x ncol complex, column c at Y + z * n * ncol + c * n) =
+//   dR[z] (n x n real) * B_z, z = mat * P + m; B_z column c = (w, r) =
+//   (c / nr, c % nr) at Bmat + w * wstride + r * n (mat 0: B0, mat 1: B1).
+// 64 x 64 tiles, 256 threads, each a 4 x 4 block of complex outputs.
+constexpr int kRcK = 16;
+__global__ __launch_bounds__(256) void k_rc_gemm(int64_t n, int64_t ncol, int64_t nr, int P,
+                                                 const double* __restrict__ dR, const double2* __restrict__ B0,
+                                                 const double2* __restrict__ B1, int64_t wstride,
+                                                 double2* __restrict__ Y) {
+    __shared__ double sA[kRcK][64 + 1];
+    __shared__ double2 sB[kRcK][64 + 1];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int z = blockIdx.z, mat = z / P;
+    const double* A = dR + (int64_t)z * n * n;
+    const double2* B = mat ? B1 : B0;
+    const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
+    for (int64_t k0 = 0; k0 < n; k0 += kRcK) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int rr = tid % 64, kk = tid / 64 + 4 * t;
+            const int64_t row = r0 + rr, k = k0 + kk;
+            sA[kk][rr] = (row < n && k < n) ? A[row + k * n] : 0.0;
+            const int kb = tid % 16, cc = tid / 16 + 16 * t;
+            const int64_t col = c0 + cc, q = k0 + kb;
+            sB[kb][cc] = (col < ncol && q < n) ? B[(col / nr) * wstride + (col % nr) * n + q]
+                                               : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kRcK; ++kk) {
+            double av[4];
+            double2 bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = sA[kk][tx + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][ty + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc[a][b].x += av[a] * bv[b].x;
+                    acc[a][b].y += av[a] * bv[b].y;
+                }
+        }
+        __syncthreads();
+    }
+    double2* Yz = Y + (int64_t)z * n * ncol;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int64_t row = r0 + tx + 16 * a, col = c0 + ty + 16 * b;
+            if (row < n && col < ncol) Yz[col * n + row] = acc[a][b];
+        }
+}
+
+// out[((w * ndof + d) * nr + r) * P + m] +=
+//   sum_p aS^d_w,p Y[m][p, c] - aD^d_w,p Y[P + m][p, c],  c = w * nr + r
+__global__ __launch_bounds__(256) void k_rc_contract(int64_t n, int64_t ndof, int64_t nr, int P, int64_t ncol,
+                                                     const double2* __restrict__ aS,
+                                                     const double2* __restrict__ aD, int64_t wstride_a,
+                                                     const double2* __restrict__ Y, double2* __restrict__ out) {
+    const int64_t o = blockIdx.x;  // ((w * ndof + d) * nr + r) * P + m
+    const int m = (int)(o % P);
+    const int64_t r = (o / P) % nr, d = (o / (P * nr)) % ndof, w = o / (P * nr * ndof);
+    const int64_t c = w * nr + r;
+    const double2* ys = Y + (int64_t)m * n * ncol + c * n;
+    const double2* yd = Y + (int64_t)(P + m) * n * ncol + c * n;
+    const double2* as = aS + w * wstride_a + d * n;
+    const double2* ad = aD + w * wstride_a + d * n;
+    double sx = 0.0, sy = 0.0;
+    for (int64_t p = threadIdx.x; p < n; p += 256) {
+        const double2 a = as[p], y = ys[p], b = ad[p], u = yd[p];
+        sx += a.x * y.x - a.y * y.y - (b.x * u.x - b.y * u.y);
+        sy += a.x * y.y + a.y * y.x - (b.x * u.y + b.y * u.x);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+    }
+    __shared__ double2 red[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = make_double2(sx, sy);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[o].x += red[0].x + red[1].x + red[2].x + red[3].x;
+        out[o].y += red[0].y + red[1].y + red[2].y + red[3].y;
     }
 }
 
@@ -404,6 +568,42 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
 #undef TP_FORM
 #undef TP_NR
 #undef TP
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_rankine_tangents(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, const double* tan, int P,
+                            int gf, int form, double* dR) {
+    const int64_t n = m->nfaces;
+    const int64_t tiles = (n + 63) / 64;
+    dim3 grid((unsigned)tiles, (unsigned)tiles);
+    ProfScope ps(ctx, "rankine_tangents", 0.0, 16.0 * P * (double)n * n);
+#define RT(FM, PP)                                                                                        \
+    hipLaunchKernelGGL((k_rankine_tangents<FM, PP>), grid, dim3(256), 0, ctx->stream, n, m->centers, m->normals, \
+                       panels, tan, gf, dR)
+    if (form == 1) {
+        if (P == 1) RT(1, 1); else if (P == 2) RT(1, 2); else RT(1, 4);
+    } else {
+        if (P == 1) RT(2, 1); else if (P == 2) RT(2, 2); else RT(2, 4);
+    }
+#undef RT
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+int launch_rankine_contract(Context* ctx, int64_t n, int P, int64_t nw, int64_t ndof, int64_t nr, const double* dR,
+                            const double2* aS, const double2* aD, const double2* sS, const double2* sD, int64_t wa,
+                            int64_t ws, double2* Y, double2* out) {
+    const int64_t ncol = nw * nr;
+    if (n == 0 || ncol == 0 || ndof == 0) return MH_OK;
+    {
+        ProfScope ps(ctx, "rc_gemm", 2.0 * 2.0 * 2.0 * P * (double)n * n * ncol, 0.0);
+        dim3 grid((unsigned)((n + 63) / 64), (unsigned)((ncol + 63) / 64), (unsigned)(2 * P));
+        hipLaunchKernelGGL(k_rc_gemm, grid, dim3(256), 0, ctx->stream, n, ncol, nr, P, dR, sS, sD, ws, Y);
+        MH_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rc_contract, dim3((unsigned)(nw * ndof * nr * P)), dim3(256), 0, ctx->stream, n, ndof, nr,
+                       P, ncol, aS, aD, wa, Y, out);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }

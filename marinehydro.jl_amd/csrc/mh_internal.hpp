@@ -163,6 +163,14 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
                          int64_t ndof, int64_t nr, const double2* aS, const double2* aD,
                          const double2* sS, const double2* sD, int64_t wa, int64_t ws, double2* part,
                          int64_t* nblk_out);
+// Frequency-independent Rankine + image tangents dR (2 P real n x n
+// matrices) and their contraction with every frequency's vectors, added to
+// out[w][d][r][m]; Y holds 2 P n x (nw nr) complex.
+int launch_rankine_tangents(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, const double* tan, int P,
+                            int gf, int form, double* dR);
+int launch_rankine_contract(Context* ctx, int64_t n, int P, int64_t nw, int64_t ndof, int64_t nr, const double* dR,
+                            const double2* aS, const double2* aD, const double2* sS, const double2* sD, int64_t wa,
+                            int64_t ws, double2* Y, double2* out);
 int launch_force_weights(Context* ctx, const mh_mesh_t* m, int64_t nw, const double* d_om, double rho,
                          int64_t ndof, const double* dofs, double2* cw, int64_t wstride);
 int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan, int64_t nw,
