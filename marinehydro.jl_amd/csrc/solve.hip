@@ -404,60 +404,68 @@ int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2*
 }
 
 // ---------------------------------------------------------------------------
-// triangular solves with the LU factors, one 256-thread workgroup per
-// (batch item, right-hand side).  The vector stays in global memory (it is
-// small and cache-resident); each 32 x 32 diagonal block is staged in LDS
-// and solved by one wave, then the rest of the vector is updated by the whole
-// workgroup with coalesced column reads.
-//   part: 1 = lower (unit L, with the pivots), 2 = upper (U), 3 = both.
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// Y = S X (trans = 0) or Y = S^T X (trans = 1) for small nrhs
+// Y = S X (trans = 0) or Y = S^T X (trans = 1) for a few right-hand sides,
+// optionally Y = Z - op(S) X.  HBM-bound: S is read once per launch for up to
+// 16 columns of X.
 // ---------------------------------------------------------------------------
 constexpr int kMaxRhs = 16;
 
-__global__ __launch_bounds__(256) void k_zgemv_multi(int64_t n, int64_t nrhs,
-                                                     const double2* __restrict__ S, int64_t ld,
-                                                     int64_t stride, const double2* __restrict__ X,
-                                                     int64_t ldx, int64_t xstride,
-                                                     double2* __restrict__ Y, int64_t ldy,
-                                                     int64_t ystride, const double2* __restrict__ Z) {
+// acc[r] += s x[r] for all NR columns (columns beyond nrhs hold zeros)
+template <int NR>
+__device__ __forceinline__ void zmac(double2 (&acc)[NR], double2 s, const double2* x) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        acc[r].x = fma(s.x, x[r].x, acc[r].x);
+        acc[r].x = fma(-s.y, x[r].y, acc[r].x);
+        acc[r].y = fma(s.x, x[r].y, acc[r].y);
+        acc[r].y = fma(s.y, x[r].x, acc[r].y);
+    }
+}
+
+// one thread per row of S, columns in chunks of 64 staged with X in LDS
+template <int NR>
+__global__ __launch_bounds__(256) void k_zgemv_nn(int64_t n, int64_t nrhs, const double2* __restrict__ S, int64_t ld,
+                                                  int64_t stride, const double2* __restrict__ X, int64_t ldx,
+                                                  int64_t xstride, double2* __restrict__ Y, int64_t ldy,
+                                                  int64_t ystride, const double2* __restrict__ Z) {
     const int64_t item = blockIdx.y;
     const double2* Sm = S + item * stride;
     const double2* Xm = X + item * xstride;
     double2* Ym = Y + item * ystride;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    __shared__ double2 sx[64][kMaxRhs];
-    double2 acc[kMaxRhs];
+    const int nr = (int)nrhs;
+    __shared__ double2 sx[64][NR];
+    double2 acc[NR];
 #pragma unroll
-    for (int r = 0; r < kMaxRhs; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
     for (int64_t j0 = 0; j0 < n; j0 += 64) {
         const int64_t jn = (n - j0) < 64 ? (n - j0) : 64;
         __syncthreads();
-        for (int q = threadIdx.x; q < 64 * kMaxRhs; q += 256) {
+        for (int q = threadIdx.x; q < 64 * NR; q += 256) {
             const int jj = q % 64, r = q / 64;
-            sx[jj][r] = (jj < jn && r < nrhs) ? Xm[(j0 + jj) + r * ldx] : make_double2(0.0, 0.0);
+            sx[jj][r] = (jj < jn && r < nr) ? Xm[(j0 + jj) + r * ldx] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         if (i < n) {
-            for (int jj = 0; jj < jn; ++jj) {
-                const double2 s = Sm[i + (j0 + jj) * ld];
+            const double2* col = Sm + i + j0 * ld;
+            if (jn == 64) {
+                for (int jb = 0; jb < 64; jb += 16) {
+                    double2 sv[16];  // 16 column loads in flight
 #pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r) {
-                    const double2 x = sx[jj][r];
-                    acc[r].x = fma(s.x, x.x, acc[r].x);
-                    acc[r].x = fma(-s.y, x.y, acc[r].x);
-                    acc[r].y = fma(s.x, x.y, acc[r].y);
-                    acc[r].y = fma(s.y, x.x, acc[r].y);
+                    for (int u = 0; u < 16; ++u) sv[u] = col[(jb + u) * ld];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) zmac<NR>(acc, sv[u], sx[jb + u]);
                 }
+            } else {
+                for (int jj = 0; jj < jn; ++jj) zmac<NR>(acc, col[jj * ld], sx[jj]);
             }
         }
     }
     if (i < n) {
         const double2* Zm = Z ? Z + item * ystride : nullptr;
 #pragma unroll
-        for (int r = 0; r < kMaxRhs; ++r)
-            if (r < nrhs) {
+        for (int r = 0; r < NR; ++r)
+            if (r < nr) {
                 if (Zm) {
                     const double2 zv = Zm[i + r * ldy];
                     Ym[i + r * ldy] = make_double2(zv.x - acc[r].x, zv.y - acc[r].y);
@@ -469,13 +477,12 @@ __global__ __launch_bounds__(256) void k_zgemv_multi(int64_t n, int64_t nrhs,
 }
 
 // Y[:, r] = S^T X[:, r]: one wave per output row i (a column of S), lanes
-// along the column, coalesced.
-__global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
-                                                       const double2* __restrict__ S, int64_t ld,
-                                                       int64_t stride, const double2* __restrict__ X,
-                                                       int64_t ldx, int64_t xstride,
-                                                       double2* __restrict__ Y, int64_t ldy,
-                                                       int64_t ystride, const double2* __restrict__ Z) {
+// along the column, coalesced; the column is read once for all r.
+template <int NR>
+__global__ __launch_bounds__(256) void k_zgemv_tn(int64_t n, int64_t nrhs, const double2* __restrict__ S, int64_t ld,
+                                                  int64_t stride, const double2* __restrict__ X, int64_t ldx,
+                                                  int64_t xstride, double2* __restrict__ Y, int64_t ldy,
+                                                  int64_t ystride, const double2* __restrict__ Z) {
     const int64_t item = blockIdx.y;
     const double2* Sm = S + item * stride;
     const double2* Xm = X + item * xstride;
@@ -483,15 +490,35 @@ __global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
+    const int nr = (int)nrhs;
     const double2* col = Sm + i * ld;
-    for (int64_t r = 0; r < nrhs; ++r) {
-        const double2* x = Xm + r * ldx;
-        double sr = 0.0, si = 0.0;
-        for (int64_t j = lane; j < n; j += 64) {
-            const double2 s = col[j], v = x[j];
-            sr += s.x * v.x - s.y * v.y;
-            si += s.x * v.y + s.y * v.x;
+    double2 acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
+    int64_t j0 = 0;
+    for (; j0 + 8 * 64 <= n; j0 += 8 * 64) {  // 8 loads of the column in flight
+        double2 sv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sv[u] = col[j0 + u * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double2 xv[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) xv[r] = (r < nr) ? Xm[j0 + u * 64 + lane + r * ldx] : make_double2(0.0, 0.0);
+            zmac<NR>(acc, sv[u], xv);
         }
+    }
+    for (int64_t j = j0 + lane; j < n; j += 64) {
+        const double2 sj = col[j];
+        double2 xv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) xv[r] = (r < nr) ? Xm[j + r * ldx] : make_double2(0.0, 0.0);
+        zmac<NR>(acc, sj, xv);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (r >= nr) continue;
+        double sr = acc[r].x, si = acc[r].y;
         for (int off = 32; off > 0; off >>= 1) {
             sr += __shfl_xor(sr, off);
             si += __shfl_xor(si, off);
@@ -507,15 +534,40 @@ __global__ __launch_bounds__(256) void k_zgemv_t_multi(int64_t n, int64_t nrhs,
     }
 }
 
+#define MH_ZGEMV_DISPATCH(KERNEL, GRID)                                                                  \
+    do {                                                                                                 \
+        if (nr <= 1)                                                                                     \
+            hipLaunchKernelGGL(KERNEL<1>, GRID, dim3(256), 0, ctx->stream, n, nr, S, ld, stride, Xc, ldx, \
+                               xstride, Yc, ldy, ystride, Zc);                                           \
+        else if (nr <= 2)                                                                                \
+            hipLaunchKernelGGL(KERNEL<2>, GRID, dim3(256), 0, ctx->stream, n, nr, S, ld, stride, Xc, ldx, \
+                               xstride, Yc, ldy, ystride, Zc);                                           \
+        else if (nr <= 4)                                                                                \
+            hipLaunchKernelGGL(KERNEL<4>, GRID, dim3(256), 0, ctx->stream, n, nr, S, ld, stride, Xc, ldx, \
+                               xstride, Yc, ldy, ystride, Zc);                                           \
+        else if (nr <= 8)                                                                                \
+            hipLaunchKernelGGL(KERNEL<8>, GRID, dim3(256), 0, ctx->stream, n, nr, S, ld, stride, Xc, ldx, \
+                               xstride, Yc, ldy, ystride, Zc);                                           \
+        else                                                                                             \
+            hipLaunchKernelGGL(KERNEL<16>, GRID, dim3(256), 0, ctx->stream, n, nr, S, ld, stride, Xc,     \
+                               ldx, xstride, Yc, ldy, ystride, Zc);                                      \
+    } while (0)
+
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
                      double2* Y, int64_t ldy, int64_t ystride, const double2* Z) {
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
-    MH_REQUIRE(nrhs <= kMaxRhs, "matvec: at most 16 right-hand sides per call");
-    ProfScope ps(ctx, "zgemv", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * batch);
-    hipLaunchKernelGGL(k_zgemv_multi, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256),
-                       0, ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride, Z);
-    MH_HIP_CHECK(hipGetLastError());
+    MH_REQUIRE(batch <= 65535, "matvec: batch too large");
+    ProfScope ps(ctx, "zgemv", 8.0 * (double)n * n * nrhs * batch,
+                 16.0 * (double)n * n * batch * (double)((nrhs + kMaxRhs - 1) / kMaxRhs));
+    for (int64_t r0 = 0; r0 < nrhs; r0 += kMaxRhs) {
+        const int64_t nr = (nrhs - r0) < kMaxRhs ? (nrhs - r0) : kMaxRhs;
+        const double2* Xc = X + r0 * ldx;
+        double2* Yc = Y + r0 * ldy;
+        const double2* Zc = Z ? Z + r0 * ldy : nullptr;
+        MH_ZGEMV_DISPATCH(k_zgemv_nn, dim3((unsigned)((n + 255) / 256), (unsigned)batch));
+        MH_HIP_CHECK(hipGetLastError());
+    }
     return MH_OK;
 }
 
@@ -523,12 +575,20 @@ int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
                      double2* Y, int64_t ldy, int64_t ystride, const double2* Z) {
     if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
-    ProfScope ps(ctx, "zgemv_t", 8.0 * (double)n * n * nrhs * batch, 16.0 * (double)n * n * nrhs * batch);
-    hipLaunchKernelGGL(k_zgemv_t_multi, dim3((unsigned)((n + 3) / 4), (unsigned)batch), dim3(256), 0,
-                       ctx->stream, n, nrhs, S, ld, stride, X, ldx, xstride, Y, ldy, ystride, Z);
-    MH_HIP_CHECK(hipGetLastError());
+    MH_REQUIRE(batch <= 65535, "matvec: batch too large");
+    ProfScope ps(ctx, "zgemv_t", 8.0 * (double)n * n * nrhs * batch,
+                 16.0 * (double)n * n * batch * (double)((nrhs + kMaxRhs - 1) / kMaxRhs));
+    for (int64_t r0 = 0; r0 < nrhs; r0 += kMaxRhs) {
+        const int64_t nr = (nrhs - r0) < kMaxRhs ? (nrhs - r0) : kMaxRhs;
+        const double2* Xc = X + r0 * ldx;
+        double2* Yc = Y + r0 * ldy;
+        const double2* Zc = Z ? Z + r0 * ldy : nullptr;
+        MH_ZGEMV_DISPATCH(k_zgemv_tn, dim3((unsigned)((n + 3) / 4), (unsigned)batch));
+        MH_HIP_CHECK(hipGetLastError());
+    }
     return MH_OK;
 }
+#undef MH_ZGEMV_DISPATCH
 
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 copies, updates and residual norms for iterative
