@@ -290,6 +290,103 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
         }
 }
 
+// Large trailing updates: 128 x 128 tile per 512-thread workgroup (8 waves,
+// each 64 x 32 = 4 x 2 MFMA blocks), the next depth chunk loaded into
+// registers while the current one is multiplied out of LDS.  Twice the
+// arithmetic intensity of the 64 x 64 kernel per byte staged.
+constexpr int kBT = 128;
+template <class T>
+__global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
+                                                     int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
+                                                     int64_t k_lo, int kw) {
+    typedef typename Cx<T>::t C2;
+    typedef typename Cx<T>::acc acc_t;
+    C2* M = A + (int64_t)blockIdx.z * stride;
+    const int64_t tr = r_lo + (int64_t)blockIdx.x * kBT;
+    const int64_t tc = c_lo + (int64_t)blockIdx.y * kBT;
+    __shared__ C2 sA[kKC][kBT];      // sA[k][row]
+    __shared__ C2 sB[kKC][kBT + 1];  // sB[k][col], padded against store conflicts
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
+    acc_t accr[4][2], acci[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
+            acci[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
+        }
+    C2 pa[4], pb[4];
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = tid + 512 * t;
+            {
+                const int r = q & (kBT - 1), k = q >> 7;
+                const int64_t gr = tr + r;
+                pa[t] = (gr < r_hi && kc + k < kw) ? M[gr + (k_lo + kc + k) * ld] : mkc<C2>(T(0), T(0));
+            }
+            {
+                const int k = q & (kKC - 1), c = q >> 4;
+                const int64_t gc = tc + c;
+                pb[t] = (gc < c_hi && kc + k < kw) ? M[(k_lo + kc + k) + gc * ld] : mkc<C2>(T(0), T(0));
+            }
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = tid + 512 * t;
+            sA[q >> 7][q & (kBT - 1)] = pa[t];
+            sB[q & (kKC - 1)][q >> 4] = pb[t];
+        }
+    };
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int kc = 0; kc < kw; kc += kKC) {
+        const bool more = kc + kKC < kw;
+        if (more) gload(kc + kKC);
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            C2 av[4], bv[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = sA[kk + (l >> 4)][64 * wr + 16 * a + (l & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = sB[kk + (l >> 4)][32 * wc + 16 * b + (l & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
+                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
+                }
+        }
+        __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t gc = tc + 32 * wc + 16 * b + (l & 15);
+            if (gc >= c_hi) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 64 * wr + 16 * a + Cx<T>::row(r, l);
+                if (gr >= r_hi) continue;
+                C2* o = M + gr + gc * ld;
+                const C2 v = *o;
+                *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+            }
+        }
+}
+
 // ---------------------------------------------------------------------------
 // host-side recursion
 // ---------------------------------------------------------------------------
@@ -320,8 +417,13 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
     ProfScope ps(P.ctx, sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32", 8.0 * (double)mr * (double)mc * kw * P.batch,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
-    hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
-                       dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
+    if (mr >= kBT && mc >= kBT)
+        hipLaunchKernelGGL(k_lu_gemm_big<T>,
+                           dim3((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch),
+                           dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
+    else
+        hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
+                           dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
     return MH_OK;
 }
 
