@@ -119,9 +119,16 @@ int mh_context_destroy(mh_context_t ctx) {
     (void)hipStreamSynchronize(c->stream);
     if (c->ws) (void)hipFree(c->ws);
     if (c->d_flag) (void)hipFree(c->d_flag);
-    if (c->tri_sync) (void)hipFree(c->tri_sync);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    for (auto& t : c->tri) {
+        if (t.sync) (void)hipFree(t.sync);
+        if (t.ws) (void)hipFree(t.ws);
+    }
     if (c->tri_err) (void)hipFree(c->tri_err);
-    if (c->tri_ws) (void)hipFree(c->tri_ws);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
     delete c;
     return MH_OK;
 }
@@ -358,34 +365,128 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
     return check_info(c, reinterpret_cast<int*>(ws + o_info), batch);
 }
 
-// Solves op(A) X = Bm (op = A or A^T) for `batch` systems with the FP32 LU
-// factors of A and iterative refinement against the FP64 A: X <- X + LU32^{-1}
-// (Bm - op(A) X), until every column's relative residual max|r| / max|b| is
-// below 1e-12 (at most 12 sweeps).  Bm, X, R share one layout (ldb, bstride);
-// R32 is n x nrhs per item.
-static int ir_solve(Context* c, int64_t n, int64_t batch, int trans, const double2* A64, int64_t ld,
-                    int64_t stride, const float2* LU32, int64_t stride32, const int* perm, const float2* inv32,
-                    const double2* Bm, int64_t nrhs, int64_t ldb, int64_t bstride, double2* X,
-                    double2* R, float2* R32, double* d_nrm) {
-    const int64_t s32 = n * nrhs;
-    MH_HIP_CHECK(hipMemsetAsync(X, 0, sizeof(double2) * (size_t)(bstride * batch), c->stream));
-    MH_HIP_CHECK(hipMemcpyAsync(R, Bm, sizeof(double2) * (size_t)(bstride * batch), hipMemcpyDeviceToDevice,
-                                c->stream));
-    std::vector<double> h(batch * nrhs);
-    for (int it = 0; it < 12; ++it) {
-        TRY(convert_to_f32(c, n, nrhs, batch, R, ldb, bstride, R32, n, s32));
-        TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, trans, 3, nrhs, R32, n, s32));
-        TRY(add_from_f32(c, n, nrhs, batch, R32, n, s32, X, ldb, bstride));
-        if (trans)
-            TRY(zgemm_tn_batched(c, n, nrhs, batch, A64, ld, stride, X, ldb, bstride, R, ldb, bstride, Bm));
-        else
-            TRY(zgemm_nn_batched(c, n, nrhs, batch, A64, ld, stride, X, ldb, bstride, R, ldb, bstride, Bm));
-        TRY(rel_residual(c, n, nrhs, batch, R, Bm, ldb, bstride, d_nrm));
-        MH_HIP_CHECK(hipMemcpyAsync(h.data(), d_nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+// ---- second stream: independent work (forward and adjoint solves) runs
+// on ctx->aux, ordered against ctx->stream by events --------------------------
+static int ensure_aux(Context* c) {
+    if (c->aux) return MH_OK;
+    MH_HIP_CHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return MH_OK;
+}
+// aux waits for everything issued on stream so far
+static int fork_aux(Context* c) {
+    TRY(ensure_aux(c));
+    MH_HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+    MH_HIP_CHECK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    return MH_OK;
+}
+// stream waits for everything issued on aux so far
+static int join_aux(Context* c) {
+    MH_HIP_CHECK(hipEventRecord(c->ev_join, c->aux));
+    MH_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return MH_OK;
+}
+// issue launches on aux (triangular-solve state slot 1) within a scope
+struct OnAux {
+    Context* c;
+    hipStream_t saved;
+    int saved_slot;
+    OnAux(Context* ctx, bool on) : c(ctx), saved(ctx->stream), saved_slot(ctx->tri_slot) {
+        if (on) {
+            c->stream = c->aux;
+            c->tri_slot = 1;
+        }
+    }
+    ~OnAux() {
+        c->stream = saved;
+        c->tri_slot = saved_slot;
+    }
+};
+static int ensure_pinned(Context* c, size_t count) {
+    if (count <= c->h_pin_count) return MH_OK;
+    if (c->h_pin) {
         MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-        double worst = 0.0;
-        for (double v : h) worst = v > worst ? v : worst;
-        if (worst < 1e-12) return MH_OK;
+        if (c->aux) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
+        MH_HIP_CHECK(hipHostFree(c->h_pin));
+        c->h_pin = nullptr;
+        c->h_pin_count = 0;
+    }
+    MH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_pin), sizeof(double) * count));
+    c->h_pin_count = count;
+    return MH_OK;
+}
+
+// One system op(A) X = Bm (op = A or A^T) for `batch` items, refined with the
+// FP32 LU factors of A against the FP64 A.  Bm, X, R share one layout (ldb,
+// bstride); R32 is n x nrhs per item; nrm holds nrhs x batch residual norms.
+struct IrSystem {
+    int trans;
+    const double2* Bm;
+    int64_t nrhs, ldb, bstride;
+    double2* X;
+    double2* R;
+    float2* R32;
+    double* nrm;
+};
+
+// Iterative refinement of up to two systems with the same FP32 factors, the
+// second on the aux stream so both proceed together: X <- X + LU32^{-1}
+// (Bm - op(A) X) until every column's relative residual max|r| / max|b| is
+// below 1e-12 (at most 12 sweeps).
+static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
+                    const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
+                    int nsys) {
+    size_t hcount = 0;
+    for (int k = 0; k < nsys; ++k) hcount += (size_t)(batch * sys[k].nrhs);
+    TRY(ensure_pinned(c, hcount));
+    if (nsys > 1) TRY(fork_aux(c));
+    bool done[2] = {false, false};
+    for (int k = 0; k < nsys; ++k) {
+        OnAux on(c, k == 1);
+        const IrSystem& q = sys[k];
+        MH_HIP_CHECK(hipMemsetAsync(q.X, 0, sizeof(double2) * (size_t)(q.bstride * batch), c->stream));
+        MH_HIP_CHECK(hipMemcpyAsync(q.R, q.Bm, sizeof(double2) * (size_t)(q.bstride * batch),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int it = 0; it < 12; ++it) {
+        size_t off = 0;
+        for (int k = 0; k < nsys; ++k) {
+            const IrSystem& q = sys[k];
+            const size_t cnt = (size_t)(batch * q.nrhs);
+            if (!done[k]) {
+                OnAux on(c, k == 1);
+                const int64_t s32 = n * q.nrhs;
+                TRY(convert_to_f32(c, n, q.nrhs, batch, q.R, q.ldb, q.bstride, q.R32, n, s32));
+                TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, q.trans, 3, q.nrhs, q.R32, n, s32));
+                TRY(add_from_f32(c, n, q.nrhs, batch, q.R32, n, s32, q.X, q.ldb, q.bstride));
+                if (q.trans)
+                    TRY(zgemm_tn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
+                                         q.bstride, q.Bm));
+                else
+                    TRY(zgemm_nn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
+                                         q.bstride, q.Bm));
+                TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
+                MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + off, q.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                            c->stream));
+            }
+            off += cnt;
+        }
+        MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (nsys > 1) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
+        off = 0;
+        bool all = true;
+        for (int k = 0; k < nsys; ++k) {
+            const size_t cnt = (size_t)(batch * sys[k].nrhs);
+            if (!done[k]) {
+                double worst = 0.0;
+                for (size_t e = 0; e < cnt; ++e) worst = c->h_pin[off + e] > worst ? c->h_pin[off + e] : worst;
+                done[k] = worst < 1e-12;
+            }
+            all = all && done[k];
+            off += cnt;
+        }
+        if (all) return MH_OK;
     }
     set_error("mixed precision: iterative refinement did not reach a 1e-12 relative residual");
     return MH_ERR_NOT_CONVERGED;
@@ -434,13 +535,16 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     const size_t o_P = bump.take(sizeof(PanelRec) * (size_t)n);
     const bool mixed = precision == MH_MIXED;
     const int64_t ncol_ir = nr > ndof ? nr : ndof;  // columns of the refinement buffers
-    size_t o_D32 = 0, o_y = 0, o_R = 0, o_R32 = 0, o_nrm = 0;
+    size_t o_D32 = 0, o_y = 0, o_R = 0, o_R32 = 0, o_nrm = 0, o_R2 = 0, o_R32b = 0, o_nrm2 = 0;
     if (mixed) {
         o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * chunk);
         o_y = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
         o_R = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
         o_R32 = bump.take(sizeof(float2) * (size_t)(n * ncol_ir * chunk));
         o_nrm = bump.take(sizeof(double) * (size_t)(ncol_ir * chunk));
+        o_R2 = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
+        o_R32b = bump.take(sizeof(float2) * (size_t)(n * ncol_ir * chunk));
+        o_nrm2 = bump.take(sizeof(double) * (size_t)(ncol_ir * chunk));
     }
     size_t o_cw = 0, o_lam = 0, o_mu = 0, o_tan = 0, o_dk = 0, o_dom = 0, o_part = 0, o_T1 = 0, o_T2 = 0;
     size_t o_dR = 0, o_Y = 0;
@@ -503,15 +607,34 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
             TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
-            if (direct) {  // phi = D \ (S bc)
+            // forward system: phi = D \ (S bc), or sigma = K \ bc (kept in ybuf)
+            IrSystem sys[2];
+            if (direct) {
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
-                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, perm, inv32, ybuf, nr, ldb, bstride, phi, Rb,
-                             R32, d_nrm));
-            } else {  // sigma = K \ bc (kept in ybuf); phi = S sigma
-                TRY(ir_solve(c, n, nw, 0, dD, ld, stride, D32, n * n, perm, inv32, rhs, nr, ldb, bstride, ybuf, Rb,
-                             R32, d_nrm));
-                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
+                sys[0] = {0, ybuf, nr, ldb, bstride, phi, Rb, R32, d_nrm};
+            } else {
+                sys[0] = {0, rhs, nr, ldb, bstride, ybuf, Rb, R32, d_nrm};
             }
+            int nsys = 1;
+            if (ga && ndof) {  // adjoint system, refined alongside: lambda = D^-T c or K^-T S^T c
+                double2* cw = reinterpret_cast<double2*>(ws + o_cw);
+                double2* lam = reinterpret_cast<double2*>(ws + o_lam);
+                double2* mu = reinterpret_cast<double2*>(ws + o_mu);
+                double2* R2 = reinterpret_cast<double2*>(ws + o_R2);
+                float2* R32b = reinterpret_cast<float2*>(ws + o_R32b);
+                double* d_nrm2 = reinterpret_cast<double*>(ws + o_nrm2);
+                TRY(launch_force_weights(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, cw, wa));
+                if (direct) {
+                    sys[1] = {1, cw, ndof, n, wa, lam, R2, R32b, d_nrm2};
+                } else {
+                    TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, mu, n, wa));
+                    sys[1] = {1, mu, ndof, n, wa, lam, R2, R32b, d_nrm2};
+                }
+                nsys = 2;
+            }
+            TRY(ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys));
+            if (nsys > 1) TRY(join_aux(c));
+            if (!direct) TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
         } else if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
             TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
         } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
@@ -523,7 +646,20 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
             TRY(tri_prepare(c, n, nw, dD, ld, stride, piv, (ga && ndof) ? perm : nullptr, inv));
+            if (ga && ndof) {  // adjoint solve on aux, alongside the back substitution
+                double2* cw = reinterpret_cast<double2*>(ws + o_cw);
+                double2* lam = reinterpret_cast<double2*>(ws + o_lam);
+                TRY(launch_force_weights(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, cw, wa));
+                TRY(fork_aux(c));
+                OnAux on(c, true);
+                if (direct)  // lambda = D^-T c
+                    MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
+                else  // lambda = K^-T S^T c
+                    TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
+                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
+            }
             TRY(tri_solve(c, n, nw, dD, ld, stride, nullptr, inv, 0, 2, nr, aug, ld, stride));
+            if (ga && ndof) TRY(join_aux(c));
             if (direct) {
                 MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
                                               sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
@@ -550,28 +686,16 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         double2* part = reinterpret_cast<double2*>(ws + o_part);
         double2* T1 = reinterpret_cast<double2*>(ws + o_T1);
         double2* T2 = reinterpret_cast<double2*>(ws + o_T2);
-        TRY(launch_force_weights(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, cw, wa));
+        // lambda (and, indirect mixed, mu = S^T c) were computed with the solves
         const double2 *aS, *aDw, *sS, *sD, *rhoW;
         int64_t wss;
         if (direct) {  // lambda = D^-T c; mu = S^T lambda
-            if (mixed) {
-                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, perm, inv32, cw, ndof, n, wa, lam, Rb, R32, d_nrm));
-            } else {
-                MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
-                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
-            }
             TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
             aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
-        } else {       // lambda = K^-T S^T c
-            if (mixed) {
-                TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, mu, n, wa));
-                TRY(ir_solve(c, n, nw, 1, dD, ld, stride, D32, n * n, perm, inv32, mu, ndof, n, wa, lam, Rb, R32, d_nrm));
-                aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
-            } else {
-                TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
-                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
-                aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
-            }
+        } else if (mixed) {  // lambda = K^-T S^T c
+            aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
+        } else {
+            aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
         }
         // outputs of this chunk, host side: F of every (dof, rhs) pair
         std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof);

@@ -210,6 +210,202 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
 }
 
 // ---------------------------------------------------------------------------
+// wave-term-only tangent contraction, evaluated once per unordered pair.
+// The wave term depends on a pair only through r = |x_p - x_q|_h and
+// z = z_p + z_q (collocation points are the panel centres), both symmetric,
+// so the expensive part -- G, dG/dr, dG/dz with their tangents -- is shared by
+// (p, q) and (q, p).  With A = (d1 dG/dr, d2 dG/dr, k dG/dz) for the pair
+// (p, q) (d1, d2 = dr/dxi), the source gradient of (q, p) is (-A1, -A2, A3),
+// and with Eq = n_q . A, Ep = (-n_p1, -n_p2, n_p3) . A:
+//   S_pq = k a_q G, S_qp = k a_p G
+//   FORM 1: D_pq = k a_q Eq, D_qp = k a_p Ep;  FORM 2: K_pq = k a_q Ep, K_qp = k a_p Eq
+// (before the -1/(4 pi) factor).  Tile pairs P < Q evaluate both orders,
+// diagonal tiles only (p, q), tiles P > Q are covered by their partner.
+// Contraction: (p, q) terms accumulate dS_pq sS_q per row p (weighted by aS_p
+// at the end, as in k_tangent_pairs); (q, p) terms accumulate aS_q dS_qp per
+// column p (weighted by sS_p at the end).
+// ---------------------------------------------------------------------------
+template <int WAVE, int FORM, int P, int NR>
+__global__ __launch_bounds__(256) void k_tangent_sym(
+    int64_t n, const double* __restrict__ C, const PanelRec* __restrict__ panels, const double* __restrict__ tan,
+    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr,
+    const double2* __restrict__ aS, const double2* __restrict__ aD, const double2* __restrict__ sS,
+    const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s, double2* __restrict__ part) {
+    typedef Dual<P> D_;
+    typedef cplxT<D_> CD;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t w = blockIdx.z;
+    const int64_t nblk = (int64_t)gridDim.x * gridDim.y;
+    const int64_t blk = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const int64_t nout = ndof * nr * P;
+    double2* pw = part + (w * nblk + blk) * nout;
+    if (blockIdx.x > blockIdx.y) {  // covered by the tile pair (y, x)
+        for (int64_t o = threadIdx.x; o < nout; o += 256) pw[o] = make_double2(0.0, 0.0);
+        return;
+    }
+    const bool sym = blockIdx.x < blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double* ti = tan + ic * kTanFields * P;
+    const D_ x0 = tangent_of<P>(C[ic], ti, 0);
+    const D_ x1 = tangent_of<P>(C[ic + n], ti, 1);
+    const D_ x2 = tangent_of<P>(C[ic + 2 * n], ti, 2);
+    const PanelRec& pi_ = panels[ic];
+    const D_ np0 = tangent_of<P>(pi_.n[0], ti, 3), np1 = tangent_of<P>(pi_.n[1], ti, 4),
+             np2 = tangent_of<P>(pi_.n[2], ti, 5);
+    const D_ ap = tangent_of<P>(pi_.area, ti, 6);
+    D_ k;
+    k.v = ks[w];
+#pragma unroll
+    for (int m = 0; m < P; ++m) k.d[m] = dks[w * P + m];
+    const double2* sSw = sS + w * wstride_s;
+    const double2* sDw = sD + w * wstride_s;
+    const double2* aSw = aS + w * wstride_a;
+    const double2* aDw = aD + w * wstride_a;
+    double2 RS[NR][P], RD[NR][P], US[NR][P], UD[NR][P];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int m = 0; m < P; ++m) RS[r][m] = RD[r][m] = US[r][m] = UD[r][m] = make_double2(0.0, 0.0);
+    auto cmul = [](const CD& a, const CD& b) {
+        CD o;
+        o.re = a.re * b.re - a.im * b.im;
+        o.im = a.re * b.im + a.im * b.re;
+        return o;
+    };
+    auto scl = [](const D_& a, const CD& b) {
+        CD o;
+        o.re = a * b.re;
+        o.im = a * b.im;
+        return o;
+    };
+    const int64_t jb = (int64_t)blockIdx.y * 64;
+    for (int jj = wave; jj < 64; jj += 4) {
+        const int64_t j = jb + jj;
+        if (j >= n) break;
+        const PanelRec& pr = panels[j];
+        const double* tj = tan + j * kTanFields * P;
+        const D_ cq0 = tangent_of<P>(pr.c[0], tj, 0), cq1 = tangent_of<P>(pr.c[1], tj, 1),
+                 cq2 = tangent_of<P>(pr.c[2], tj, 2);
+        const D_ nq0 = tangent_of<P>(pr.n[0], tj, 3), nq1 = tangent_of<P>(pr.n[1], tj, 4),
+                 nq2 = tangent_of<P>(pr.n[2], tj, 5);
+        const D_ aq = tangent_of<P>(pr.area, tj, 6);
+        const D_ dx = x0 - cq0, dy = x1 - cq1, zs = x2 + cq2;
+        // G, dG/dr, dG/dz with local tangents (d/dr, d/dz), chained to P
+        const D_ rr = k * horiz_dist(dx, dy);
+        const D_ zz = k * zs;
+        Dual<2> rl, zl;
+        rl.v = rr.v;
+        rl.d[0] = 1.0;
+        rl.d[1] = 0.0;
+        zl.v = zz.v;
+        zl.d[0] = 0.0;
+        zl.d[1] = 1.0;
+        cplxT<Dual<2>> Gl, Grl, Gzl;
+        if (WAVE == 1) wu_dimless<true>(rl, zl, Gl, Grl, Gzl);
+        if (WAVE == 2) delhommeau_dimless(rl, zl, Gl, Grl, Gzl);
+        CD G, Gr, Gz;
+        G.re = lift_rz(Gl.re, rr, zz);
+        G.im = lift_rz(Gl.im, rr, zz);
+        Gr.re = lift_rz(Grl.re, rr, zz);
+        Gr.im = lift_rz(Grl.im, rr, zz);
+        Gz.re = lift_rz(Gzl.re, rr, zz);
+        Gz.im = lift_rz(Gzl.im, rr, zz);
+        D_ d1 = dconst<P>(0.0), d2 = dconst<P>(0.0);
+        if (::fabs(val(rr)) > 1e-6) {
+            const D_ kk = k * k / rr;
+            d1 = kk * (-dx);
+            d2 = kk * (-dy);
+        }
+        const CD A1 = scl(d1, Gr), A2 = scl(d2, Gr), A3 = scl(k, Gz);
+        CD Eq, Ep;
+        Eq.re = nq0 * A1.re + nq1 * A2.re + nq2 * A3.re;
+        Eq.im = nq0 * A1.im + nq1 * A2.im + nq2 * A3.im;
+        Ep.re = np2 * A3.re - np0 * A1.re - np1 * A2.re;
+        Ep.im = np2 * A3.im - np0 * A1.im - np1 * A2.im;
+        const D_ kaq = k * aq;
+        const CD Spq = scl(kaq, G);
+        const CD Dpq = scl(kaq, FORM == 1 ? Eq : Ep);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r < nr) {
+                const double2 ss = sSw[j + r * n];
+                const double2 sd = sDw[j + r * n];
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    RS[r][m] = cmac(RS[r][m], -kInv4Pi * Spq.re.d[m], -kInv4Pi * Spq.im.d[m], ss);
+                    RD[r][m] = cmac(RD[r][m], -kInv4Pi * Dpq.re.d[m], -kInv4Pi * Dpq.im.d[m], sd);
+                }
+            }
+        }
+        if (sym) {
+            const D_ kap = k * ap;
+            const CD Sqp = scl(kap, G);
+            const CD Dqp = scl(kap, FORM == 1 ? Ep : Eq);
+#pragma unroll
+            for (int d = 0; d < NR; ++d) {
+                if (d < ndof) {
+                    const double2 as = aSw[j + d * n];
+                    const double2 ad = aDw[j + d * n];
+#pragma unroll
+                    for (int m = 0; m < P; ++m) {
+                        US[d][m] = cmac(US[d][m], -kInv4Pi * Sqp.re.d[m], -kInv4Pi * Sqp.im.d[m], as);
+                        UD[d][m] = cmac(UD[d][m], -kInv4Pi * Dqp.re.d[m], -kInv4Pi * Dqp.im.d[m], ad);
+                    }
+                }
+            }
+        }
+    }
+    (void)cmul;
+    // contract and reduce over the workgroup
+    __shared__ double2 red[4][NR * P];
+    for (int64_t d = 0; d < ndof; ++d) {
+        const double2 as = row_ok ? aSw[i + d * n] : make_double2(0.0, 0.0);
+        const double2 ad = row_ok ? aDw[i + d * n] : make_double2(0.0, 0.0);
+        double2 usd[P], udd[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            usd[m] = make_double2(0.0, 0.0);
+            udd[m] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int dd = 0; dd < NR; ++dd)
+                if (dd == d) {
+                    usd[m] = US[dd][m];
+                    udd[m] = UD[dd][m];
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (r >= nr) continue;
+            const double2 ssp = row_ok ? sSw[i + r * n] : make_double2(0.0, 0.0);
+            const double2 sdp = row_ok ? sDw[i + r * n] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                double vx = as.x * RS[r][m].x - as.y * RS[r][m].y - (ad.x * RD[r][m].x - ad.y * RD[r][m].y);
+                double vy = as.x * RS[r][m].y + as.y * RS[r][m].x - (ad.x * RD[r][m].y + ad.y * RD[r][m].x);
+                vx += usd[m].x * ssp.x - usd[m].y * ssp.y - (udd[m].x * sdp.x - udd[m].y * sdp.y);
+                vy += usd[m].x * ssp.y + usd[m].y * ssp.x - (udd[m].x * sdp.y + udd[m].y * sdp.x);
+                for (int off = 32; off > 0; off >>= 1) {
+                    vx += __shfl_xor(vx, off);
+                    vy += __shfl_xor(vy, off);
+                }
+                if (lane == 0) red[wave][r * P + m] = make_double2(vx, vy);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < nr * P) {
+            const int r = threadIdx.x / P, m = threadIdx.x % P;
+            const double2 a = red[0][r * P + m], b = red[1][r * P + m], c2 = red[2][r * P + m],
+                          e = red[3][r * P + m];
+            pw[(d * nr + r) * P + m] = make_double2(a.x + b.x + c2.x + e.x, a.y + b.y + c2.y + e.y);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // frequency-independent Rankine + image tangents, stored:
 //   dR[(mat * P + m) * n * n + p + q * n], mat 0: dS_pq,m, mat 1: dDK_pq,m
 // (the real parts of S and D/K from the Rankine terms; FORM as above)
@@ -532,6 +728,38 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
     ProfScope ps(ctx, "tangent_pairs", 0.0, 0.0);
     const int wave = (gf & GF_WU) ? 1 : ((gf & GF_DELHOMMEAU) ? 2 : 0);
     const bool nr2 = nr <= 2;
+    if (wave && !(gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG))) {
+        // wave term only: one evaluation per unordered pair
+#define TS(WV, FM, PP, NRR)                                                                                 \
+    hipLaunchKernelGGL((k_tangent_sym<WV, FM, PP, NRR>), grid, dim3(256), 0, ctx->stream, n, m->centers, panels, \
+                       tan, d_k, d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)
+#define TS_NR(WV, FM, PP)           \
+    do {                            \
+        if (nr2)                    \
+            TS(WV, FM, PP, 2);      \
+        else                        \
+            TS(WV, FM, PP, 4);      \
+    } while (0)
+#define TS_P(WV, FM)                \
+    do {                            \
+        if (P == 1)                 \
+            TS_NR(WV, FM, 1);       \
+        else if (P == 2)            \
+            TS_NR(WV, FM, 2);       \
+        else                        \
+            TS_NR(WV, FM, 4);       \
+    } while (0)
+        if (wave == 1) {
+            if (form == 1) TS_P(1, 1); else TS_P(1, 2);
+        } else {
+            if (form == 1) TS_P(2, 1); else TS_P(2, 2);
+        }
+#undef TS_P
+#undef TS_NR
+#undef TS
+        MH_HIP_CHECK(hipGetLastError());
+        return MH_OK;
+    }
 #define TP(WV, FM, PP, NRR)                                                                        \
     launch_tp<WV, FM, PP, NRR>(grid, ctx->stream, n, m->centers, m->normals, panels, tan, gf, d_k, \
                                d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)

@@ -30,14 +30,25 @@ struct Context {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     int* d_flag = nullptr;  // device error / info flag (one int)
-    // triangular solves: [ticket u64 | pad | per-block flags u32], epoch of
-    // the last launch, error word, out-of-place solution workspace
-    void* tri_sync = nullptr;
-    size_t tri_sync_bytes = 0;
-    unsigned tri_epoch = 0;
+    // triangular solves, one state per stream that issues them (0: stream,
+    // 1: aux): [ticket u64 | pad | per-block flags u32], epoch of the last
+    // launch, out-of-place solution workspace; one shared error word
+    struct TriSlot {
+        void* sync = nullptr;
+        size_t sync_bytes = 0;
+        unsigned epoch = 0;
+        void* ws = nullptr;
+        size_t ws_bytes = 0;
+    } tri[2];
+    int tri_slot = 0;
     int* tri_err = nullptr;
-    void* tri_ws = nullptr;
-    size_t tri_ws_bytes = 0;
+    // second stream for independent work (forward and adjoint solves),
+    // default priority, ordered against `stream` by events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // pinned host scratch for small device-to-host reads
+    double* h_pin = nullptr;
+    size_t h_pin_count = 0;
     bool profiling = false;
     std::vector<ProfRec> prof;
 };

@@ -4,11 +4,12 @@
 // A solve with one n x n triangle and a few right-hand sides reads the whole
 // triangle once; one workgroup per matrix would leave most of the chip idle
 // and be bound by one CU's bandwidth.  Here every matrix is cut into slabs of
-// 128 rows, one workgroup per slab (one row per thread).  The unknowns are
+// 256 rows, one workgroup per slab (one row per thread).  The unknowns are
 // produced in 32-row blocks in order.  A workgroup keeps its rows' partial
-// sums in registers; for each earlier block it loads its rows' slice of the
-// triangle ahead of time, waits for the block's solution (a per-block flag
-// written by the block's owner), then subtracts the block's contribution.
+// sums in registers; it applies the blocks of earlier slabs as their
+// solutions are published (a per-block flag written by the block's owner),
+// several ready blocks per round, with its rows' slice of the next block
+// loaded ahead of time.
 // For its own blocks it multiplies the block's residual by the inverse of
 // the 32 x 32 diagonal block (precomputed once per factorisation, so the
 // dependency chain carries no sequential substitution), publishes the
@@ -39,10 +40,11 @@
 
 namespace mh {
 
-constexpr int kTsRows = 128;  // rows per workgroup, one per thread
+constexpr int kTsRows = 256;  // rows per workgroup, one per thread
 constexpr int kTsCB = 32;     // block of unknowns = diagonal block
 constexpr int kTsNR = 4;      // right-hand sides per launch
 constexpr int kTsOwn = kTsRows / kTsCB;
+constexpr int kTsBatch = 8;   // ready blocks of earlier slabs applied per round
 constexpr unsigned kSpinLimit = 1u << 22;
 
 // Thread 0 waits until *flag == epoch (or an error is flagged), then the
@@ -87,10 +89,11 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
     int nsubp, unsigned epoch, unsigned long long* __restrict__ ticket, int* __restrict__ err) {
     typedef typename Cx<T>::t C2;
     __shared__ unsigned long long s_tk;
-    __shared__ C2 sI[kTsOwn][kTsCB][kTsCB + 1];  // inverse diagonal blocks of this slab
-    __shared__ C2 sx[kTsCB][NR];                 // solution of the current block
-    __shared__ C2 sr[kTsCB][NR];                 // residual of the current own block
-    const int tid = threadIdx.x;
+    __shared__ int s_ok[kTsBatch];
+    __shared__ C2 sI[2][kTsCB][kTsCB + 1];  // inverse diagonal blocks, double-buffered
+    __shared__ C2 sx[kTsBatch][kTsCB][NR];  // solutions of the blocks being applied
+    __shared__ C2 sr[kTsCB][NR];            // residual of the current own block
+    const int tid = threadIdx.x, wave = tid >> 6;
     if (tid == 0) s_tk = atomicAdd(ticket, 1ull);
     __syncthreads();
     const int64_t tk = (int64_t)s_tk;
@@ -114,14 +117,15 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
     const int cown = (int)(i0 / kTsCB);
     const int cend = (cown + kTsOwn) < nsubp ? cown + kTsOwn : nsubp;
 
-    // inverse diagonal blocks of this slab's own blocks (physical block K,
-    // element (a, b) at Inv[((item * nsubp + K) * 32 + b) * 32 + a])
-    for (int s = 0; s < cend - cown; ++s) {
+    // inverse of own block s (physical block K, element (a, b) at
+    // Inv[((item * nsubp + K) * 32 + b) * 32 + a]) into buffer s & 1
+    auto load_inv = [&](int s) {
         const int c = cown + s;
         const int64_t K = REV ? nsubp - 1 - c : c;
         const C2* src = Inv + (item * nsubp + K) * kTsCB * kTsCB;
-        for (int e = tid; e < kTsCB * kTsCB; e += kTsRows) sI[s][e % kTsCB][e / kTsCB] = src[e];
-    }
+        for (int e = tid; e < kTsCB * kTsCB; e += kTsRows) sI[s & 1][e % kTsCB][e / kTsCB] = src[e];
+    };
+    if (cend > cown) load_inv(0);
 
     C2 v[NR];
     {
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
     }
 
     // this row's slice of block c (logical columns 32c .. 32c+31), for rows
-    // after the block
+    // after the block, and its application with the block's solution x
     C2 m[kTsCB];
     auto prefetch = [&](int c) {
         const bool need = valid && i >= (int64_t)(c + 1) * kTsCB;
@@ -141,62 +145,90 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
             m[q] = (need && pj < n) ? (TR ? M[pj + pi * ld] : M[pi + pj * ld]) : zero;
         }
     };
-    prefetch(0);
-    for (int c = 0; c < cend; ++c) {
-        if (c < cown) {
-            wait_flag(fl + c, epoch, err);
-            if (tid < kTsCB * nrhs) {
-                const int q = tid % kTsCB, r = tid / kTsCB;
-                const int64_t pj = phys((int64_t)c * kTsCB + q);
-                sx[q][r] = (pj < n) ? load_published(bout + (sp ? (int64_t)sp[pj] : pj) + r * ldo) : zero;
-            }
-            __syncthreads();
-        } else {
-            const int s = c - cown, base = s * kTsCB;
-            const bool mine = tid >= base && tid < base + kTsCB;
-            if (mine) {
-#pragma unroll
-                for (int r = 0; r < NR; ++r) sr[tid - base][r] = v[r];
-            }
-            __syncthreads();
-            if (mine) {
-                const int a = tid - base;
-                const int pa = REV ? kTsCB - 1 - a : a;
-                C2 acc[NR];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) acc[r] = zero;
-#pragma unroll 8
-                for (int b = 0; b < kTsCB; ++b) {
-                    const int pb = REV ? kTsCB - 1 - b : b;
-                    const C2 w = TR ? sI[s][pb][pa] : sI[s][pa][pb];
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) {
-                        const C2 y = sr[b][r];
-                        acc[r].x += w.x * y.x - w.y * y.y;
-                        acc[r].y += w.x * y.y + w.y * y.x;
-                    }
-                }
-                const int64_t dst = sp ? (valid ? (int64_t)sp[pi] : 0) : pi;
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    v[r] = acc[r];
-                    sx[a][r] = acc[r];
-                    if (valid && r < nrhs) store_published(bout + dst + r * ldo, acc[r]);
-                }
-            }
-            if ((tid >> 6) == (base >> 6)) __builtin_amdgcn_s_waitcnt(0);  // solution stores complete
-            __syncthreads();
-            if (tid == 0) __hip_atomic_store(fl + c, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    auto apply = [&](int c, C2 (*x)[NR]) {
         if (valid && i >= (int64_t)(c + 1) * kTsCB) {
 #pragma unroll
             for (int q = 0; q < kTsCB; ++q)
 #pragma unroll
                 for (int r = 0; r < NR; ++r)
-                    if (r < nrhs) v[r] = cfms(v[r], m[q], sx[q][r]);
+                    if (r < nrhs) v[r] = cfms(v[r], m[q], x[q][r]);
+        }
+    };
+    prefetch(0);
+
+    // 1. blocks of earlier slabs, applied in rounds of up to kTsBatch blocks
+    //    that are ready
+    int c = 0;
+    while (c < cown) {
+        wait_flag(fl + c, epoch, err);
+        if (tid < kTsBatch) {
+            bool ok = tid == 0;
+            if (tid > 0 && c + tid < cown)
+                ok = __hip_atomic_load(fl + c + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+            s_ok[tid] = ok ? 1 : 0;
         }
         __syncthreads();
-        if (c + 1 < cend) prefetch(c + 1);
+        int nb = 1;
+        while (nb < kTsBatch && s_ok[nb]) ++nb;
+        for (int e = tid; e < nb * kTsCB * NR; e += kTsRows) {
+            const int b = e / (kTsCB * NR), q = e % kTsCB, r = (e / kTsCB) % NR;
+            const int64_t pj = phys((int64_t)(c + b) * kTsCB + q);
+            sx[b][q][r] = (r < nrhs && pj < n) ? load_published(bout + (sp ? (int64_t)sp[pj] : pj) + r * ldo)
+                                               : zero;
+        }
+        __syncthreads();
+        apply(c, sx[0]);
+        for (int b = 1; b < nb; ++b) {
+            prefetch(c + b);
+            apply(c + b, sx[b]);
+        }
+        c += nb;
+        __syncthreads();
+        prefetch(c);
+    }
+
+    // 2. this slab's own blocks: x = T_cc^-1 r, published, then applied to
+    //    the rows after the block
+    for (int s = 0; s < cend - cown; ++s) {
+        const int cb = cown + s, base = s * kTsCB;
+        const bool mine = tid >= base && tid < base + kTsCB;
+        if (mine) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sr[tid - base][r] = v[r];
+        }
+        if (s + 1 < cend - cown) load_inv(s + 1);
+        __syncthreads();
+        if (mine) {
+            const int a = tid - base;
+            const int pa = REV ? kTsCB - 1 - a : a;
+            C2 acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = zero;
+#pragma unroll 8
+            for (int b = 0; b < kTsCB; ++b) {
+                const int pb = REV ? kTsCB - 1 - b : b;
+                const C2 w = TR ? sI[s & 1][pb][pa] : sI[s & 1][pa][pb];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const C2 y = sr[b][r];
+                    acc[r].x += w.x * y.x - w.y * y.y;
+                    acc[r].y += w.x * y.y + w.y * y.x;
+                }
+            }
+            const int64_t dst = sp ? (valid ? (int64_t)sp[pi] : 0) : pi;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                v[r] = acc[r];
+                sx[0][a][r] = acc[r];
+                if (valid && r < nrhs) store_published(bout + dst + r * ldo, acc[r]);
+            }
+        }
+        if (wave == (base >> 6)) __builtin_amdgcn_s_waitcnt(0);  // solution stores complete
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(fl + cb, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        apply(cb, sx[0]);
+        __syncthreads();
+        if (s + 1 < cend - cown) prefetch(cb + 1);
     }
 }
 
@@ -288,27 +320,28 @@ __global__ void k_copy_cols(int64_t rows, int64_t cols, int64_t batch, const C2*
 // per-context synchronisation state: [ticket u64 | pad | flags u32 ...], the
 // error word and a solution workspace for the out-of-place steps
 static int tri_state(Context* c, int64_t batch, int64_t nsub, size_t ws_bytes) {
+    Context::TriSlot& t = c->tri[c->tri_slot];
     const size_t need = 256 + sizeof(unsigned) * (size_t)(batch * nsub);
-    if (need > c->tri_sync_bytes) {
-        if (c->tri_sync) {
+    if (need > t.sync_bytes) {
+        if (t.sync) {
             MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-            MH_HIP_CHECK(hipFree(c->tri_sync));
-            c->tri_sync = nullptr;
-            c->tri_sync_bytes = 0;
+            MH_HIP_CHECK(hipFree(t.sync));
+            t.sync = nullptr;
+            t.sync_bytes = 0;
         }
-        MH_HIP_CHECK(hipMalloc(&c->tri_sync, need));
-        MH_HIP_CHECK(hipMemset(c->tri_sync, 0, need));
-        c->tri_sync_bytes = need;
+        MH_HIP_CHECK(hipMalloc(&t.sync, need));
+        MH_HIP_CHECK(hipMemset(t.sync, 0, need));
+        t.sync_bytes = need;
     }
-    if (ws_bytes > c->tri_ws_bytes) {
-        if (c->tri_ws) {
+    if (ws_bytes > t.ws_bytes) {
+        if (t.ws) {
             MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-            MH_HIP_CHECK(hipFree(c->tri_ws));
-            c->tri_ws = nullptr;
-            c->tri_ws_bytes = 0;
+            MH_HIP_CHECK(hipFree(t.ws));
+            t.ws = nullptr;
+            t.ws_bytes = 0;
         }
-        MH_HIP_CHECK(hipMalloc(&c->tri_ws, ws_bytes));
-        c->tri_ws_bytes = ws_bytes;
+        MH_HIP_CHECK(hipMalloc(&t.ws, ws_bytes));
+        t.ws_bytes = ws_bytes;
     }
     if (!c->tri_err) {
         MH_HIP_CHECK(hipMalloc(&c->tri_err, sizeof(int)));
@@ -358,14 +391,15 @@ static int launch_trsv(Context* c, int64_t n, int64_t batch, const typename Cx<T
                        typename Cx<T>::t* out, int64_t ldo, int64_t ostride) {
     const int nsubp = (int)((n + kTsCB - 1) / kTsCB);
     const int nslab = (int)((nsubp * (int64_t)kTsCB + kTsRows - 1) / kTsRows);
-    unsigned long long* ticket = reinterpret_cast<unsigned long long*>(c->tri_sync);
-    unsigned* flags = reinterpret_cast<unsigned*>(static_cast<char*>(c->tri_sync) + 256);
-    if (++c->tri_epoch == 0) c->tri_epoch = 1;
+    Context::TriSlot& ts = c->tri[c->tri_slot];
+    unsigned long long* ticket = reinterpret_cast<unsigned long long*>(ts.sync);
+    unsigned* flags = reinterpret_cast<unsigned*>(static_cast<char*>(ts.sync) + 256);
+    if (++ts.epoch == 0) ts.epoch = 1;
     hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, c->stream, ticket);
 #define MH_TRSV(NRR)                                                                                         \
     hipLaunchKernelGGL((k_trsv<T, TR, REV, NRR>), dim3((unsigned)(nslab * batch)), dim3(kTsRows), 0, c->stream, \
                        n, batch, A, ld, stride, inv, gperm, sperm, nrhs, in, ldi, istride, out, ldo, ostride,    \
-                       nslab, flags, nsubp, c->tri_epoch, ticket, c->tri_err)
+                       nslab, flags, nsubp, ts.epoch, ticket, c->tri_err)
     if (nrhs <= 1)
         MH_TRSV(1);
     else if (nrhs <= 2)
@@ -390,7 +424,7 @@ static int tri_solve_t(Context* c, int64_t n, int64_t batch, const typename Cx<T
     MH_REQUIRE((part & 1) == 0 || perm, "lu_solve: permutation missing");
     MH_REQUIRE(inv, "lu_solve: inverse diagonal blocks missing");
     TRY_RC(tri_state(c, batch, nsubp, sizeof(C2) * (size_t)(n * kTsNR * batch)));
-    C2* W = static_cast<C2*>(c->tri_ws);
+    C2* W = static_cast<C2*>(c->tri[c->tri_slot].ws);
     const int64_t ws = n * kTsNR;
     const C2* invL = inv;
     const C2* invU = inv + batch * nsubp * kTsCB * kTsCB;
