@@ -264,11 +264,21 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
     const double2* sDw = sD + w * wstride_s;
     const double2* aSw = aS + w * wstride_a;
     const double2* aDw = aD + w * wstride_a;
-    double2 RS[NR][P], RD[NR][P], US[NR][P], UD[NR][P];
+    // (q, p) accumulators in LDS (one slot per thread) when small enough:
+    // keeps the pair loop within two waves per SIMD
+    constexpr bool kLdsU = NR * P <= 8;
+    __shared__ double2 sU[kLdsU ? 2 * NR * P : 1][256];
+    double2 RS[NR][P], RD[NR][P], US[kLdsU ? 1 : NR][P], UD[kLdsU ? 1 : NR][P];
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int m = 0; m < P; ++m) RS[r][m] = RD[r][m] = US[r][m] = UD[r][m] = make_double2(0.0, 0.0);
+        for (int m = 0; m < P; ++m) RS[r][m] = RD[r][m] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < (kLdsU ? 1 : NR); ++r)
+#pragma unroll
+        for (int m = 0; m < P; ++m) US[r][m] = UD[r][m] = make_double2(0.0, 0.0);
+    if (kLdsU)
+        for (int e = 0; e < 2 * NR * P; ++e) sU[e][threadIdx.x] = make_double2(0.0, 0.0);
     auto cmul = [](const CD& a, const CD& b) {
         CD o;
         o.re = a.re * b.re - a.im * b.im;
@@ -351,8 +361,15 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
                     const double2 ad = aDw[j + d * n];
 #pragma unroll
                     for (int m = 0; m < P; ++m) {
-                        US[d][m] = cmac(US[d][m], -kInv4Pi * Sqp.re.d[m], -kInv4Pi * Sqp.im.d[m], as);
-                        UD[d][m] = cmac(UD[d][m], -kInv4Pi * Dqp.re.d[m], -kInv4Pi * Dqp.im.d[m], ad);
+                        if (kLdsU) {
+                            double2& us = sU[d * P + m][threadIdx.x];
+                            double2& ud = sU[(NR + d) * P + m][threadIdx.x];
+                            us = cmac(us, -kInv4Pi * Sqp.re.d[m], -kInv4Pi * Sqp.im.d[m], as);
+                            ud = cmac(ud, -kInv4Pi * Dqp.re.d[m], -kInv4Pi * Dqp.im.d[m], ad);
+                        } else {
+                            US[d][m] = cmac(US[d][m], -kInv4Pi * Sqp.re.d[m], -kInv4Pi * Sqp.im.d[m], as);
+                            UD[d][m] = cmac(UD[d][m], -kInv4Pi * Dqp.re.d[m], -kInv4Pi * Dqp.im.d[m], ad);
+                        }
                     }
                 }
             }
@@ -369,12 +386,17 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
         for (int m = 0; m < P; ++m) {
             usd[m] = make_double2(0.0, 0.0);
             udd[m] = make_double2(0.0, 0.0);
+            if (kLdsU) {
+                usd[m] = sU[d * P + m][threadIdx.x];
+                udd[m] = sU[(NR + d) * P + m][threadIdx.x];
+            } else {
 #pragma unroll
-            for (int dd = 0; dd < NR; ++dd)
-                if (dd == d) {
-                    usd[m] = US[dd][m];
-                    udd[m] = UD[dd][m];
-                }
+                for (int dd = 0; dd < (kLdsU ? 1 : NR); ++dd)
+                    if (dd == d) {
+                        usd[m] = US[dd][m];
+                        udd[m] = UD[dd][m];
+                    }
+            }
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
