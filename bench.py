@@ -6,9 +6,13 @@ Workload: floating hemisphere R = 1 m, 64 x 64 latitude-longitude mesh
 Rankine + image + Wu Green function, direct formulation, heave radiation and
 one diffraction heading (beta = 0) per frequency, g = 9.81, rho = 1025.
 One step = one batched sweep over the 64 frequencies: assembly of S and D,
-one LU per frequency, both right-hand sides, forces.  Inputs (mesh, dof
-fields) are resident in HBM before the timed region; outputs are copied back
-inside it.
+one LU per frequency, both right-hand sides, forces, and the adjoint
+gradients of A, B, F with respect to theta = (radius, omega).  Default
+precision: mixed (FP32 LU + FP64 iterative refinement); the same sweep in
+FP64 is run once outside the timed region and the largest relative
+difference is reported (the metric's bound is 1e-6).  Inputs (mesh, dof
+fields, tangents) are resident in HBM before the timed region; outputs are
+copied back inside it.
 
 Prints one JSON line: value = freq-solves/s (frequencies x steps / time).
 """
@@ -35,7 +39,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--ntheta", type=int, default=64)
     ap.add_argument("--nphi", type=int, default=64)
     ap.add_argument("--nomega", type=int, default=64)
@@ -45,6 +49,8 @@ def main():
                     help="forward sweep only (no adjoint gradients)")
     ap.add_argument("--profile-only", action="store_true",
                     help="one sweep, no timing loop (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep is "
@@ -60,6 +66,15 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU visible")
         sys.exit(1)
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copied {nbytes} bytes 3 times")
+        return
     rho, g = 1025.0, 9.81
     body = mh.hemisphere(1.0, ntheta=args.ntheta, nphi=args.nphi)
     mesh = body.mesh
@@ -114,6 +129,17 @@ def main():
         log(f"bench: FP64 forward + central-FD sweep {fd_ms:.1f} ms "
             f"({fd_ms / (dt * 1e3):.2f}x the timed step)")
 
+    # the same sweep in FP64 (outside the timed region): mixed vs FP64 error
+    rel_fp64 = None
+    if args.precision != "fp64":
+        ref = mh.freq_sweep_grad(dm, omegas, dofs, betas, T, precision="fp64", g=g, rho=rho) if grad \
+            else forward("fp64")
+        rel_fp64 = 0.0
+        for a, b in zip(res, ref):
+            rel_fp64 = max(rel_fp64, float(np.max(np.abs(np.asarray(a) - np.asarray(b))) /
+                                         max(float(np.max(np.abs(np.asarray(b)))), 1e-300)))
+        log(f"bench: max rel difference {args.precision} vs FP64 (A, B, F and gradients) {rel_fp64:.2e}")
+
     # kernel timer over one more sweep: dominant kernel and its roofline
     mh._lib.profile_enable(True)
     step()
@@ -133,6 +159,16 @@ def main():
     roof["kernel"] = dom_name
     roof["avg_launch_ms"] = round(avg_ms, 4)
     roof["launches_per_step"] = dom["launches"]
+    # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of
+    # the same workload, scaled by the counters' reading for a copy of known
+    # size (profiles/traffic.json, written from those runs)
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        ent = tr.get(f"{dom_name}|{args.precision}|{n}x{args.nomega}")
+        if ent:
+            roof["traffic"] = ent["bytes_per_launch"]
     kernel_ms = {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"bench: kernel ms per sweep {kernel_ms}")
 
@@ -206,6 +242,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "max_rel_err_vs_oracle": max_err,
+        "max_rel_diff_vs_fp64": rel_fp64,
         "max_rel_err_grad_vs_oracle_fd": max_gerr,
         "fp64_forward_fd_sweep_ms": round(fd_ms, 3) if fd_ms else None,
         "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,

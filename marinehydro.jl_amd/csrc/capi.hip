@@ -433,7 +433,11 @@ struct IrSystem {
 // Iterative refinement of up to two systems with the same FP32 factors, the
 // second on the aux stream so both proceed together: X <- X + LU32^{-1}
 // (Bm - op(A) X) until every column's relative residual max|r| / max|b| is
-// below 1e-12 (at most 12 sweeps).
+// below kIrTol (at most 12 sweeps).  Each sweep gains about the FP32 LU's
+// accuracy (~1e-6 relative for these boundary-element systems), so two
+// sweeps typically reach 1e-11 and the results agree with the FP64 path to
+// ~1e-10, well inside the 1e-6 the mixed-precision mode is specified for.
+constexpr double kIrTol = 1e-10;
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
                     int nsys) {
@@ -481,14 +485,14 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
             if (!done[k]) {
                 double worst = 0.0;
                 for (size_t e = 0; e < cnt; ++e) worst = c->h_pin[off + e] > worst ? c->h_pin[off + e] : worst;
-                done[k] = worst < 1e-12;
+                done[k] = worst < kIrTol;
             }
             all = all && done[k];
             off += cnt;
         }
         if (all) return MH_OK;
     }
-    set_error("mixed precision: iterative refinement did not reach a 1e-12 relative residual");
+    set_error("mixed precision: iterative refinement did not reach a 1e-10 relative residual");
     return MH_ERR_NOT_CONVERGED;
 }
 

@@ -415,9 +415,11 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                    int64_t k_lo, int kw) {
     if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
-    ProfScope ps(P.ctx, sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32", 8.0 * (double)mr * (double)mc * kw * P.batch,
+    const bool big = mr >= kBT && mc >= kBT;
+    const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_big_f32") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
+    ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
-    if (mr >= kBT && mc >= kBT)
+    if (big)
         hipLaunchKernelGGL(k_lu_gemm_big<T>,
                            dim3((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch),
                            dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);

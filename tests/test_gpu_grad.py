@@ -69,8 +69,9 @@ def test_single_hemisphere_radius_position_omega(mh, oracle, gpu, direct, precis
     A, B, F, dA, dB, dF = mh.freq_sweep_grad(mesh, [omega], dofs_of(mesh), betas, T, direct=direct,
                                              rho=1025.0, precision=precision)
     Ao, Bo, Fo = oracle.freq_solve(GF_WU, mesh, omega, dofs_of(mesh), betas, direct=direct, rho=1025.0)
-    check(A[0], Ao, 1e-9, "A")
-    check(F[0].T, Fo, 1e-9, "F")
+    tol = 1e-10 if precision == "fp64" else 1e-8  # mixed: refinement to a 1e-10 residual
+    check(A[0], Ao, tol, "A")
+    check(F[0].T, Fo, tol, "F")
     for m in range(3):
         gA, gB, gF = fd(oracle, make, theta0, m, lambda t: t[2], dofs_of, betas, direct, 1e-5)
         check(dA[0, m], gA, 1e-6, ("dA", m), A)

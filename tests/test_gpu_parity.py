@@ -205,13 +205,14 @@ def test_solve_matches_oracle(mh, oracle, gpu, direct):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("direct", [True, False])
 def test_freq_sweep_matches_oracle(mh, oracle, gpu, direct, precision):
-    # mixed: FP32 LU + FP64 iterative refinement to a 1e-12 relative residual
+    # mixed: FP32 LU + FP64 iterative refinement to a 1e-10 relative residual
+    # (bound 1e-8 on the outputs; the mode's specification is 1e-6)
     m = mh.hemisphere(1.0, ntheta=12, nphi=12).mesh
     omegas = [0.3, 0.9, 1.03, 2.0, 3.0]
     dofs = mh.rigid_dofs(m, ["Surge", "Heave", "Pitch"])
     betas = [0.0, 0.7]
     A, B, F = mh.freq_sweep(m, omegas, dofs, betas, direct=direct, rho=1025.0, precision=precision)
-    tol = 1e-10 if precision == "fp64" else 1e-9
+    tol = 1e-10 if precision == "fp64" else 1e-8
     for w, om in enumerate(omegas):
         Ao, Bo, Fo = oracle.freq_solve(GF_WU, m, om, dofs, betas, direct=direct, rho=1025.0)
         assert relmax(A[w], Ao) < tol, om
@@ -220,13 +221,13 @@ def test_freq_sweep_matches_oracle(mh, oracle, gpu, direct, precision):
 
 
 def test_mixed_precision_north_star_size(mh, gpu):
-    """2048 panels: mixed-precision sweep against the FP64 sweep at the
-    north-star tolerance (1e-6 relative; refinement gives far better)."""
+    """2048 panels: mixed-precision sweep against the FP64 sweep; the mode is
+    specified to 1e-6 relative, refinement to a 1e-10 residual gives ~1e-10."""
     m = mh.hemisphere(1.0, ntheta=64, nphi=64).mesh
     omegas = [0.3, 1.2, 3.0]
     A, B, F = mh.freq_sweep(m, omegas, [[0, 0, 1.0]], [0.0], rho=1025.0)
     Am, Bm, Fm = mh.freq_sweep(m, omegas, [[0, 0, 1.0]], [0.0], rho=1025.0, precision="mixed")
-    assert relmax(Am, A) < 1e-9 and relmax(Bm, B) < 1e-9 and relmax(Fm, F) < 1e-9
+    assert relmax(Am, A) < 1e-8 and relmax(Bm, B) < 1e-8 and relmax(Fm, F) < 1e-8
 
 
 def test_reference_entry_points(mh, oracle, gpu):
