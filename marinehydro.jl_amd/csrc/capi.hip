@@ -62,6 +62,21 @@ static int check_mesh(const mh_mesh_t* m) {
     return MH_OK;
 }
 
+int ensure_aux(Context* c) {
+    if (c->aux) return MH_OK;
+    MH_HIP_CHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return MH_OK;
+}
+int ensure_events(Context* c, size_t count) {
+    while (c->ev_pool.size() < count) {
+        hipEvent_t e;
+        MH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_pool.push_back(e);
+    }
+    return MH_OK;
+}
 }  // namespace mh
 
 using namespace mh;
@@ -129,6 +144,7 @@ int mh_context_destroy(mh_context_t ctx) {
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     delete c;
     return MH_OK;
 }
@@ -367,13 +383,6 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
 
 // ---- second stream: independent work (forward and adjoint solves) runs
 // on ctx->aux, ordered against ctx->stream by events --------------------------
-static int ensure_aux(Context* c) {
-    if (c->aux) return MH_OK;
-    MH_HIP_CHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
-    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-    return MH_OK;
-}
 // aux waits for everything issued on stream so far
 static int fork_aux(Context* c) {
     TRY(ensure_aux(c));

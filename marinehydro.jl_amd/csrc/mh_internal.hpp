@@ -49,6 +49,8 @@ struct Context {
     // pinned host scratch for small device-to-host reads
     double* h_pin = nullptr;
     size_t h_pin_count = 0;
+    // events for ordering work between stream and aux (LU look-ahead)
+    std::vector<hipEvent_t> ev_pool;
     bool profiling = false;
     std::vector<ProfRec> prof;
 };
@@ -73,6 +75,10 @@ struct ProfScope {
 
 // Grows ctx->ws to at least `bytes` (synchronous hipMalloc; only when larger).
 bool ensure_workspace(Context* ctx, size_t bytes);
+// Creates ctx->aux and its fork/join events on first use.
+int ensure_aux(Context* ctx);
+// At least `count` events (timing disabled) in ctx->ev_pool.
+int ensure_events(Context* ctx, size_t count);
 
 #define MH_HIP_CHECK(expr)                                                   \
     do {                                                                     \

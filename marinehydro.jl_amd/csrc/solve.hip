@@ -478,6 +478,15 @@ static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
     return MH_OK;
 }
 
+// Right-looking blocked LU with look-ahead: the factorisation of panel k+1
+// (latency-bound: pivot searches, small updates) runs on `stream` while the
+// wide trailing update of panel k (the large GEMM) runs on `aux`:
+//   stream: [wait B(k-2)] factor P_k | [wait B(k-1)] swap + TRSM + GEMM on
+//           the next panel's columns only
+//   aux:    [wait P_k factored] left row swaps of P_k, swap + TRSM + GEMM on
+//           the columns beyond the next panel
+// The row interchanges of P_k on the columns left of it are applied in B(k),
+// after every update that reads those columns has completed.
 template <class T>
 static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, typename Cx<T>::t* A,
                        int64_t ld, int64_t stride, int64_t* ipiv, int* d_info) {
@@ -485,14 +494,48 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
     LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info};
-    for (int64_t k0 = 0; k0 < n; k0 += kNBOuter) {
+    const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
+    TRY_RC(ensure_aux(ctx));
+    TRY_RC(ensure_events(ctx, (size_t)(2 * npan + 1)));
+    hipEvent_t* evF = ctx->ev_pool.data();         // panel k factored (stream)
+    hipEvent_t* evB = ctx->ev_pool.data() + npan;  // wide update k done (aux)
+    hipEvent_t ev0 = ctx->ev_pool[2 * npan];
+    const hipStream_t sA = ctx->stream, sB = ctx->aux;
+    MH_HIP_CHECK(hipEventRecord(ev0, sA));  // aux starts after the work already issued
+    MH_HIP_CHECK(hipStreamWaitEvent(sB, ev0, 0));
+    int rc = MH_OK;
+    for (int64_t k = 0; k < npan && rc == MH_OK; ++k) {
+        const int64_t k0 = k * kNBOuter;
         const int w = (int)((n - k0) < kNBOuter ? (n - k0) : kNBOuter);
-        TRY_RC(lu_panel(P, k0, w));
-        TRY_RC(lu_swap(P, k0, w, 0, k0));
-        TRY_RC(lu_swap(P, k0, w, k0 + w, ncols));
-        TRY_RC(lu_trsm(P, k0, w, k0 + w, ncols));
-        TRY_RC(lu_gemm(P, k0 + w, n, k0 + w, ncols, k0, w));
+        const int64_t kn0 = k0 + w;
+        const int wn = (int)((n - kn0) < kNBOuter ? (n - kn0) : kNBOuter);
+        const int64_t cw0 = kn0 + wn;  // first column of the wide update
+        if (k >= 2) MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[k - 2], 0));
+        rc = lu_panel(P, k0, w);
+        if (rc != MH_OK) break;
+        MH_HIP_CHECK(hipEventRecord(evF[k], sA));
+        // wide update on aux
+        ctx->stream = sB;
+        MH_HIP_CHECK(hipStreamWaitEvent(sB, evF[k], 0));
+        rc = lu_swap(P, k0, w, 0, k0);
+        if (rc == MH_OK) rc = lu_swap(P, k0, w, cw0, ncols);
+        if (rc == MH_OK) rc = lu_trsm(P, k0, w, cw0, ncols);
+        if (rc == MH_OK) rc = lu_gemm(P, kn0, n, cw0, ncols, k0, w);
+        ctx->stream = sA;
+        if (rc != MH_OK) break;
+        MH_HIP_CHECK(hipEventRecord(evB[k], sB));
+        // narrow update of the next panel on stream
+        if (wn > 0) {
+            if (k >= 1) MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[k - 1], 0));
+            rc = lu_swap(P, k0, w, kn0, cw0);
+            if (rc == MH_OK) rc = lu_trsm(P, k0, w, kn0, cw0);
+            if (rc == MH_OK) rc = lu_gemm(P, kn0, n, kn0, cw0, k0, w);
+        }
     }
+    ctx->stream = sA;
+    // stream continues after all of aux's updates
+    MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[npan - 1], 0));
+    if (rc != MH_OK) return rc;
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
