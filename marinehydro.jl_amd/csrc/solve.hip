@@ -21,14 +21,13 @@
 namespace mh {
 
 constexpr int kNBOuter = 128;  // block-column width of the trailing updates
-constexpr int kLeaf = 8;       // register-resident leaf width
 constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= 8, held in registers.
 // Thread t owns local rows t + 1024 q, q < R.
 // ---------------------------------------------------------------------------
-template <class T, int R>
+template <class T, int R, int kLeaf>
 __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
                                                   int64_t stride, int64_t k0, int w,
                                                   int64_t* __restrict__ ipiv,
@@ -446,19 +445,34 @@ static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t 
     return MH_OK;
 }
 
+// Leaf width for a panel of m rows: the leaf keeps R x width complex values
+// in registers per thread (R rows of 1024 threads, so at most 128 VGPRs per
+// thread): 64 VGPRs of panel values and no spills.
+template <class T>
+static int leaf_width(int64_t m) {
+    const int R = m <= 1024 ? 1 : (m <= 2048 ? 2 : 4);
+    if (sizeof(T) == 4) return R <= 2 ? 16 : 8;
+    return R == 1 ? 16 : 8;
+}
+
 template <class T>
 static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
     const int64_t m = P.n - k0;
     ProfScope ps(P.ctx, "lu_leaf", 8.0 * (double)m * w * w / 2.0 * P.batch, 32.0 * (double)m * w * P.batch);
-    if (m <= 1024)
-        hipLaunchKernelGGL((k_lu_leaf<T, 1>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
-                           P.ld, P.stride, k0, w, P.ipiv, P.info);
-    else if (m <= 2048)
-        hipLaunchKernelGGL((k_lu_leaf<T, 2>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
-                           P.ld, P.stride, k0, w, P.ipiv, P.info);
-    else
-        hipLaunchKernelGGL((k_lu_leaf<T, 4>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A,
-                           P.ld, P.stride, k0, w, P.ipiv, P.info);
+    const int W = leaf_width<T>(m);
+#define LEAF(RR, WW)                                                                                        \
+    hipLaunchKernelGGL((k_lu_leaf<T, RR, WW>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A, \
+                       P.ld, P.stride, k0, w, P.ipiv, P.info)
+    (void)W;
+    if (m <= 1024) {
+        LEAF(1, 16);
+    } else if (m <= 2048) {
+        if constexpr (sizeof(T) == 4) LEAF(2, 16);
+        else LEAF(2, 8);
+    } else {
+        LEAF(4, 8);
+    }
+#undef LEAF
     return MH_OK;
 }
 
@@ -466,9 +480,10 @@ static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
 // the panel's columns (zgetrf2 order)
 template <class T>
 static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
-    if (w <= kLeaf) return lu_leaf(P, k0, w);
-    int h = ((w / 2) + kLeaf - 1) / kLeaf * kLeaf;
-    if (h >= w) h = w - kLeaf;
+    const int L = leaf_width<T>(P.n - k0);
+    if (w <= L) return lu_leaf(P, k0, w);
+    int h = ((w / 2) + L - 1) / L * L;
+    if (h >= w) h = w - L;
     TRY_RC(lu_panel(P, k0, h));
     TRY_RC(lu_swap(P, k0, h, k0 + h, k0 + w));
     TRY_RC(lu_trsm(P, k0, h, k0 + h, k0 + w));
