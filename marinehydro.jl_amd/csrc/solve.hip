@@ -28,7 +28,7 @@ constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 // Thread t owns local rows t + 1024 q, q < R.
 // ---------------------------------------------------------------------------
 template <class T, int R, int kLeaf>
-__global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
                                                   int64_t stride, int64_t k0, int w,
                                                   int64_t* __restrict__ ipiv,
                                                   int* __restrict__ info) {
@@ -37,10 +37,10 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* 
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = (int)(n - k0);
-    __shared__ T s_v[16];
-    __shared__ int s_i[16];
-    __shared__ C2 s_crow[kLeaf], s_prow[kLeaf];
-    __shared__ int s_p, s_zero;
+    // double-buffered by column parity: two barriers per column
+    __shared__ T s_v[2][16];
+    __shared__ int s_i[2][16];
+    __shared__ C2 s_crow[2][kLeaf], s_prow[2][kLeaf];
     C2 a[R][kLeaf];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -52,6 +52,7 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* 
 #pragma unroll
     for (int c = 0; c < kLeaf; ++c) {
         if (c < w) {  // uniform
+        const int bf = c & 1;
         T best = T(-1);
         int bi = 0x7fffffff;
 #pragma unroll
@@ -74,60 +75,66 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* 
             }
         }
         if (lane == 0) {
-            s_v[wave] = best;
-            s_i[wave] = bi;
+            s_v[bf][wave] = best;
+            s_i[bf][wave] = bi;
         }
         __syncthreads();
-        if (wave == 0) {
-            T b = (lane < 16) ? s_v[lane] : T(-1);
-            int p = (lane < 16) ? s_i[lane] : 0x7fffffff;
-            for (int off = 8; off > 0; off >>= 1) {
-                const T ov = __shfl_xor(b, off);
-                const int oi = __shfl_xor(p, off);
-                if (ov > b || (ov == b && oi < p)) {
-                    b = ov;
-                    p = oi;
-                }
-            }
-            if (lane == 0) {
-                if (b <= T(0)) {  // exact zero pivot: record, no interchange, no update
-                    p = c;
-                    s_zero = 1;
-                    atomicOr(info + blockIdx.x, 1);
-                } else {
-                    s_zero = 0;
-                }
-                s_p = p;
-                piv[k0 + c] = k0 + p;
+        // every wave reduces the 16 wave maxima itself (first index on ties)
+        T b = s_v[bf][lane & 15];
+        int p = s_i[bf][lane & 15];
+        for (int off = 8; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(b, off);
+            const int oi = __shfl_xor(p, off);
+            if (ov > b || (ov == b && oi < p)) {
+                b = ov;
+                p = oi;
             }
         }
-        __syncthreads();
-        const int p = s_p;
+        const bool zero = !(b > T(0));  // exact zero pivot: record, no interchange, no update
+        if (zero) p = c;
+        if (tid == 0) {
+            if (zero) atomicOr(info + blockIdx.x, 1);
+            piv[k0 + c] = k0 + p;
+        }
         if (tid == c) {
 #pragma unroll
-            for (int j = 0; j < kLeaf; ++j) s_crow[j] = a[0][j];
+            for (int j = 0; j < kLeaf; ++j) s_crow[bf][j] = a[0][j];
         }
+        {  // the pivot row's owner publishes it (selects keep register indices static)
+            bool own = false;
 #pragma unroll
-        for (int q = 0; q < R; ++q)
-            if (tid + 1024 * q == p) {
+            for (int q = 0; q < R; ++q) own = own || (tid + 1024 * q == p);
+            if (own) {
 #pragma unroll
-                for (int j = 0; j < kLeaf; ++j) s_prow[j] = a[q][j];
+                for (int j = 0; j < kLeaf; ++j) {
+                    C2 v = a[0][j];
+#pragma unroll
+                    for (int q = 1; q < R; ++q) {
+                        const bool o = tid + 1024 * q == p;
+                        v = mkc<C2>(o ? a[q][j].x : v.x, o ? a[q][j].y : v.y);
+                    }
+                    s_prow[bf][j] = v;
+                }
             }
+        }
         __syncthreads();
         if (p != c) {
             if (tid == c) {
 #pragma unroll
-                for (int j = 0; j < kLeaf; ++j) a[0][j] = s_prow[j];
+                for (int j = 0; j < kLeaf; ++j) a[0][j] = s_prow[bf][j];
             }
 #pragma unroll
-            for (int q = 0; q < R; ++q)
-                if (tid + 1024 * q == p) {
+            for (int q = 0; q < R; ++q) {
+                const bool own = tid + 1024 * q == p;
 #pragma unroll
-                    for (int j = 0; j < kLeaf; ++j) a[q][j] = s_crow[j];
+                for (int j = 0; j < kLeaf; ++j) {
+                    const C2 cr = s_crow[bf][j];
+                    a[q][j] = mkc<C2>(own ? cr.x : a[q][j].x, own ? cr.y : a[q][j].y);
                 }
+            }
         }
-        if (!s_zero) {
-            const C2 inv = crecip(s_prow[c]);
+        if (!zero) {
+            const C2 inv = crecip(s_prow[bf][c]);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int r = tid + 1024 * q;
@@ -135,11 +142,11 @@ __global__ __launch_bounds__(1024) void k_lu_leaf(int64_t n, typename Cx<T>::t* 
                     const C2 l = cmul(a[q][c], inv);
                     a[q][c] = l;
 #pragma unroll
-                    for (int j = c + 1; j < kLeaf; ++j) a[q][j] = cfms(a[q][j], l, s_prow[j]);
+                    for (int j = 1; j < kLeaf; ++j)  // constant trip count: unrolled before c is known
+                        if (j > c) a[q][j] = cfms(a[q][j], l, s_prow[bf][j]);
                 }
             }
         }
-        __syncthreads();
         }
     }
 #pragma unroll
