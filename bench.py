@@ -48,7 +48,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true",
                     help="forward sweep only (no adjoint gradients)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="one sweep, no timing loop (for rocprofv3 runs)")
+                    help="one sweep with the kernel timer on (serialised schedule), no timing loop; "
+                         "for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -99,9 +100,14 @@ def main():
         return forward()
 
     if args.profile_only:
+        mh._lib.profile_enable(True)
         step()
         torch.cuda.synchronize()
-        log("bench: profile-only sweep done")
+        stats = mh._lib.profile_report()
+        mh._lib.profile_enable(False)
+        log("bench: profile-only sweep done; kernel timer (ms, launches, avg ms): " +
+            str({k: (round(v["ms"], 3), v["launches"], round(v["ms"] / v["launches"], 4))
+                 for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}))
         return
 
     for w in range(args.warmup):
@@ -140,7 +146,9 @@ def main():
                                          max(float(np.max(np.abs(np.asarray(b)))), 1e-300)))
         log(f"bench: max rel difference {args.precision} vs FP64 (A, B, F and gradients) {rel_fp64:.2e}")
 
-    # kernel timer over one more sweep: dominant kernel and its roofline
+    # kernel timer over one more sweep (the library serialises its two streams
+    # while the timer is on, so each duration is the kernel's own): dominant
+    # kernel and its roofline
     mh._lib.profile_enable(True)
     step()
     stats = mh._lib.profile_report()

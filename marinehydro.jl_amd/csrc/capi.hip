@@ -382,9 +382,11 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK,
 }
 
 // ---- second stream: independent work (forward and adjoint solves) runs
-// on ctx->aux, ordered against ctx->stream by events --------------------------
+// on ctx->aux, ordered against ctx->stream by events.  With the kernel timer
+// on, everything stays on ctx->stream so each timed kernel runs alone. ------
 // aux waits for everything issued on stream so far
 static int fork_aux(Context* c) {
+    if (c->profiling) return MH_OK;
     TRY(ensure_aux(c));
     MH_HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
     MH_HIP_CHECK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
@@ -392,6 +394,7 @@ static int fork_aux(Context* c) {
 }
 // stream waits for everything issued on aux so far
 static int join_aux(Context* c) {
+    if (c->profiling) return MH_OK;
     MH_HIP_CHECK(hipEventRecord(c->ev_join, c->aux));
     MH_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     return MH_OK;
@@ -402,7 +405,7 @@ struct OnAux {
     hipStream_t saved;
     int saved_slot;
     OnAux(Context* ctx, bool on) : c(ctx), saved(ctx->stream), saved_slot(ctx->tri_slot) {
-        if (on) {
+        if (on && !c->profiling) {
             c->stream = c->aux;
             c->tri_slot = 1;
         }
@@ -486,7 +489,7 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
             off += cnt;
         }
         MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-        if (nsys > 1) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
+        if (nsys > 1 && !c->profiling) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
         off = 0;
         bool all = true;
         for (int k = 0; k < nsys; ++k) {

@@ -508,7 +508,9 @@ static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
 //   aux:    [wait P_k factored] left row swaps of P_k, swap + TRSM + GEMM on
 //           the columns beyond the next panel
 // The row interchanges of P_k on the columns left of it are applied in B(k),
-// after every update that reads those columns has completed.
+// after every update that reads those columns has completed.  With the
+// kernel timer on, the same schedule runs on one stream (each timed kernel
+// alone on the device).
 template <class T>
 static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, typename Cx<T>::t* A,
                        int64_t ld, int64_t stride, int64_t* ipiv, int* d_info) {
@@ -522,7 +524,7 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     hipEvent_t* evF = ctx->ev_pool.data();         // panel k factored (stream)
     hipEvent_t* evB = ctx->ev_pool.data() + npan;  // wide update k done (aux)
     hipEvent_t ev0 = ctx->ev_pool[2 * npan];
-    const hipStream_t sA = ctx->stream, sB = ctx->aux;
+    const hipStream_t sA = ctx->stream, sB = ctx->profiling ? ctx->stream : ctx->aux;
     MH_HIP_CHECK(hipEventRecord(ev0, sA));  // aux starts after the work already issued
     MH_HIP_CHECK(hipStreamWaitEvent(sB, ev0, 0));
     int rc = MH_OK;
