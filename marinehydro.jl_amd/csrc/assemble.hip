@@ -51,7 +51,9 @@ __global__ __launch_bounds__(256) void k_assemble(
     int64_t n, const double* __restrict__ C, const double* __restrict__ N,
     const PanelRec* __restrict__ panels, int gf, int64_t nk, const double* __restrict__ ks,
     double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K, int64_t ld,
-    int64_t stride) {
+    int64_t stride, float2* __restrict__ DK32) {
+    // DK32 (optional): complex64 copy of D (FORM 1) or K (FORM 2), n x n per
+    // wavenumber, for the FP32 factorisation of the mixed-precision path
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t i = (int64_t)blockIdx.x * 64 + lane;
@@ -86,13 +88,17 @@ __global__ __launch_bounds__(256) void k_assemble(
                 const double re = nj0 * (gxiR[0] + gxi[0].re) + nj1 * (gxiR[1] + gxi[1].re) +
                                   nj2 * (gxiR[2] + gxi[2].re);
                 const double im = nj0 * gxi[0].im + nj1 * gxi[1].im + nj2 * gxi[2].im;
-                D[off] = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+                const double2 v = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+                D[off] = v;
+                if (DK32) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
             }
             if (FORM & 2) {
                 const double re = ni0 * (gxR[0] + gx[0].re) + ni1 * (gxR[1] + gx[1].re) +
                                   ni2 * (gxR[2] + gx[2].re);
                 const double im = ni0 * gx[0].im + ni1 * gx[1].im + ni2 * gx[2].im;
-                K[off] = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+                const double2 v = make_double2(diag - kInv4Pi * re, -kInv4Pi * im);
+                K[off] = v;
+                if (DK32 && !(FORM & 1)) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
             }
         }
     }
@@ -120,26 +126,26 @@ template <int WAVE>
 static void dispatch_form(int form, dim3 grid, hipStream_t st, int64_t n, const double* C,
                           const double* N, const PanelRec* P, int gf, int64_t nk,
                           const double* ks, double2* S, double2* D, double2* K, int64_t ld,
-                          int64_t stride) {
+                          int64_t stride, float2* DK32) {
     switch (form) {
         case 1:
             hipLaunchKernelGGL((k_assemble<WAVE, 1>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride);
+                               S, D, K, ld, stride, DK32);
             break;
         case 2:
             hipLaunchKernelGGL((k_assemble<WAVE, 2>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride);
+                               S, D, K, ld, stride, DK32);
             break;
         default:
             hipLaunchKernelGGL((k_assemble<WAVE, 3>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride);
+                               S, D, K, ld, stride, DK32);
             break;
     }
 }
 
 int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, int gf, int form,
                     int64_t nk, const double* d_k, double2* S, double2* D, double2* K, int64_t ld,
-                    int64_t stride) {
+                    int64_t stride, float2* DK32) {
     const int64_t n = m->nfaces;
     if (n == 0 || nk == 0) return MH_OK;
     const int64_t tiles = (n + 63) / 64;
@@ -153,13 +159,13 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
     ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
     if (gf & GF_WU)
         dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
-                         D, K, ld, stride);
+                         D, K, ld, stride, DK32);
     else if (gf & GF_DELHOMMEAU)
         dispatch_form<2>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
-                         D, K, ld, stride);
+                         D, K, ld, stride, DK32);
     else
         dispatch_form<0>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
-                         D, K, ld, stride);
+                         D, K, ld, stride, DK32);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }

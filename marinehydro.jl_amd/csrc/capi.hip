@@ -610,7 +610,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
         const int64_t nw = (nomega - w0) < chunk ? (nomega - w0) : chunk;
         TRY(launch_assemble(c, mesh, P, gf_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
-                            direct ? nullptr : dD, ld, stride));
+                            direct ? nullptr : dD, ld, stride,
+                            mixed ? reinterpret_cast<float2*>(ws + o_D32) : nullptr));
         TRY(launch_bcs(c, mesh, nw, d_om + w0, g, ndof, dof_fields, nbeta, d_be, rhs, ldb, bstride));
         float2* D32 = reinterpret_cast<float2*>(ws + o_D32);
         double2* ybuf = reinterpret_cast<double2*>(ws + o_y);
@@ -619,7 +620,6 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         double* d_nrm = reinterpret_cast<double*>(ws + o_nrm);
         if (mixed) {
             // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64
-            TRY(convert_to_f32(c, n, n, nw, dD, ld, stride, D32, n, n * n));
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
             TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
