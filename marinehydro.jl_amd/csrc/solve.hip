@@ -316,15 +316,13 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
     const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
     // four real products per complex product (re = Ar Br - Ai Bi,
     // im = Ar Bi + Ai Br); Gauss's three-product form measured slower here
-    constexpr bool k3m = false;
-    acc_t accr[4][2], acci[4][2], acc3[k3m ? 4 : 1][k3m ? 2 : 1];
+    acc_t accr[4][2], acci[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             accr[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
             acci[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
-            if (k3m) acc3[k3m ? a : 0][k3m ? b : 0] = (acc_t){T(0), T(0), T(0), T(0)};
         }
     C2 pa[4], pb[4];
     auto gload = [&](int kc) {
@@ -368,17 +366,10 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    if (k3m) {
-                        accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);  // T1
-                        acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].y, acci[a][b]);  // T2
-                        acc3[k3m ? a : 0][k3m ? b : 0] =
-                            Cx<T>::mfma(av[a].x + av[a].y, bv[b].x + bv[b].y, acc3[k3m ? a : 0][k3m ? b : 0]);
-                    } else {
-                        accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
-                        accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
-                        acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
-                        acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
-                    }
+                    accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
+                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
+                    acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
                 }
         }
         __syncthreads();
@@ -399,12 +390,7 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
                 if (gr >= r_hi) continue;
                 C2* o = M + gr + gc * ld;
                 const C2 v = *o;
-                if (k3m) {
-                    const T t1 = accr[a][b][r], t2 = acci[a][b][r], t3 = acc3[k3m ? a : 0][k3m ? b : 0][r];
-                    *o = mkc<C2>(v.x - (t1 - t2), v.y - (t3 - t1 - t2));
-                } else {
-                    *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
-                }
+                *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
             }
         }
 }
