@@ -362,13 +362,19 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
             for (int a = 0; a < 4; ++a) av[a] = sA[kk + (l >> 4)][64 * wr + 16 * a + (l & 15)];
 #pragma unroll
             for (int b = 0; b < 2; ++b) bv[b] = sB[kk + (l >> 4)][32 * wc + 16 * b + (l & 15)];
+            // the two products into each accumulator are 8 MFMAs apart
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
                     accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
-                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
                     acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
+                }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
                     acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
                 }
         }
