@@ -271,10 +271,13 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
-                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
-                    acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
-                    acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
+                    // operands swapped: the MFMA computes the transposed block, so
+                    // the 16 lanes of a result row hold 16 consecutive rows of C
+                    // (coalesced column-major read-modify-write in the epilogue)
+                    accr[a][b] = Cx<T>::mfma(bv[b].x, av[a].x, accr[a][b]);
+                    accr[a][b] = Cx<T>::mfma(-bv[b].y, av[a].y, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(bv[b].y, av[a].x, acci[a][b]);
+                    acci[a][b] = Cx<T>::mfma(bv[b].x, av[a].y, acci[a][b]);
                 }
         }
         __syncthreads();
@@ -283,12 +286,12 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int64_t gc = tc + 32 * wc + 16 * b + (l & 15);
-            if (gc >= c_hi) continue;
+            const int64_t gr = tr + 32 * wr + 16 * a + (l & 15);
+            if (gr >= r_hi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = tr + 32 * wr + 16 * a + Cx<T>::row(r, l);
-                if (gr >= r_hi) continue;
+                const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
+                if (gc >= c_hi) continue;
                 C2* o = M + gr + gc * ld;
                 const C2 v = *o;
                 *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
@@ -367,15 +370,15 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accr[a][b] = Cx<T>::mfma(av[a].x, bv[b].x, accr[a][b]);
-                    acci[a][b] = Cx<T>::mfma(av[a].x, bv[b].y, acci[a][b]);
+                    accr[a][b] = Cx<T>::mfma(bv[b].x, av[a].x, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(bv[b].y, av[a].x, acci[a][b]);
                 }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accr[a][b] = Cx<T>::mfma(-av[a].y, bv[b].y, accr[a][b]);
-                    acci[a][b] = Cx<T>::mfma(av[a].y, bv[b].x, acci[a][b]);
+                    accr[a][b] = Cx<T>::mfma(-bv[b].y, av[a].y, accr[a][b]);
+                    acci[a][b] = Cx<T>::mfma(bv[b].x, av[a].y, acci[a][b]);
                 }
         }
         __syncthreads();
@@ -388,12 +391,13 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int64_t gc = tc + 32 * wc + 16 * b + (l & 15);
-            if (gc >= c_hi) continue;
+            // transposed blocks (operands swapped): lanes run along rows of C
+            const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
+            if (gr >= r_hi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = tr + 64 * wr + 16 * a + Cx<T>::row(r, l);
-                if (gr >= r_hi) continue;
+                const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
+                if (gc >= c_hi) continue;
                 C2* o = M + gr + gc * ld;
                 const C2 v = *o;
                 *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
