@@ -450,9 +450,11 @@ struct IrSystem {
 // sweeps typically reach 1e-11 and the results agree with the FP64 path to
 // ~1e-10, well inside the 1e-6 the mixed-precision mode is specified for.
 constexpr double kIrTol = 1e-10;
+// LUt32 (optional): space for the transposed factors; the first transposed
+// system fills it (on its stream) and solves with coalesced reads.
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
-                    int nsys) {
+                    int nsys, float2* LUt32) {
     size_t hcount = 0;
     for (int k = 0; k < nsys; ++k) hcount += (size_t)(batch * sys[k].nrhs);
     TRY(ensure_pinned(c, hcount));
@@ -461,6 +463,7 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
     for (int k = 0; k < nsys; ++k) {
         OnAux on(c, k == 1);
         const IrSystem& q = sys[k];
+        if (q.trans && LUt32) TRY(tri_transpose_f32(c, n, batch, LU32, n, stride32, LUt32));
         MH_HIP_CHECK(hipMemsetAsync(q.X, 0, sizeof(double2) * (size_t)(q.bstride * batch), c->stream));
         MH_HIP_CHECK(hipMemcpyAsync(q.R, q.Bm, sizeof(double2) * (size_t)(q.bstride * batch),
                                     hipMemcpyDeviceToDevice, c->stream));
@@ -474,7 +477,8 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
                 OnAux on(c, k == 1);
                 const int64_t s32 = n * q.nrhs;
                 TRY(convert_to_f32(c, n, q.nrhs, batch, q.R, q.ldb, q.bstride, q.R32, n, s32));
-                TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, q.trans, 3, q.nrhs, q.R32, n, s32));
+                TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, q.trans, 3, q.nrhs, q.R32, n, s32,
+                                  q.trans ? LUt32 : nullptr));
                 TRY(add_from_f32(c, n, q.nrhs, batch, q.R32, n, s32, q.X, q.ldb, q.bstride));
                 if (q.trans)
                     TRY(zgemm_tn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
@@ -552,6 +556,11 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     const bool mixed = precision == MH_MIXED;
     const int64_t ncol_ir = nr > ndof ? nr : ndof;  // columns of the refinement buffers
     size_t o_D32 = 0, o_y = 0, o_R = 0, o_R32 = 0, o_nrm = 0, o_R2 = 0, o_R32b = 0, o_nrm2 = 0;
+    // transposed factors for the adjoint solves (FP32 copy, or FP64 in the
+    // layout of the augmented matrices)
+    const size_t o_Dt = (ga && ndof) ? bump.take(mixed ? sizeof(float2) * (size_t)n * (size_t)n * chunk
+                                                       : mat_aug * chunk)
+                                     : 0;
     if (mixed) {
         o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * chunk);
         o_y = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
@@ -648,7 +657,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 }
                 nsys = 2;
             }
-            TRY(ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys));
+            TRY(ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
+                         nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr));
             if (nsys > 1) TRY(join_aux(c));
             if (!direct) TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
         } else if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
@@ -672,7 +682,9 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                     MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
                 else  // lambda = K^-T S^T c
                     TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
-                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa));
+                double2* dDt = reinterpret_cast<double2*>(ws + o_Dt);
+                TRY(tri_transpose(c, n, nw, dD, ld, stride, dDt));
+                TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa, dDt));
             }
             TRY(tri_solve(c, n, nw, dD, ld, stride, nullptr, inv, 0, 2, nr, aug, ld, stride));
             if (ga && ndof) TRY(join_aux(c));

@@ -31,12 +31,11 @@ struct Context {
     size_t ws_bytes = 0;
     int* d_flag = nullptr;  // device error / info flag (one int)
     // triangular solves, one state per stream that issues them (0: stream,
-    // 1: aux): [ticket u64 | pad | per-block flags u32], epoch of the last
-    // launch, out-of-place solution workspace; one shared error word
+    // 1: aux): the workgroup ticket (u64) and an out-of-place solution
+    // workspace; one shared error word
     struct TriSlot {
         void* sync = nullptr;
         size_t sync_bytes = 0;
-        unsigned epoch = 0;
         void* ws = nullptr;
         size_t ws_bytes = 0;
     } tri[2];
@@ -129,14 +128,21 @@ int tri_prepare(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64
                 const int64_t* ipiv, int* perm, double2* inv);
 int tri_prepare_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
                     const int64_t* ipiv, int* perm, float2* inv);
+// LUt = LU^T per item (same ld and stride), for transposed solves that read
+// the factors row-wise.
+int tri_transpose(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+                  double2* LUt);
+int tri_transpose_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                      float2* LUt);
 // Solves with the factors of A = P L U, in place on B (n x nrhs per item):
 // trans 0: A X = B, 1: A^T X = B; part 1 = L (with P), 2 = U, 3 = both.
+// LUt (optional): tri_transpose of LU, used by the transposed solves.
 int tri_solve(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
               const int* perm, const double2* inv, int trans, int part, int64_t nrhs, double2* B,
-              int64_t ldb, int64_t bstride);
+              int64_t ldb, int64_t bstride, const double2* LUt = nullptr);
 int tri_solve_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
                   const int* perm, const float2* inv, int trans, int part, int64_t nrhs, float2* B,
-                  int64_t ldb, int64_t bstride);
+                  int64_t ldb, int64_t bstride, const float2* LUt = nullptr);
 // Reports (and clears) a failed inter-workgroup wait or a bad pivot index of
 // the solves issued so far; synchronises the context stream.
 int tri_check(Context* ctx);

@@ -7,13 +7,13 @@
 // 256 rows, one workgroup per slab (one row per thread).  The unknowns are
 // produced in 32-row blocks in order.  A workgroup keeps its rows' partial
 // sums in registers; it applies the blocks of earlier slabs as their
-// solutions are published (a per-block flag written by the block's owner),
-// several ready blocks per round, with its rows' slice of the next block
-// loaded ahead of time.
+// solutions are published, several ready blocks per round, with its rows'
+// slice of the next block in flight while the current one is applied.
 // For its own blocks it multiplies the block's residual by the inverse of
-// the 32 x 32 diagonal block (precomputed once per factorisation, so the
-// dependency chain carries no sequential substitution), publishes the
-// solution and the block's flag, and updates its remaining rows.
+// the 32 x 32 diagonal block (precomputed once per factorisation and fetched
+// one block ahead, so the dependency chain carries no sequential
+// substitution and no memory latency), publishes the solution and updates
+// its remaining rows.
 //
 // The four triangles of A = P L U are one kernel in "logical" coordinates.
 // With np = n rounded up to a multiple of 32, unknown i of the logical
@@ -25,14 +25,20 @@
 //     U x = y                backward, U^-1 blocks
 //     U^T y = b              forward,  U^-1 blocks transposed
 //     L^T z = y, x = P^T z   backward, L^-1 blocks transposed, scatter
+// For the transposed solves the factors can also be given as an explicit
+// transposed copy, which turns the strided reads into coalesced ones.
 //
-// Synchronisation: solution entries and flags are written and read with
-// device-scope (agent) atomic accesses, which are coherent across the XCDs'
-// L2 caches; a producer waits for its solution stores to complete before it
-// stores the flag.  Workgroups take their (slab, matrix) from an atomic
-// ticket in the order they start, so every workgroup a running one waits for
-// has started before it.  Waits are bounded; on timeout an error word is set
-// and the kernel runs to completion (the host reports the error).
+// Publication: the output is filled with an all-ones bit pattern ("not yet
+// written") before the launch; a producer writes each solution entry with
+// 64-bit device-scope (agent) atomic stores, which are coherent across the
+// XCDs' L2 caches, and consumers poll the entries themselves, so a block is
+// usable as soon as its values land, with no flag and no wait for store
+// completion on the producer's side.  A computed value with the reserved
+// pattern (a NaN) is stored as the canonical NaN.  Workgroups take their
+// (slab, matrix) from an atomic ticket in the order they start, so every
+// workgroup a running one waits for has started before it.  Waits are
+// bounded; on timeout an error word is set and the kernel runs to
+// completion (the host reports the error).
 // Reference: the LAPACK triangular solves behind `\` / implicit_linear
 // (src/matrix_assembly.jl:108-126).
 #include "mh_cx.hpp"
@@ -46,55 +52,104 @@ constexpr int kTsNR = 4;      // right-hand sides per launch
 constexpr int kTsOwn = kTsRows / kTsCB;
 constexpr int kTsBatch = 8;   // ready blocks of earlier slabs applied per round
 constexpr unsigned kSpinLimit = 1u << 22;
+constexpr unsigned long long kUnset = ~0ull;  // "not yet written"
 
-// Thread 0 waits until *flag == epoch (or an error is flagged), then the
-// workgroup proceeds.
-__device__ __forceinline__ void wait_flag(const unsigned* flag, unsigned epoch, int* err) {
-    if (threadIdx.x == 0) {
-        unsigned spins = 0;
-        unsigned* f = const_cast<unsigned*>(flag);
-        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++spins > kSpinLimit) {
-                atomicOr(err, 1);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
+__device__ __forceinline__ unsigned long long ld_pub(const void* p) {
+    return __hip_atomic_load(reinterpret_cast<unsigned long long*>(const_cast<void*>(p)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_pub(void* p, unsigned long long b) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Loads a solution entry; true once it has been written.
+__device__ __forceinline__ bool pub_load(const float2* p, float2& v) {
+    const unsigned long long b = ld_pub(p);
+    v = make_float2(__uint_as_float((unsigned)b), __uint_as_float((unsigned)(b >> 32)));
+    return b != kUnset;
+}
+__device__ __forceinline__ bool pub_load(const double2* p, double2& v) {
+    const unsigned long long a = ld_pub(p), b = ld_pub(reinterpret_cast<const double*>(p) + 1);
+    v = make_double2(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+    return a != kUnset && b != kUnset;
+}
+__device__ __forceinline__ void pub_store(float2* p, float2 v) {
+    unsigned long long b =
+        (unsigned long long)__float_as_uint(v.x) | ((unsigned long long)__float_as_uint(v.y) << 32);
+    if (b == kUnset) b = 0x7fc000007fc00000ull;
+    st_pub(p, b);
+}
+__device__ __forceinline__ void pub_store(double2* p, double2 v) {
+    unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
+    unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
+    if (a == kUnset) a = 0x7ff8000000000000ull;
+    if (b == kUnset) b = 0x7ff8000000000000ull;
+    st_pub(p, a);
+    st_pub(reinterpret_cast<double*>(p) + 1, b);
+}
+
+// Raw buffer loads of one complex entry: a per-lane byte offset plus a
+// wave-uniform one (scalar register), so a row's 32 strided loads need no
+// 64-bit address registers.
+__device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float2) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, double2) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0));
+}
+
+// A row's slice of block c (logical columns 32c .. 32c+31) for rows after
+// the block, and its application with the block's solution x.  Every load
+// stays inside the matrix (offsets of rows/columns outside it are replaced
+// by 0 and the value by zero).
+template <bool TRM, bool REV, class C2>
+__device__ __forceinline__ void trsv_fetch(C2 (&m)[kTsCB], __amdgpu_buffer_rsrc_t rs, uint32_t vo, uint32_t es,
+                                           int np, int n, int nsubp, int i, bool valid, int c) {
+    const bool need = valid && c < nsubp && i >= (c + 1) * kTsCB;
+    // physical row/column of entry q: p0 + q (forward) or p0 - q (backward);
+    // entries with p >= n are padding
+    const int p0 = REV ? np - 1 - c * kTsCB : c * kTsCB;
+#pragma unroll
+    for (int q = 0; q < kTsCB; ++q) {
+        const int pj = REV ? p0 - q : p0 + q;
+        const bool in = pj < n;
+        const uint32_t so = (uint32_t)(in ? pj : 0) * es;
+        const C2 val = buf_load(rs, vo, so, C2());
+        m[q] = (need && in) ? val : mkc<C2>(0, 0);
     }
-    __syncthreads();
+}
+template <int NR, class C2>
+__device__ __forceinline__ void trsv_apply(const C2 (&m)[kTsCB], C2 (&v)[NR], int nrhs, int i, bool valid, int c,
+                                           const C2 (*x)[NR]) {
+    if (valid && i >= (c + 1) * kTsCB) {
+#pragma unroll
+        for (int q = 0; q < kTsCB; ++q)
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) v[r] = cfms(v[r], m[q], x[q][r]);
+    }
 }
 
-template <class C2>
-__device__ __forceinline__ C2 load_published(const C2* p) {
-    typedef decltype(C2().x) T;
-    T* q = reinterpret_cast<T*>(const_cast<C2*>(p));
-    return mkc<C2>(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                   __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-template <class C2>
-__device__ __forceinline__ void store_published(C2* p, C2 v) {
-    typedef decltype(C2().x) T;
-    T* q = reinterpret_cast<T*>(p);
-    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <class T, bool TR, bool REV, int NR>
-__global__ __launch_bounds__(kTsRows) void k_trsv(
+// TR: transposed solve (inverse blocks read transposed); TRM: the matrix is
+// read transposed (false when A is an explicit transposed copy).
+template <class T, bool TR, bool TRM, bool REV, int NR>
+__global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) void k_trsv(
     int64_t n, int64_t batch, const typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
     const typename Cx<T>::t* __restrict__ Inv, const int* __restrict__ gperm,
     const int* __restrict__ sperm, int nrhs, const typename Cx<T>::t* in, int64_t ldi, int64_t istride,
-    typename Cx<T>::t* out, int64_t ldo, int64_t ostride, int nslab, unsigned* __restrict__ flags,
-    int nsubp, unsigned epoch, unsigned long long* __restrict__ ticket, int* __restrict__ err) {
+    typename Cx<T>::t* out, int64_t ldo, int64_t ostride, int nslab, int nsubp,
+    unsigned long long* __restrict__ ticket, int* __restrict__ err) {
     typedef typename Cx<T>::t C2;
     __shared__ unsigned long long s_tk;
-    __shared__ int s_ok[kTsBatch];
+    __shared__ unsigned s_bad[2];           // per round (parity): blocks c+b not yet complete
     __shared__ C2 sI[2][kTsCB][kTsCB + 1];  // inverse diagonal blocks, double-buffered
     __shared__ C2 sx[kTsBatch][kTsCB][NR];  // solutions of the blocks being applied
     __shared__ C2 sr[kTsCB][NR];            // residual of the current own block
-    const int tid = threadIdx.x, wave = tid >> 6;
-    if (tid == 0) s_tk = atomicAdd(ticket, 1ull);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) {
+        s_tk = atomicAdd(ticket, 1ull);
+        s_bad[0] = 0u;
+        s_bad[1] = 0u;
+    }
     __syncthreads();
     const int64_t tk = (int64_t)s_tk;
     if (tk >= (int64_t)nslab * batch) return;  // cannot happen with a reset ticket
@@ -102,30 +157,48 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
     const int64_t item = tk / nslab;
     const int64_t np = (int64_t)nsubp * kTsCB;
     const C2* M = A + item * stride;
+    // extent (n-1) * (ld+1) + 1 entries; the host checks it fits 31 bits
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<C2*>(M), (short)0, (int)(((n - 1) * (ld + 1) + 1) * (int64_t)sizeof(C2)), 0x00020000);
     const int* gp = gperm ? gperm + item * n : nullptr;
     const int* sp = sperm ? sperm + item * n : nullptr;
     const C2* bin = in + item * istride;
     C2* bout = out + item * ostride;
-    unsigned* fl = flags + item * nsubp;
     const C2 zero = mkc<C2>(T(0), T(0));
     auto phys = [&](int64_t i) -> int64_t { return REV ? np - 1 - i : i; };
+    auto dst_of = [&](int64_t p) -> int64_t { return sp ? (int64_t)sp[p] : p; };
 
     const int64_t i0 = (int64_t)slab * kTsRows;
     const int64_t i = i0 + tid;
     const int64_t pi = phys(i);
     const bool valid = i < np && pi < n;
+    // byte offsets into the matrix: per lane (row pi, or column pi of the
+    // transposed read) and per entry of a slice (scaled by es)
+    const uint32_t vo = valid ? (uint32_t)(TRM ? pi * ld : pi) * (uint32_t)sizeof(C2) : 0u;
+    const uint32_t es = (uint32_t)(TRM ? 1 : ld) * (uint32_t)sizeof(C2);
     const int cown = (int)(i0 / kTsCB);
     const int cend = (cown + kTsOwn) < nsubp ? cown + kTsOwn : nsubp;
 
     // inverse of own block s (physical block K, element (a, b) at
-    // Inv[((item * nsubp + K) * 32 + b) * 32 + a]) into buffer s & 1
-    auto load_inv = [&](int s) {
-        const int c = cown + s;
-        const int64_t K = REV ? nsubp - 1 - c : c;
+    // Inv[((item * nsubp + K) * 32 + b) * 32 + a]): fetched into registers
+    // one block ahead, staged into LDS buffer s & 1 when its turn comes
+    constexpr int kIE = kTsCB * kTsCB / kTsRows;
+    C2 ib[kIE];
+    auto fetch_inv = [&](int s) {
+        const int cc = cown + s;
+        const int64_t K = REV ? nsubp - 1 - cc : cc;
         const C2* src = Inv + (item * nsubp + K) * kTsCB * kTsCB;
-        for (int e = tid; e < kTsCB * kTsCB; e += kTsRows) sI[s & 1][e % kTsCB][e / kTsCB] = src[e];
+#pragma unroll
+        for (int e = 0; e < kIE; ++e) ib[e] = src[tid + kTsRows * e];
     };
-    if (cend > cown) load_inv(0);
+    auto stash_inv = [&](int s) {
+#pragma unroll
+        for (int e = 0; e < kIE; ++e) {
+            const int x = tid + kTsRows * e;
+            sI[s & 1][x % kTsCB][x / kTsCB] = ib[e];
+        }
+    };
+    if (cend > cown) fetch_inv(0);
 
     C2 v[NR];
     {
@@ -134,57 +207,67 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
         for (int r = 0; r < NR; ++r) v[r] = (valid && r < nrhs) ? bin[src + r * ldi] : zero;
     }
 
-    // this row's slice of block c (logical columns 32c .. 32c+31), for rows
-    // after the block, and its application with the block's solution x
-    C2 m[kTsCB];
-    auto prefetch = [&](int c) {
-        const bool need = valid && i >= (int64_t)(c + 1) * kTsCB;
-#pragma unroll
-        for (int q = 0; q < kTsCB; ++q) {
-            const int64_t pj = phys((int64_t)c * kTsCB + q);
-            m[q] = (need && pj < n) ? (TR ? M[pj + pi * ld] : M[pi + pj * ld]) : zero;
-        }
-    };
-    auto apply = [&](int c, C2 (*x)[NR]) {
-        if (valid && i >= (int64_t)(c + 1) * kTsCB) {
-#pragma unroll
-            for (int q = 0; q < kTsCB; ++q)
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    if (r < nrhs) v[r] = cfms(v[r], m[q], x[q][r]);
-        }
-    };
-    prefetch(0);
+    // the row's slices are consumed in block order 0, 1, ..., cend-1: apply
+    // block c, then fetch block c + 1's slice
+    C2 ma[kTsCB];
+#define MH_TRSV_STEP(C, X)                                                                      \
+    do {                                                                                        \
+        const int c_ = (C);                                                                     \
+        trsv_apply<NR>(ma, v, nrhs, (int)i, valid, c_, (X));                                    \
+        trsv_fetch<TRM, REV>(ma, rs, vo, es, (int)np, (int)n, nsubp, (int)i, valid, c_ + 1);    \
+    } while (0)
+    trsv_fetch<TRM, REV>(ma, rs, vo, es, (int)np, (int)n, nsubp, (int)i, valid, 0);
 
-    // 1. blocks of earlier slabs, applied in rounds of up to kTsBatch blocks
-    //    that are ready
-    int c = 0;
+    // 1. blocks of earlier slabs, in rounds: wave 0 waits for block c while
+    //    the other waves take the entries of blocks c+1 .. c+7 that are
+    //    already written; the round applies block c and the complete ones
+    //    that follow it
+    int c = 0, par = 0;
     while (c < cown) {
-        wait_flag(fl + c, epoch, err);
-        if (tid < kTsBatch) {
-            bool ok = tid == 0;
-            if (tid > 0 && c + tid < cown)
-                ok = __hip_atomic_load(fl + c + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
-            s_ok[tid] = ok ? 1 : 0;
+        if (tid == 0) s_bad[par ^ 1] = 0u;
+        if (wave == 0) {
+            unsigned spins = 0;
+            for (;;) {
+                bool ok = true;
+#pragma unroll 1
+                for (int e = lane; e < kTsCB * NR; e += 64) {
+                    const int q = e % kTsCB, r = e / kTsCB;
+                    const int64_t pj = phys((int64_t)c * kTsCB + q);
+                    C2 val = zero;
+                    if (r < nrhs && pj < n) ok = pub_load(bout + dst_of(pj) + r * ldo, val) && ok;
+                    sx[0][q][r] = val;
+                }
+                if (__all(ok)) break;
+                const int bad = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__any(bad != 0)) break;
+                if (++spins > kSpinLimit) {
+                    if (lane == 0) atomicOr(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        } else {
+            unsigned bad = 0u;
+#pragma unroll 1
+            for (int e = tid - 64; e < (kTsBatch - 1) * kTsCB * NR; e += kTsRows - 64) {
+                const int b = 1 + e / (kTsCB * NR), q = e % kTsCB, r = (e / kTsCB) % NR;
+                if (c + b >= cown) {
+                    bad |= 1u << b;
+                    continue;
+                }
+                const int64_t pj = phys((int64_t)(c + b) * kTsCB + q);
+                C2 val = zero;
+                if (r < nrhs && pj < n && !pub_load(bout + dst_of(pj) + r * ldo, val)) bad |= 1u << b;
+                sx[b][q][r] = val;
+            }
+            if (bad) atomicOr(&s_bad[par], bad);
         }
         __syncthreads();
-        int nb = 1;
-        while (nb < kTsBatch && s_ok[nb]) ++nb;
-        for (int e = tid; e < nb * kTsCB * NR; e += kTsRows) {
-            const int b = e / (kTsCB * NR), q = e % kTsCB, r = (e / kTsCB) % NR;
-            const int64_t pj = phys((int64_t)(c + b) * kTsCB + q);
-            sx[b][q][r] = (r < nrhs && pj < n) ? load_published(bout + (sp ? (int64_t)sp[pj] : pj) + r * ldo)
-                                               : zero;
-        }
-        __syncthreads();
-        apply(c, sx[0]);
-        for (int b = 1; b < nb; ++b) {
-            prefetch(c + b);
-            apply(c + b, sx[b]);
-        }
+        const int nb = __builtin_ctz(s_bad[par] | (1u << kTsBatch));
+        for (int b = 0; b < nb; ++b) MH_TRSV_STEP(c + b, sx[b]);
         c += nb;
+        par ^= 1;
         __syncthreads();
-        prefetch(c);
     }
 
     // 2. this slab's own blocks: x = T_cc^-1 r, published, then applied to
@@ -196,7 +279,8 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
 #pragma unroll
             for (int r = 0; r < NR; ++r) sr[tid - base][r] = v[r];
         }
-        if (s + 1 < cend - cown) load_inv(s + 1);
+        stash_inv(s);
+        if (s + 1 < cend - cown) fetch_inv(s + 1);
         __syncthreads();
         if (mine) {
             const int a = tid - base;
@@ -215,21 +299,31 @@ __global__ __launch_bounds__(kTsRows) void k_trsv(
                     acc[r].y += w.x * y.y + w.y * y.x;
                 }
             }
-            const int64_t dst = sp ? (valid ? (int64_t)sp[pi] : 0) : pi;
+            const int64_t dst = valid ? dst_of(pi) : 0;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 v[r] = acc[r];
                 sx[0][a][r] = acc[r];
-                if (valid && r < nrhs) store_published(bout + dst + r * ldo, acc[r]);
+                if (valid && r < nrhs) pub_store(bout + dst + r * ldo, acc[r]);
             }
         }
-        if (wave == (base >> 6)) __builtin_amdgcn_s_waitcnt(0);  // solution stores complete
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(fl + cb, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        apply(cb, sx[0]);
-        __syncthreads();
-        if (s + 1 < cend - cown) prefetch(cb + 1);
+        MH_TRSV_STEP(cb, sx[0]);
     }
+#undef MH_TRSV_STEP
+}
+
+// Marks the n x cols x batch output of a solve as not yet written.
+template <class C2>
+__global__ void k_fill_unset(int64_t rows, int64_t cols, int64_t batch, C2* __restrict__ out, int64_t ldo,
+                             int64_t ostride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = rows * cols * batch;
+    if (t >= tot) return;
+    const int64_t r = t % rows, c = (t / rows) % cols, b = t / (rows * cols);
+    unsigned long long* p = reinterpret_cast<unsigned long long*>(out + b * ostride + c * ldo + r);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(C2) / 8); ++w) p[w] = kUnset;
 }
 
 // Inverses of the 32 x 32 diagonal blocks of the unit-lower L and of U
@@ -317,11 +411,11 @@ __global__ void k_copy_cols(int64_t rows, int64_t cols, int64_t batch, const C2*
     dst[b * dstride + c * ldd + r] = src[b * sstride + c * lds + r];
 }
 
-// per-context synchronisation state: [ticket u64 | pad | flags u32 ...], the
-// error word and a solution workspace for the out-of-place steps
-static int tri_state(Context* c, int64_t batch, int64_t nsub, size_t ws_bytes) {
+// per-context synchronisation state: the workgroup ticket, the error word
+// and a solution workspace for the out-of-place steps
+static int tri_state(Context* c, size_t ws_bytes) {
     Context::TriSlot& t = c->tri[c->tri_slot];
-    const size_t need = 256 + sizeof(unsigned) * (size_t)(batch * nsub);
+    const size_t need = 256;
     if (need > t.sync_bytes) {
         if (t.sync) {
             MH_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -361,7 +455,7 @@ static int tri_prepare_t(Context* c, int64_t n, int64_t batch, const typename Cx
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
     MH_REQUIRE(batch <= 65535, "lu_solve: batch too large");
-    TRY_RC(tri_state(c, 1, 1, 0));
+    TRY_RC(tri_state(c, 0));
     if (perm) {
         hipLaunchKernelGGL(k_perm_from_ipiv, dim3((unsigned)batch), dim3(256), 0, c->stream, n, ipiv, perm,
                            c->tri_err);
@@ -384,22 +478,27 @@ int tri_prepare_f32(Context* c, int64_t n, int64_t batch, const float2* LU, int6
     return tri_prepare_t<float>(c, n, batch, LU, ld, stride, ipiv, perm, inv);
 }
 
-template <class T, bool TR, bool REV>
+// One triangle (see the kernel); A is the factor matrix, or its explicit
+// transposed copy when TRM is false for a transposed solve.  `out` must not
+// overlap `in`.
+template <class T, bool TR, bool TRM, bool REV>
 static int launch_trsv(Context* c, int64_t n, int64_t batch, const typename Cx<T>::t* A, int64_t ld,
                        int64_t stride, const typename Cx<T>::t* inv, const int* gperm, const int* sperm,
                        int nrhs, const typename Cx<T>::t* in, int64_t ldi, int64_t istride,
                        typename Cx<T>::t* out, int64_t ldo, int64_t ostride) {
+    typedef typename Cx<T>::t C2;
     const int nsubp = (int)((n + kTsCB - 1) / kTsCB);
     const int nslab = (int)((nsubp * (int64_t)kTsCB + kTsRows - 1) / kTsRows);
     Context::TriSlot& ts = c->tri[c->tri_slot];
     unsigned long long* ticket = reinterpret_cast<unsigned long long*>(ts.sync);
-    unsigned* flags = reinterpret_cast<unsigned*>(static_cast<char*>(ts.sync) + 256);
-    if (++ts.epoch == 0) ts.epoch = 1;
     hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, c->stream, ticket);
-#define MH_TRSV(NRR)                                                                                         \
-    hipLaunchKernelGGL((k_trsv<T, TR, REV, NRR>), dim3((unsigned)(nslab * batch)), dim3(kTsRows), 0, c->stream, \
-                       n, batch, A, ld, stride, inv, gperm, sperm, nrhs, in, ldi, istride, out, ldo, ostride,    \
-                       nslab, flags, nsubp, ts.epoch, ticket, c->tri_err)
+    const int64_t tot = n * nrhs * batch;
+    hipLaunchKernelGGL(k_fill_unset<C2>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, n,
+                       (int64_t)nrhs, batch, out, ldo, ostride);
+#define MH_TRSV(NRR)                                                                                              \
+    hipLaunchKernelGGL((k_trsv<T, TR, TRM, REV, NRR>), dim3((unsigned)(nslab * batch)), dim3(kTsRows), 0,         \
+                       c->stream, n, batch, A, ld, stride, inv, gperm, sperm, nrhs, in, ldi, istride, out, ldo,   \
+                       ostride, nslab, nsubp, ticket, c->tri_err)
     if (nrhs <= 1)
         MH_TRSV(1);
     else if (nrhs <= 2)
@@ -411,10 +510,67 @@ static int launch_trsv(Context* c, int64_t n, int64_t batch, const typename Cx<T
     return MH_OK;
 }
 
+// 64 x 64 tiles through LDS: reads and writes both run down columns.
+template <class C2>
+__global__ __launch_bounds__(256) void k_transpose(int64_t n, const C2* __restrict__ A, int64_t ld,
+                                                   int64_t stride, C2* __restrict__ At) {
+    __shared__ C2 t[64][65];
+    const int64_t item = blockIdx.z;
+    const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+    const C2* src = A + item * stride;
+    C2* dst = At + item * stride;
+    const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
+    for (int cc = y; cc < 64; cc += 4)
+        if (r0 + x < n && c0 + cc < n) t[cc][x] = src[(r0 + x) + (c0 + cc) * ld];
+    __syncthreads();
+    // At[c0 + x, r0 + rr] = A[r0 + rr, c0 + x]
+    for (int rr = y; rr < 64; rr += 4)
+        if (c0 + x < n && r0 + rr < n) dst[(c0 + x) + (r0 + rr) * ld] = t[x][rr];
+}
+
+template <class T>
+static int tri_transpose_t(Context* c, int64_t n, int64_t batch, const typename Cx<T>::t* LU, int64_t ld,
+                           int64_t stride, typename Cx<T>::t* LUt) {
+    if (n == 0 || batch == 0) return MH_OK;
+    MH_REQUIRE(batch <= 65535, "lu_solve: batch too large");
+    const unsigned t = (unsigned)((n + 63) / 64);
+    ProfScope ps(c, "transpose", 0.0, 2.0 * sizeof(typename Cx<T>::t) * (double)n * n * batch);
+    hipLaunchKernelGGL(k_transpose<typename Cx<T>::t>, dim3(t, t, (unsigned)batch), dim3(256), 0, c->stream, n,
+                       LU, ld, stride, LUt);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+int tri_transpose(Context* c, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+                  double2* LUt) {
+    return tri_transpose_t<double>(c, n, batch, LU, ld, stride, LUt);
+}
+int tri_transpose_f32(Context* c, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                      float2* LUt) {
+    return tri_transpose_t<float>(c, n, batch, LU, ld, stride, LUt);
+}
+
+// One triangle: as stored, or transposed (from the explicit copy LUt when
+// given, else by strided reads of LU).
+template <class T, bool REV>
+static int one_tri(Context* c, bool trans, int64_t n, int64_t batch, const typename Cx<T>::t* LU,
+                   const typename Cx<T>::t* LUt, int64_t ld, int64_t stride, const typename Cx<T>::t* inv,
+                   const int* gperm, const int* sperm, int nr, const typename Cx<T>::t* in, int64_t ldi,
+                   int64_t istride, typename Cx<T>::t* out, int64_t ldo, int64_t ostride) {
+    if (!trans)
+        return launch_trsv<T, false, false, REV>(c, n, batch, LU, ld, stride, inv, gperm, sperm, nr, in, ldi,
+                                                 istride, out, ldo, ostride);
+    if (LUt)
+        return launch_trsv<T, true, false, REV>(c, n, batch, LUt, ld, stride, inv, gperm, sperm, nr, in, ldi,
+                                                istride, out, ldo, ostride);
+    return launch_trsv<T, true, true, REV>(c, n, batch, LU, ld, stride, inv, gperm, sperm, nr, in, ldi, istride,
+                                           out, ldo, ostride);
+}
+
 template <class T>
 static int tri_solve_t(Context* c, int64_t n, int64_t batch, const typename Cx<T>::t* LU, int64_t ld,
                        int64_t stride, const int* perm, const typename Cx<T>::t* inv, int trans, int part,
-                       int64_t nrhs, typename Cx<T>::t* B, int64_t ldb, int64_t bstride) {
+                       int64_t nrhs, typename Cx<T>::t* B, int64_t ldb, int64_t bstride,
+                       const typename Cx<T>::t* LUt) {
     typedef typename Cx<T>::t C2;
     if (n == 0 || batch == 0 || nrhs == 0 || part == 0) return MH_OK;
     MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
@@ -423,7 +579,9 @@ static int tri_solve_t(Context* c, int64_t n, int64_t batch, const typename Cx<T
     MH_REQUIRE(nslab * batch <= 0x7fffffff, "lu_solve: batch too large");
     MH_REQUIRE((part & 1) == 0 || perm, "lu_solve: permutation missing");
     MH_REQUIRE(inv, "lu_solve: inverse diagonal blocks missing");
-    TRY_RC(tri_state(c, batch, nsubp, sizeof(C2) * (size_t)(n * kTsNR * batch)));
+    MH_REQUIRE(((n - 1) * (ld + 1) + 1) * (int64_t)sizeof(C2) <= 0x7fffffff,
+               "lu_solve: matrix extent (leading dimension x n) above 2 GiB");
+    TRY_RC(tri_state(c, sizeof(C2) * (size_t)(n * kTsNR * batch)));
     C2* W = static_cast<C2*>(c->tri[c->tri_slot].ws);
     const int64_t ws = n * kTsNR;
     const C2* invL = inv;
@@ -431,43 +589,35 @@ static int tri_solve_t(Context* c, int64_t n, int64_t batch, const typename Cx<T
     const double tri = ((part & 1) ? 0.5 : 0.0) + ((part & 2) ? 0.5 : 0.0);
     ProfScope ps(c, sizeof(T) == 8 ? "lu_solve" : "lu_solve_f32", 8.0 * tri * (double)n * n * nrhs * batch,
                  sizeof(C2) * tri * (double)n * n * batch * (double)((nrhs + kTsNR - 1) / kTsNR));
+    // first triangle: L (forward, gather through perm) or U^T; second: U or
+    // L^T (scatter through perm); each solve writes a buffer it does not read
+    const bool tr = trans != 0;
+    const bool do_first = tr ? (part & 2) != 0 : (part & 1) != 0;
+    const bool do_second = tr ? (part & 1) != 0 : (part & 2) != 0;
+    const C2* inv_first = tr ? invU : invL;
+    const C2* inv_second = tr ? invL : invU;
+    const int* gp = tr ? nullptr : perm;
+    const int* sp = tr ? perm : nullptr;
     for (int64_t r0 = 0; r0 < nrhs; r0 += kTsNR) {
         const int nr = (int)((nrhs - r0) < kTsNR ? (nrhs - r0) : kTsNR);
         C2* b = B + r0 * ldb;
         const int64_t tot = n * nr * batch;
         const unsigned cgrid = (unsigned)((tot + 255) / 256);
-        if (!trans) {
-            if (part == 3) {
-                TRY_RC((launch_trsv<T, false, false>(c, n, batch, LU, ld, stride, invL, perm, nullptr, nr, b, ldb,
-                                                     bstride, W, n, ws)));
-                TRY_RC((launch_trsv<T, false, true>(c, n, batch, LU, ld, stride, invU, nullptr, nullptr, nr, W, n,
-                                                    ws, b, ldb, bstride)));
-            } else if (part == 2) {
-                TRY_RC((launch_trsv<T, false, true>(c, n, batch, LU, ld, stride, invU, nullptr, nullptr, nr, b,
-                                                    ldb, bstride, b, ldb, bstride)));
-            } else {
-                TRY_RC((launch_trsv<T, false, false>(c, n, batch, LU, ld, stride, invL, perm, nullptr, nr, b, ldb,
-                                                     bstride, W, n, ws)));
-                hipLaunchKernelGGL(k_copy_cols<C2>, dim3(cgrid), dim3(256), 0, c->stream, n, (int64_t)nr, batch,
-                                   (const C2*)W, n, ws, b, ldb, bstride);
-                MH_HIP_CHECK(hipGetLastError());
-            }
-        } else {
-            if (part == 3) {
-                TRY_RC((launch_trsv<T, true, false>(c, n, batch, LU, ld, stride, invU, nullptr, nullptr, nr, b,
-                                                    ldb, bstride, W, n, ws)));
-                TRY_RC((launch_trsv<T, true, true>(c, n, batch, LU, ld, stride, invL, nullptr, perm, nr, W, n, ws,
-                                                   b, ldb, bstride)));
-            } else if (part == 2) {
-                TRY_RC((launch_trsv<T, true, false>(c, n, batch, LU, ld, stride, invU, nullptr, nullptr, nr, b,
-                                                    ldb, bstride, b, ldb, bstride)));
-            } else {
-                hipLaunchKernelGGL(k_copy_cols<C2>, dim3(cgrid), dim3(256), 0, c->stream, n, (int64_t)nr, batch,
-                                   (const C2*)b, ldb, bstride, W, n, ws);
-                MH_HIP_CHECK(hipGetLastError());
-                TRY_RC((launch_trsv<T, true, true>(c, n, batch, LU, ld, stride, invL, nullptr, perm, nr, W, n, ws,
-                                                   b, ldb, bstride)));
-            }
+        if (do_first)
+            TRY_RC((one_tri<T, false>(c, tr, n, batch, LU, LUt, ld, stride, inv_first, gp, nullptr, nr, b, ldb,
+                                      bstride, W, n, ws)));
+        else {
+            hipLaunchKernelGGL(k_copy_cols<C2>, dim3(cgrid), dim3(256), 0, c->stream, n, (int64_t)nr, batch,
+                               (const C2*)b, ldb, bstride, W, n, ws);
+            MH_HIP_CHECK(hipGetLastError());
+        }
+        if (do_second)
+            TRY_RC((one_tri<T, true>(c, tr, n, batch, LU, LUt, ld, stride, inv_second, nullptr, sp, nr, W, n, ws,
+                                     b, ldb, bstride)));
+        else {
+            hipLaunchKernelGGL(k_copy_cols<C2>, dim3(cgrid), dim3(256), 0, c->stream, n, (int64_t)nr, batch,
+                               (const C2*)W, n, ws, b, ldb, bstride);
+            MH_HIP_CHECK(hipGetLastError());
         }
     }
     return MH_OK;
@@ -475,14 +625,14 @@ static int tri_solve_t(Context* c, int64_t n, int64_t batch, const typename Cx<T
 
 int tri_solve(Context* c, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
               const int* perm, const double2* inv, int trans, int part, int64_t nrhs, double2* B, int64_t ldb,
-              int64_t bstride) {
-    return tri_solve_t<double>(c, n, batch, LU, ld, stride, perm, inv, trans, part, nrhs, B, ldb, bstride);
+              int64_t bstride, const double2* LUt) {
+    return tri_solve_t<double>(c, n, batch, LU, ld, stride, perm, inv, trans, part, nrhs, B, ldb, bstride, LUt);
 }
 
 int tri_solve_f32(Context* c, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
                   const int* perm, const float2* inv, int trans, int part, int64_t nrhs, float2* B, int64_t ldb,
-                  int64_t bstride) {
-    return tri_solve_t<float>(c, n, batch, LU, ld, stride, perm, inv, trans, part, nrhs, B, ldb, bstride);
+                  int64_t bstride, const float2* LUt) {
+    return tri_solve_t<float>(c, n, batch, LU, ld, stride, perm, inv, trans, part, nrhs, B, ldb, bstride, LUt);
 }
 
 // Reads and clears the inter-workgroup error word (synchronises the stream).
