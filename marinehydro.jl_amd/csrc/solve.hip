@@ -24,14 +24,19 @@ constexpr int kNBOuter = 128;  // block-column width of the trailing updates
 constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
 // ---------------------------------------------------------------------------
-// leaf panel: columns [k0, k0+w), rows [k0, n), w <= 8, held in registers.
-// Thread t owns local rows t + 1024 q, q < R.
+// leaf panel: columns [k0, k0+w), rows [k0, n), w <= kLeaf, held in
+// registers.  Thread t owns local rows t + 1024 q, q < R.  The leaf's row
+// interchanges are then applied to the other columns of the enclosing block
+// column [pc_lo, pc_hi) (at most kNBOuter wide), which replaces the separate
+// swap passes of the recursion: the left columns hold final L values whose
+// rows are only re-ordered, the right columns are not updated before the
+// interchanges of every earlier leaf have reached them.
 // ---------------------------------------------------------------------------
 template <class T, int R, int kLeaf>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
                                                   int64_t stride, int64_t k0, int w,
                                                   int64_t* __restrict__ ipiv,
-                                                  int* __restrict__ info) {
+                                                  int* __restrict__ info, int64_t pc_lo, int64_t pc_hi) {
     typedef typename Cx<T>::t C2;
     C2* M = A + (int64_t)blockIdx.x * stride;
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
@@ -41,6 +46,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
     __shared__ T s_v[2][16];
     __shared__ int s_i[2][16];
     __shared__ C2 s_crow[2][kLeaf], s_prow[2][kLeaf];
+    __shared__ int s_p[kLeaf];                    // local pivot rows
+    __shared__ int s_rows[2 * kLeaf], s_src[2 * kLeaf];
     C2 a[R][kLeaf];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -95,6 +102,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
         if (tid == 0) {
             if (zero) atomicOr(info + blockIdx.x, 1);
             piv[k0 + c] = k0 + p;
+            s_p[c] = p;
         }
         if (tid == c) {
 #pragma unroll
@@ -155,6 +163,67 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
 #pragma unroll
         for (int c = 0; c < kLeaf; ++c)
             if (r < m && c < w) M[(k0 + r) + (k0 + c) * ld] = a[q][c];
+    }
+    // interchanges on the block column's other columns: the net permutation
+    // of the rows they touch (value of row s_rows[s] <- row s_src[s]; slot
+    // j < w is row j, slot w + j the pivot row of column j when it lies below
+    // the leaf and first appears there, else unused = -1), then every
+    // (column, row) gathered before any is written
+    const int nleft = (int)(k0 - pc_lo), nright = (int)(pc_hi - (k0 + w));
+    if (nleft + nright <= 0) return;
+    if (wave == 0) {
+        // lane j < w: slot of column j's pivot row; lanes 0..2w-1: slot rows
+        const int pj = lane < w ? s_p[lane] : 0;
+        int first = lane;
+        if (lane < w && pj >= w)
+            for (int i = 0; i < lane; ++i)
+                if (s_p[i] == pj) {
+                    first = i;
+                    break;
+                }
+        const int bslot = lane < w ? (pj < w ? pj : w + first) : 0;
+        const int jw = (lane >= w && lane < 2 * w) ? lane - w : 0;
+        const int pr = __shfl(pj, jw);  // every lane takes part in the shuffles
+        const int fr = __shfl(first, jw);
+        const int row = lane < w ? lane : ((lane < 2 * w && pr >= w && fr == jw) ? pr : -1);
+        // labels: the original row whose value each slot holds, interchanged in order
+        int lab = row;
+        for (int j = 0; j < w; ++j) {
+            const int b = __shfl(bslot, j);
+            const int va = __shfl(lab, j), vb = __shfl(lab, b);
+            if (b != j) lab = lane == j ? vb : (lane == b ? va : lab);
+        }
+        if (lane < 2 * kLeaf) {
+            s_rows[lane] = lane < 2 * w ? row : -1;
+            s_src[lane] = lane < 2 * w ? lab : -1;
+        }
+    }
+    __syncthreads();
+    const int nr = 2 * w;
+    const int items = (nleft + nright) * nr;  // <= kNBOuter * 2 * kLeaf
+    constexpr int kItems = (kNBOuter * 2 * kLeaf + 1023) / 1024;
+    C2 g[kItems];
+#pragma unroll
+    for (int e = 0; e < kItems; ++e) {
+        const int it = tid + 1024 * e;
+        if (it < items) {
+            const int ci = it / nr, si = it % nr;
+            const int64_t col = ci < nleft ? pc_lo + ci : k0 + w + (ci - nleft);
+            const int sr = s_src[si];
+            if (sr >= 0) g[e] = M[(k0 + sr) + col * ld];
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // every gather has returned before any row is overwritten
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kItems; ++e) {
+        const int it = tid + 1024 * e;
+        if (it < items) {
+            const int ci = it / nr, si = it % nr;
+            const int64_t col = ci < nleft ? pc_lo + ci : k0 + w + (ci - nleft);
+            const int rr = s_rows[si];
+            if (rr >= 0) M[(k0 + rr) + col * ld] = g[e];
+        }
     }
 }
 
@@ -416,6 +485,7 @@ struct LuPlan {
     int64_t n, ncols, ld, stride, batch;
     int64_t* ipiv;
     int* info;
+    int64_t pc_lo, pc_hi;  // block column being factored
 };
 
 template <class T>
@@ -481,7 +551,7 @@ static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
     const int W = leaf_width<T>(m);
 #define LEAF(RR, WW)                                                                                        \
     hipLaunchKernelGGL((k_lu_leaf<T, RR, WW>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A, \
-                       P.ld, P.stride, k0, w, P.ipiv, P.info)
+                       P.ld, P.stride, k0, w, P.ipiv, P.info, P.pc_lo, P.pc_hi)
     (void)W;
     if (m <= 1024) {
         LEAF(1, 16);
@@ -497,18 +567,17 @@ static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
 
 // recursive factorisation of the panel [k0, k0+w) (rows k0..n), touching only
 // the panel's columns (zgetrf2 order)
+// (each leaf applies its interchanges to the rest of the block column)
 template <class T>
 static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
-    const int L = leaf_width<T>(P.n - k0);
+    const int L = leaf_width<T>(P.n - P.pc_lo);
     if (w <= L) return lu_leaf(P, k0, w);
     int h = ((w / 2) + L - 1) / L * L;
     if (h >= w) h = w - L;
     TRY_RC(lu_panel(P, k0, h));
-    TRY_RC(lu_swap(P, k0, h, k0 + h, k0 + w));
     TRY_RC(lu_trsm(P, k0, h, k0 + h, k0 + w));
     TRY_RC(lu_gemm(P, k0 + h, P.n, k0 + h, k0 + w, k0, h));
     TRY_RC(lu_panel(P, k0 + h, w - h));
-    TRY_RC(lu_swap(P, k0 + h, w - h, k0, k0 + h));
     return MH_OK;
 }
 
@@ -529,7 +598,7 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
-    LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info};
+    LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info, 0, 0};
     const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
     TRY_RC(ensure_aux(ctx));
     TRY_RC(ensure_events(ctx, (size_t)(2 * npan + 1)));
@@ -547,6 +616,8 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         const int wn = (int)((n - kn0) < kNBOuter ? (n - kn0) : kNBOuter);
         const int64_t cw0 = kn0 + wn;  // first column of the wide update
         if (k >= 2) MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[k - 2], 0));
+        P.pc_lo = k0;
+        P.pc_hi = k0 + w;
         rc = lu_panel(P, k0, w);
         if (rc != MH_OK) break;
         MH_HIP_CHECK(hipEventRecord(evF[k], sA));
