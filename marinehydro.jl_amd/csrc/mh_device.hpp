@@ -190,13 +190,36 @@ __device__ __forceinline__ T struve_h1(T xx, const T& Y1) {  // wu.jl:331-362
 }
 
 // Horner in the reference's association: (c[d-1] + c[d] t) t, (c[i] + acc) t,
-// ..., c[0] + acc.
+// ..., c[0] + acc.  For a dual argument the value is formed the same way and
+// the polynomial's derivative in t alongside (one extra FMA per step), then
+// chained to t's tangents once.
+template <class T>
+struct DualWidth {
+    static constexpr int value = 0;
+};
+template <int P>
+struct DualWidth<Dual<P>> {
+    static constexpr int value = P;
+};
 template <int DEG, class T>
 __device__ __forceinline__ T horner_ref(const double (&c)[DEG + 1], T t) {
-    T acc = (c[DEG - 1] + c[DEG] * t) * t;
+    if constexpr (DualWidth<T>::value > 0) {
+        const double x = t.v;
+        double acc = (c[DEG - 1] + c[DEG] * x) * x;
+        double dacc = c[DEG - 1] + 2.0 * c[DEG] * x;
 #pragma unroll
-    for (int i = DEG - 2; i >= 1; --i) acc = (c[i] + acc) * t;
-    return c[0] + acc;
+        for (int i = DEG - 2; i >= 1; --i) {
+            const double u = c[i] + acc;
+            dacc = dacc * x + u;
+            acc = u * x;
+        }
+        return chain(t, c[0] + acc, dacc);
+    } else {
+        T acc = (c[DEG - 1] + c[DEG] * t) * t;
+#pragma unroll
+        for (int i = DEG - 2; i >= 1; --i) acc = (c[i] + acc) * t;
+        return c[0] + acc;
+    }
 }
 
 // G(h, v), dG/dh, dG/dv of the Wu wave term in reduced coordinates

@@ -85,12 +85,15 @@ __device__ __forceinline__ Dual<P> operator*(const Dual<P>& a, const Dual<P>& b)
     for (int m = 0; m < P; ++m) r.d[m] = a.d[m] * b.v + a.v * b.d[m];
     return r;
 }
+// Quotients take one division (the reciprocal): the gradient kernels use
+// dual numbers for derivatives, which a one-rounding-different value does
+// not change beyond the last bits.
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(const Dual<P>& a, const Dual<P>& b) {
     Dual<P> r;
-    const double q = a.v / b.v;
-    r.v = q;
     const double ib = 1.0 / b.v;
+    const double q = a.v * ib;
+    r.v = q;
 #pragma unroll
     for (int m = 0; m < P; ++m) r.d[m] = (a.d[m] - q * b.d[m]) * ib;
     return r;
@@ -135,15 +138,17 @@ __device__ __forceinline__ Dual<P> operator*(double a, const Dual<P>& b) {
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(const Dual<P>& a, double b) {
     Dual<P> r;
-    r.v = a.v / b;
+    const double ib = 1.0 / b;
+    r.v = a.v * ib;
 #pragma unroll
-    for (int m = 0; m < P; ++m) r.d[m] = a.d[m] / b;
+    for (int m = 0; m < P; ++m) r.d[m] = a.d[m] * ib;
     return r;
 }
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(double a, const Dual<P>& b) {
-    const double q = a / b.v;
-    return chain(b, q, -q / b.v);
+    const double ib = 1.0 / b.v;
+    const double q = a * ib;
+    return chain(b, q, -q * ib);
 }
 template <int P>
 __device__ __forceinline__ Dual<P>& operator+=(Dual<P>& a, const Dual<P>& b) {
