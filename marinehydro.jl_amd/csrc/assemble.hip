@@ -44,7 +44,7 @@ __global__ void k_pack_panels(int64_t nf, int64_t nv, const double* __restrict__
     out[j] = p;
 }
 
-constexpr int kFG = 4;  // wavenumbers per thread (blockIdx.z group)
+constexpr int kFG = 16;  // wavenumbers per thread (blockIdx.z group)
 
 template <int WAVE, int FORM>
 __global__ __launch_bounds__(256) void k_assemble(
