@@ -13,7 +13,9 @@
 // and the 64 lanes of a wave write one contiguous 1 KiB run of each
 // column-major complex128 matrix.  Rankine terms do not depend on the
 // wavenumber, so they are computed once per pair and reused for every
-// wavenumber of the group (blockIdx.z).
+// wavenumber of the group (blockIdx.z).  With a wave term the launch is
+// split: the Rankine parts once (k_rankine_parts), then the wave term once
+// per unordered pair (k_assemble_sym), since it is symmetric in (i, j).
 #include "mh_internal.hpp"
 
 namespace mh {
@@ -104,6 +106,179 @@ __global__ __launch_bounds__(256) void k_assemble(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric wave term.  The collocation points are the panel centres, so the
+// wave term of the pair (i, j) and of (j, i) is one evaluation at the same
+// reduced coordinates (horizontal distance and z_i + z_j); only the source
+// area, the sign of the horizontal gradient and the normals differ.
+// k_rankine_parts stores the frequency-independent Rankine parts once per
+// launch (real n x n: RS = I, RD = n_j . grad_xi, RK = n_i . grad_x), and
+// k_assemble_sym covers each unordered tile pair (I <= J) once: entry (i, j)
+// is written straight from the lanes (rows i), entry (j, i) goes through LDS
+// so that it is written down columns as well.
+// ---------------------------------------------------------------------------
+template <int FORM>
+__global__ __launch_bounds__(256) void k_rankine_parts(int64_t n, const double* __restrict__ C,
+                                                       const double* __restrict__ N,
+                                                       const PanelRec* __restrict__ panels, int gf,
+                                                       double* __restrict__ RS, double* __restrict__ RD,
+                                                       double* __restrict__ RK) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double x0 = C[ic], x1 = C[ic + n], x2 = C[ic + 2 * n];
+    double ni0 = 0.0, ni1 = 0.0, ni2 = 0.0;
+    if (FORM & 2) {
+        ni0 = N[ic];
+        ni1 = N[ic + n];
+        ni2 = N[ic + 2 * n];
+    }
+    const int64_t jb = (int64_t)blockIdx.y * 64;
+    for (int jj = wave; jj < 64; jj += 4) {
+        const int64_t j = jb + jj;
+        if (j >= n) break;
+        const PanelRec& p = panels[j];
+        double IR, gxiR[3], gxR[3];
+        rankine_terms(gf, x0, x1, x2, p, IR, gxiR, gxR);
+        if (!row_ok) continue;
+        const int64_t off = i + j * n;
+        RS[off] = IR;
+        if (FORM & 1) RD[off] = p.n[0] * gxiR[0] + p.n[1] * gxiR[1] + p.n[2] * gxiR[2];
+        if (FORM & 2) RK[off] = ni0 * gxR[0] + ni1 * gxR[1] + ni2 * gxR[2];
+    }
+}
+
+constexpr int kSymFG = 8;  // wavenumbers per workgroup (blockIdx.y group)
+constexpr int kSymFB = 4;  // wavenumbers staged in LDS per round
+
+template <int WAVE, int FORM>
+__global__ __launch_bounds__(256) void k_assemble_sym(
+    int64_t n, const double* __restrict__ C, const double* __restrict__ N, const PanelRec* __restrict__ panels,
+    int64_t nk, const double* __restrict__ ks, const double* __restrict__ RS, const double* __restrict__ RD,
+    const double* __restrict__ RK, double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K,
+    int64_t ld, int64_t stride, float2* __restrict__ DK32) {
+    // tile pair (I, J), I <= J: blockIdx.x = J (J + 1) / 2 + I
+    const int pidx = (int)blockIdx.x;
+    int J = (int)((::sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+    while ((J + 1) * (J + 2) / 2 <= pidx) ++J;
+    while (J * (J + 1) / 2 > pidx) --J;
+    const int I = pidx - J * (J + 1) / 2;
+    const bool dtile = I == J;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t i = (int64_t)I * 64 + lane;
+    const bool row_ok = i < n;
+    const int64_t ic = row_ok ? i : n - 1;
+    const double x0 = C[ic], x1 = C[ic + n], x2 = C[ic + 2 * n];
+    const double ni0 = N[ic], ni1 = N[ic + n], ni2 = N[ic + 2 * n];
+    const double ai = panels[ic].area;
+    const int64_t f0 = (int64_t)blockIdx.y * kSymFG;
+    const int nfg = (int)((nk - f0) < kSymFG ? (nk - f0) : kSymFG);
+    // wave parts of the (j, i) entries of one round: [wavenumber][j - row of
+    // the round][i - column]
+    __shared__ double2 sS[kSymFB][4][64];
+    __shared__ double2 sD[(FORM & 1) ? kSymFB : 1][4][64];
+    __shared__ double2 sK[(FORM & 2) ? kSymFB : 1][4][64];
+    for (int t = 0; t < 16; ++t) {
+        const int64_t j = (int64_t)J * 64 + 4 * t + wave;  // wave-uniform source column
+        const bool jv = j < n;
+        const bool dir = row_ok && jv && (!dtile || j >= i);
+        const bool trn = row_ok && jv && (!dtile || j > i);
+        const PanelRec& pj = panels[jv ? j : n - 1];
+        const double nj0 = pj.n[0], nj1 = pj.n[1], nj2 = pj.n[2];
+        const double aj = pj.area;
+        const double dx = x0 - pj.c[0], dy = x1 - pj.c[1], zs = x2 + pj.c[2];
+        const double hd = horiz_dist(dx, dy);
+        double rs = 0.0, rd = 0.0, rk = 0.0;
+        if (dir && RS) {
+            rs = RS[i + j * n];
+            if (FORM & 1) rd = RD[i + j * n];
+            if (FORM & 2) rk = RK[i + j * n];
+        }
+        const double dg = (i == j) ? 0.5 : 0.0;
+        for (int fb = 0; fb < nfg; fb += kSymFB) {
+#pragma unroll 1
+            for (int f = 0; f < kSymFB; ++f) {
+                const int64_t m = f0 + fb + f;
+                if (fb + f < nfg && (dir || trn)) {
+                    const double k = ks[m];
+                    cplx G, Gr, Gz;
+                    double d1, d2;
+                    wave_raw<WAVE>(k, dx, dy, zs, hd, G, Gr, Gz, d1, d2);
+                    // gradient w.r.t. the source point of (i, j), without the area:
+                    // (d1 Gr, d2 Gr, k Gz); of (j, i): (-d1 Gr, -d2 Gr, k Gz)
+                    const double kaj = k * aj, kai = k * ai;
+                    if (dir) {
+                        const int64_t off = m * stride + i + j * ld;
+                        S[off] = make_double2(-kInv4Pi * (rs + kaj * G.re), -kInv4Pi * (kaj * G.im));
+                        if (FORM & 1) {
+                            const double wr = kaj * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re));
+                            const double wi = kaj * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im));
+                            const double2 v = make_double2(dg - kInv4Pi * (rd + wr), -kInv4Pi * wi);
+                            D[off] = v;
+                            if (DK32) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
+                        }
+                        if (FORM & 2) {
+                            const double wr = kaj * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re));
+                            const double wi = kaj * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im));
+                            const double2 v = make_double2(dg - kInv4Pi * (rk + wr), -kInv4Pi * wi);
+                            K[off] = v;
+                            if (DK32 && !(FORM & 1)) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
+                        }
+                    }
+                    if (trn) {
+                        sS[f][wave][lane] = make_double2(kai * G.re, kai * G.im);
+                        if (FORM & 1)
+                            sD[f][wave][lane] =
+                                make_double2(kai * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re)),
+                                             kai * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im)));
+                        if (FORM & 2)
+                            sK[f][wave][lane] =
+                                make_double2(kai * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re)),
+                                             kai * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im)));
+                    }
+                }
+            }
+            __syncthreads();
+            {  // entries (row = J*64 + 4t + jj, column = I*64 + ii), four rows per column
+                const int jj = threadIdx.x & 3, ii = threadIdx.x >> 2;
+                const int64_t row = (int64_t)J * 64 + 4 * t + jj, col = (int64_t)I * 64 + ii;
+                if (row < n && col < n && (!dtile || row > col)) {
+                    double qs = 0.0, qd = 0.0, qk = 0.0;
+                    if (RS) {
+                        qs = RS[row + col * n];
+                        if (FORM & 1) qd = RD[row + col * n];
+                        if (FORM & 2) qk = RK[row + col * n];
+                    }
+#pragma unroll 1
+                    for (int f = 0; f < kSymFB && fb + f < nfg; ++f) {
+                        const int64_t m = f0 + fb + f;
+                        const int64_t off = m * stride + row + col * ld;
+                        const double2 ws = sS[f][jj][ii];
+                        S[off] = make_double2(-kInv4Pi * (qs + ws.x), -kInv4Pi * ws.y);
+                        if (FORM & 1) {
+                            const double2 w = sD[f][jj][ii];
+                            const double2 v = make_double2(-kInv4Pi * (qd + w.x), -kInv4Pi * w.y);
+                            D[off] = v;
+                            if (DK32) DK32[m * n * n + row + col * n] = make_float2((float)v.x, (float)v.y);
+                        }
+                        if (FORM & 2) {
+                            const double2 w = sK[f][jj][ii];
+                            const double2 v = make_double2(-kInv4Pi * (qk + w.x), -kInv4Pi * w.y);
+                            K[off] = v;
+                            if (DK32 && !(FORM & 1))
+                                DK32[m * n * n + row + col * n] = make_float2((float)v.x, (float)v.y);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out) {
     const int64_t nf = m->nfaces;
     if (nf == 0) return MH_OK;
@@ -156,6 +331,61 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
     }
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)groups);
     const double nmat = 1.0 + ((form & 1) ? 1.0 : 0.0) + ((form & 2) ? 1.0 : 0.0);
+    if (gf & (GF_WU | GF_DELHOMMEAU)) {
+        // Rankine parts once, then the symmetric wave-term pass
+        const bool rank = (gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) != 0;
+        const int nparts = 1 + ((form & 1) ? 1 : 0) + ((form & 2) ? 1 : 0);
+        double *RS = nullptr, *RD = nullptr, *RK = nullptr;
+        if (rank) {
+            const size_t need = sizeof(double) * (size_t)n * (size_t)n * (size_t)nparts;
+            if (need > ctx->rk_bytes) {
+                if (ctx->rk) {
+                    MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+                    MH_HIP_CHECK(hipFree(ctx->rk));
+                    ctx->rk = nullptr;
+                    ctx->rk_bytes = 0;
+                }
+                MH_HIP_CHECK(hipMalloc(&ctx->rk, need));
+                ctx->rk_bytes = need;
+            }
+            double* b = static_cast<double*>(ctx->rk);
+            RS = b;
+            int q = 1;
+            if (form & 1) RD = b + (size_t)(q++) * n * n;
+            if (form & 2) RK = b + (size_t)(q++) * n * n;
+            ProfScope pr(ctx, "rankine_parts", 0.0, 8.0 * nparts * (double)n * (double)n);
+            dim3 g2((unsigned)tiles, (unsigned)tiles);
+#define MH_RP(F) hipLaunchKernelGGL((k_rankine_parts<F>), g2, dim3(256), 0, ctx->stream, n, m->centers, \
+                                    m->normals, panels, gf, RS, RD, RK)
+            if (form == 1) MH_RP(1);
+            else if (form == 2) MH_RP(2);
+            else MH_RP(3);
+#undef MH_RP
+            MH_HIP_CHECK(hipGetLastError());
+        }
+        const int64_t pairs = tiles * (tiles + 1) / 2;
+        const int64_t sgroups = (nk + kSymFG - 1) / kSymFG;
+        if (pairs > 0x7fffffff || sgroups > 65535) {
+            set_error("assemble: problem too large for one launch");
+            return MH_ERR_ARGUMENT;
+        }
+        dim3 gs((unsigned)pairs, (unsigned)sgroups);
+        ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
+#define MH_AS(W, F) hipLaunchKernelGGL((k_assemble_sym<W, F>), gs, dim3(256), 0, ctx->stream, n, m->centers, \
+                                       m->normals, panels, nk, d_k, RS, RD, RK, S, D, K, ld, stride, DK32)
+        if (gf & GF_WU) {
+            if (form == 1) MH_AS(1, 1);
+            else if (form == 2) MH_AS(1, 2);
+            else MH_AS(1, 3);
+        } else {
+            if (form == 1) MH_AS(2, 1);
+            else if (form == 2) MH_AS(2, 2);
+            else MH_AS(2, 3);
+        }
+#undef MH_AS
+        MH_HIP_CHECK(hipGetLastError());
+        return MH_OK;
+    }
     ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
     if (gf & GF_WU)
         dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,

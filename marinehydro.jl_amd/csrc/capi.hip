@@ -140,6 +140,7 @@ int mh_context_destroy(mh_context_t ctx) {
         if (t.ws) (void)hipFree(t.ws);
     }
     if (c->tri_err) (void)hipFree(c->tri_err);
+    if (c->rk) (void)hipFree(c->rk);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->aux) (void)hipStreamDestroy(c->aux);

@@ -378,6 +378,25 @@ __device__ __forceinline__ void rankine_terms(int gf, T x0, T x1, T x2, const PT
     }
 }
 
+// Wave term of one pair at wavenumber k without the source area: G, dG/dr,
+// dG/dz in reduced coordinates and d1, d2 = dr/dxi_1, dr/dxi_2 (zero at
+// r <= 1e-6), for horizontal offset (dx, dy) of horizontal length hd.
+template <int WAVE>
+__device__ __forceinline__ void wave_raw(double k, double dx, double dy, double zsum, double hd, cplx& G,
+                                         cplx& Gr, cplx& Gz, double& d1, double& d2) {
+    const double r = k * hd;
+    const double z = k * zsum;
+    if (WAVE == 1) wu_dimless<true>(r, z, G, Gr, Gz);
+    if (WAVE == 2) delhommeau_dimless(r, z, G, Gr, Gz);
+    d1 = 0.0;
+    d2 = 0.0;
+    if (::fabs(r) > 1e-6) {
+        const double kk = k * k / r;
+        d1 = kk * (-dx);
+        d2 = kk * (-dy);
+    }
+}
+
 // Wave term for one pair at wavenumber k, already multiplied by the source
 // area (one-point rule, wu.jl:33-47): value, gradient w.r.t. the source
 // point (gxi) and w.r.t. the collocation point (gx).  Reduced coordinates per

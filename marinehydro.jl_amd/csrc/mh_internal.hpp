@@ -50,6 +50,9 @@ struct Context {
     size_t h_pin_count = 0;
     // events for ordering work between stream and aux (LU look-ahead)
     std::vector<hipEvent_t> ev_pool;
+    // frequency-independent Rankine parts of the assembly (grow-only)
+    void* rk = nullptr;
+    size_t rk_bytes = 0;
     bool profiling = false;
     std::vector<ProfRec> prof;
 };
