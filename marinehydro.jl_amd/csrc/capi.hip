@@ -67,6 +67,9 @@ int ensure_aux(Context* c) {
     MH_HIP_CHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    MH_HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_side_fork, hipEventDisableTiming));
+    MH_HIP_CHECK(hipEventCreateWithFlags(&c->ev_side_join, hipEventDisableTiming));
     return MH_OK;
 }
 int ensure_events(Context* c, size_t count) {
@@ -135,6 +138,7 @@ int mh_context_destroy(mh_context_t ctx) {
     if (c->ws) (void)hipFree(c->ws);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto& t : c->tri) {
         if (t.sync) (void)hipFree(t.sync);
         if (t.ws) (void)hipFree(t.ws);
@@ -144,6 +148,9 @@ int mh_context_destroy(mh_context_t ctx) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->ev_side_fork) (void)hipEventDestroy(c->ev_side_fork);
+    if (c->ev_side_join) (void)hipEventDestroy(c->ev_side_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     delete c;
@@ -400,6 +407,30 @@ static int join_aux(Context* c) {
     MH_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     return MH_OK;
 }
+// the side stream starts after the work issued on stream so far
+static int fork_side(Context* c) {
+    if (c->profiling) return MH_OK;
+    TRY(ensure_aux(c));
+    MH_HIP_CHECK(hipEventRecord(c->ev_side_fork, c->stream));
+    MH_HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_side_fork, 0));
+    return MH_OK;
+}
+// stream waits for everything issued on side so far
+static int join_side(Context* c) {
+    if (c->profiling) return MH_OK;
+    MH_HIP_CHECK(hipEventRecord(c->ev_side_join, c->side));
+    MH_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_side_join, 0));
+    return MH_OK;
+}
+// issue launches on side within a scope (no triangular solves there)
+struct OnSide {
+    Context* c;
+    hipStream_t saved;
+    explicit OnSide(Context* ctx) : c(ctx), saved(ctx->stream) {
+        if (!c->profiling) c->stream = c->side;
+    }
+    ~OnSide() { c->stream = saved; }
+};
 // issue launches on aux (triangular-solve state slot 1) within a scope
 struct OnAux {
     Context* c;
@@ -629,14 +660,21 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         float2* R32 = reinterpret_cast<float2*>(ws + o_R32);
         double* d_nrm = reinterpret_cast<double*>(ws + o_nrm);
         if (mixed) {
-            // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64
+            // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64.
+            // The forward right-hand side S bc (direct) is formed on the side
+            // stream while the LU runs.
+            if (direct) {
+                TRY(fork_side(c));
+                OnSide on(c);
+                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
+            }
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
             TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
             // forward system: phi = D \ (S bc), or sigma = K \ bc (kept in ybuf)
             IrSystem sys[2];
             if (direct) {
-                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
+                TRY(join_side(c));
                 sys[0] = {0, ybuf, nr, ldb, bstride, phi, Rb, R32, d_nrm};
             } else {
                 sys[0] = {0, rhs, nr, ldb, bstride, ybuf, Rb, R32, d_nrm};
@@ -718,8 +756,12 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         // lambda (and, indirect mixed, mu = S^T c) were computed with the solves
         const double2 *aS, *aDw, *sS, *sD, *rhoW;
         int64_t wss;
-        if (direct) {  // lambda = D^-T c; mu = S^T lambda
-            TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
+        if (direct) {  // lambda = D^-T c; mu = S^T lambda (on aux, alongside the pair kernels)
+            TRY(fork_aux(c));
+            {
+                OnAux on(c, true);
+                TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
+            }
             aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
         } else if (mixed) {  // lambda = K^-T S^T c
             aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
@@ -757,6 +799,7 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, dR));
                 TRY(launch_rankine_contract(c, n, Pg, nw, ndof, nr, dR, aS, aDw, sS, sD, wa, wss, Yb, T2));
             }
+            if (direct) TRY(join_aux(c));
             TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
             MH_HIP_CHECK(hipMemcpyAsync(hT1.data(), T1, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));

@@ -45,6 +45,9 @@ struct Context {
     // default priority, ordered against `stream` by events
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream for independent side work while aux serves the LU
+    hipStream_t side = nullptr;
+    hipEvent_t ev_side_fork = nullptr, ev_side_join = nullptr;
     // pinned host scratch for small device-to-host reads
     double* h_pin = nullptr;
     size_t h_pin_count = 0;
