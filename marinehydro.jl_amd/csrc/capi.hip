@@ -350,11 +350,17 @@ static int check_info(Context* c, const int* info, int64_t batch) {
     std::vector<int> h(batch);
     MH_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
     MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-    for (int64_t b = 0; b < batch; ++b)
-        if (h[b]) {
+    for (int64_t b = 0; b < batch; ++b) {
+        if (h[b] & 1) {
             set_error("solve: exact zero pivot in batch item " + std::to_string(b));
             return MH_ERR_SINGULAR;
         }
+        if (h[b] & 2) {
+            set_error("mixed precision: matrix entries beyond the half-precision split range (|x| >= 65504) in "
+                      "batch item " + std::to_string(b) + "; use the FP64 path");
+            return MH_ERR_NOT_CONVERGED;
+        }
+    }
     return MH_OK;
 }
 
@@ -696,8 +702,12 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 }
                 nsys = 2;
             }
-            TRY(ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
-                         nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr));
+            {
+                const int rc = ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
+                                        nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr);
+                if (rc == MH_ERR_NOT_CONVERGED) TRY(check_info(c, info, nw));  // a more specific cause
+                TRY(rc);
+            }
             if (nsys > 1) TRY(join_aux(c));
             if (!direct) TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
         } else if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)

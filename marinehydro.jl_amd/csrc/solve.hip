@@ -474,6 +474,168 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
         }
 }
 
+// FP32 trailing updates of the mixed-precision LU on half-precision matrix
+// cores.  Every FP32 operand x is split as x = hi + lo (hi = fp16(x), lo =
+// fp16(x - hi): 22 significant bits together) and each real product is
+// formed as hi*hi + hi*lo + lo*hi with FP32 accumulation; the dropped lo*lo
+// term is below 2^-22 of the product, the size of the FP32 accumulation
+// error over a 128-deep update.  The complex product folds into real
+// products of doubled depth: with the 16 complex depths of a chunk laid out
+// as [re (16) | im (16)],
+//   Re C = [Ar | Ai] . [Br ; -Bi],   Im C = [Ar | Ai] . [Bi ; Br]
+// (B is staged as [re | im | -im], so both operands of B are plain loads),
+// so one v_mfma_f32_16x16x32_f16 covers a 16 x 16 block, 16 complex depths
+// and one split term (six per block and chunk, against sixteen FP32 MFMAs
+// of four times the cycles).  Lane layout of that instruction: A[m = l%16]
+// [k = 8 (l/16) + e], B[k = 8 (l/16) + e][n = l%16], result register r is
+// row 4 (l/16) + r.  As in k_lu_gemm_big the tile's columns are the
+// instruction's rows, so results run along columns of C.  An operand outside
+// the half-precision range (|x| >= 65504) sets bit 1 of info[item].
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+constexpr int kHP = 40;   // LDS pitch of A in halves: 32 depths + 8 padding
+constexpr int kHPB = 56;  // LDS pitch of B in halves: 48 depths + 8 padding
+
+__device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo, bool& bad) {
+    bad = bad || !(::fabsf(x) < 65504.0f);
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+__global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int64_t ld, int64_t stride,
+                                                    int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
+                                                    int64_t k_lo, int kw, int* __restrict__ info) {
+    float2* M = A + (int64_t)blockIdx.z * stride;
+    const int64_t tr = r_lo + (int64_t)blockIdx.x * kBT;
+    const int64_t tc = c_lo + (int64_t)blockIdx.y * kBT;
+    // [hi, lo][tile row][depth: re 0..15, im 16..31] and
+    // [hi, lo][tile column][depth: re 0..15, im 16..31, -im 32..47]
+    __shared__ __attribute__((aligned(16))) _Float16 sA[2][kBT][kHP];
+    __shared__ __attribute__((aligned(16))) _Float16 sB[2][kBT][kHPB];
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
+    f4 accr[4][2], acci[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+            acci[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+    // global loads of a 16-deep chunk: A by (row, 4 depths), B by (column, 4
+    // depths); raw buffer loads with a per-lane byte offset and a
+    // wave-uniform one, offsets of entries outside the ranges replaced by 0
+    // and the values by zero (the host checks the extent fits 31 bits)
+    const int ra = tid & (kBT - 1), ga = __builtin_amdgcn_readfirstlane(tid >> 7);
+    const int cb = tid >> 2, gb = tid & 3;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        M, (short)0, (int)(((r_hi - 1) + (c_hi - 1) * ld + 1) * 8), 0x00020000);
+    const bool rok = tr + ra < r_hi, cok = tc + cb < c_hi;
+    const uint32_t voa = rok ? (uint32_t)(tr + ra) * 8u : 0u;
+    const uint32_t vob = cok ? (uint32_t)((tc + cb) * ld + k_lo + 4 * gb) * 8u : 0u;
+    float2 pa[4], pb[4];
+    bool bad = false;
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ka = kc + 4 * ga + e, kb = kc + 4 * gb + e;
+            const bool ina = rok && ka < kw, inb = cok && kb < kw;
+            const uint32_t soa = ka < kw ? (uint32_t)((k_lo + ka) * ld) * 8u : 0u;
+            const uint32_t sob = (uint32_t)(kc + e) * 8u;
+            const float2 va =
+                __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voa, (int)soa, 0));
+            const float2 vb = __builtin_bit_cast(
+                float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(inb ? vob : 0u), (int)(inb ? sob : 0u), 0));
+            pa[e] = ina ? va : make_float2(0.f, 0.f);
+            pb[e] = inb ? vb : make_float2(0.f, 0.f);
+        }
+    };
+    auto lstore = [&]() {
+        h4 rh, rl, ih, il;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            _Float16 a0, a1, a2, a3;
+            split_h(pa[e].x, a0, a1, bad);
+            split_h(pa[e].y, a2, a3, bad);
+            rh[e] = a0;
+            rl[e] = a1;
+            ih[e] = a2;
+            il[e] = a3;
+        }
+        *reinterpret_cast<h4*>(&sA[0][ra][4 * ga]) = rh;
+        *reinterpret_cast<h4*>(&sA[1][ra][4 * ga]) = rl;
+        *reinterpret_cast<h4*>(&sA[0][ra][16 + 4 * ga]) = ih;
+        *reinterpret_cast<h4*>(&sA[1][ra][16 + 4 * ga]) = il;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            _Float16 a0, a1, a2, a3;
+            split_h(pb[e].x, a0, a1, bad);
+            split_h(pb[e].y, a2, a3, bad);
+            rh[e] = a0;
+            rl[e] = a1;
+            ih[e] = a2;
+            il[e] = a3;
+        }
+        *reinterpret_cast<h4*>(&sB[0][cb][4 * gb]) = rh;
+        *reinterpret_cast<h4*>(&sB[1][cb][4 * gb]) = rl;
+        *reinterpret_cast<h4*>(&sB[0][cb][16 + 4 * gb]) = ih;
+        *reinterpret_cast<h4*>(&sB[1][cb][16 + 4 * gb]) = il;
+        *reinterpret_cast<h4*>(&sB[0][cb][32 + 4 * gb]) = -ih;
+        *reinterpret_cast<h4*>(&sB[1][cb][32 + 4 * gb]) = -il;
+    };
+    const int dq = 8 * (l >> 4);                 // this lane's depths of [Ar|Ai]
+    const int dq1 = l < 32 ? dq : dq + 16;       // of [Br|-Bi]
+    const int dq2 = l < 32 ? dq + 16 : dq - 16;  // of [Bi|Br]
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int kc = 0; kc < kw; kc += 16) {
+        const bool more = kc + 16 < kw;
+        if (more) gload(kc + 16);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int col = 32 * wc + 16 * b + (l & 15);
+            const h8 b1h = *reinterpret_cast<const h8*>(&sB[0][col][dq1]);
+            const h8 b1l = *reinterpret_cast<const h8*>(&sB[1][col][dq1]);
+            const h8 b2h = *reinterpret_cast<const h8*>(&sB[0][col][dq2]);
+            const h8 b2l = *reinterpret_cast<const h8*>(&sB[1][col][dq2]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int row = 64 * wr + 16 * a + (l & 15);
+                const h8 ah = *reinterpret_cast<const h8*>(&sA[0][row][dq]);
+                const h8 al = *reinterpret_cast<const h8*>(&sA[1][row][dq]);
+                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, ah, accr[a][b], 0, 0, 0);
+                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, ah, acci[a][b], 0, 0, 0);
+                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1l, ah, accr[a][b], 0, 0, 0);
+                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2l, ah, acci[a][b], 0, 0, 0);
+                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, al, accr[a][b], 0, 0, 0);
+                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, al, acci[a][b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
+    }
+    if (bad) atomicOr(info + blockIdx.z, 2);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
+            if (gr >= r_hi) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
+                if (gc >= c_hi) continue;
+                float2* o = M + gr + gc * ld;
+                const float2 v = *o;
+                *o = make_float2(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+            }
+        }
+}
+
 // ---------------------------------------------------------------------------
 // host-side recursion
 // ---------------------------------------------------------------------------
@@ -507,11 +669,16 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
     const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_big_f32") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
     ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
-    if (big)
-        hipLaunchKernelGGL(k_lu_gemm_big<T>,
-                           dim3((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch),
-                           dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
-    else
+    if (big) {
+        const dim3 grid((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch);
+        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores
+            hipLaunchKernelGGL(k_lu_gemm_h3, grid, dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi,
+                               c_lo, c_hi, k_lo, kw, P.info);
+        } else {
+            hipLaunchKernelGGL(k_lu_gemm_big<T>, grid, dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo,
+                               r_hi, c_lo, c_hi, k_lo, kw);
+        }
+    } else
         hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
                            dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
     return MH_OK;
@@ -598,6 +765,7 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
+    MH_REQUIRE(sizeof(T) == 8 || (ncols * ld) * 8 <= 0x7fffffff, "lu: FP32 matrix extent above 2 GiB");
     LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info, 0, 0};
     const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
     TRY_RC(ensure_aux(ctx));
