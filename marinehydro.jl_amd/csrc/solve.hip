@@ -351,21 +351,30 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
         }
         __syncthreads();
     }
+    // C -= acc: every entry of this lane read before any is written (a store
+    // between two loads would make each load wait for the store as well)
+    C2 cv[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int64_t gr = tr + 32 * wr + 16 * a + (l & 15);
-            if (gr >= r_hi) continue;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 32 * wr + 16 * a + (l & 15);
                 const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
-                if (gc >= c_hi) continue;
-                C2* o = M + gr + gc * ld;
-                const C2 v = *o;
-                *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+                cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : mkc<C2>(T(0), T(0));
             }
-        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 32 * wr + 16 * a + (l & 15);
+                const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
+                if (gr < r_hi && gc < c_hi)
+                    M[gr + gc * ld] = mkc<C2>(cv[a][b][r].x - accr[a][b][r], cv[a][b][r].y - acci[a][b][r]);
+            }
 }
 
 // Large trailing updates: 128 x 128 tile per 512-thread workgroup (8 waves,
@@ -456,22 +465,30 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
             __syncthreads();
         }
     }
+    // C -= acc, transposed blocks (operands swapped): lanes run along rows of
+    // C.  Per row block, its eight entries of this lane are read before any is
+    // written (a store between two loads would make each load wait for the
+    // store as well).
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+        const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
+        C2 cv[2][4];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            // transposed blocks (operands swapped): lanes run along rows of C
-            const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
-            if (gr >= r_hi) continue;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
-                if (gc >= c_hi) continue;
-                C2* o = M + gr + gc * ld;
-                const C2 v = *o;
-                *o = mkc<C2>(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+                cv[b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : mkc<C2>(T(0), T(0));
             }
-        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
+                if (gr < r_hi && gc < c_hi)
+                    M[gr + gc * ld] = mkc<C2>(cv[b][r].x - accr[a][b][r], cv[b][r].y - acci[a][b][r]);
+            }
+    }
 }
 
 // FP32 trailing updates of the mixed-precision LU on half-precision matrix
@@ -480,21 +497,21 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
 // formed as hi*hi + hi*lo + lo*hi with FP32 accumulation; the dropped lo*lo
 // term is below 2^-22 of the product, the size of the FP32 accumulation
 // error over a 128-deep update.  The complex product folds into real
-// products of doubled depth: with the 16 complex depths of a chunk laid out
-// as [re (16) | im (16)],
+// products of doubled depth: with the 32 complex depths of a chunk laid out
+// as [re (32) | im (32)],
 //   Re C = [Ar | Ai] . [Br ; -Bi],   Im C = [Ar | Ai] . [Bi ; Br]
-// (B is staged as [re | im | -im], so both operands of B are plain loads),
-// so one v_mfma_f32_16x16x32_f16 covers a 16 x 16 block, 16 complex depths
-// and one split term (six per block and chunk, against sixteen FP32 MFMAs
-// of four times the cycles).  Lane layout of that instruction: A[m = l%16]
-// [k = 8 (l/16) + e], B[k = 8 (l/16) + e][n = l%16], result register r is
-// row 4 (l/16) + r.  As in k_lu_gemm_big the tile's columns are the
-// instruction's rows, so results run along columns of C.  An operand outside
-// the half-precision range (|x| >= 65504) sets bit 1 of info[item].
+// (B is staged as [re | im | -im], so every operand is a plain LDS load),
+// and one v_mfma_f32_16x16x32_f16 covers a 16 x 16 block, 32 real depths
+// and one split term: twelve per block and chunk, against 32 FP32 MFMAs of
+// twice the cycles.  Lane layout of that instruction: A[m = l%16][k = 8 (l/16)
+// + e], B[k = 8 (l/16) + e][n = l%16], result register r is row 4 (l/16) + r.
+// As in k_lu_gemm_big the tile's columns are the instruction's rows, so the
+// results run along columns of C.  An operand outside the half-precision
+// range (|x| >= 65504) sets bit 1 of info[item].
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-constexpr int kHP = 40;   // LDS pitch of A in halves: 32 depths + 8 padding
-constexpr int kHPB = 56;  // LDS pitch of B in halves: 48 depths + 8 padding
+constexpr int kHK = 32;   // complex depths per chunk
+constexpr int kHPA = 72;  // LDS pitch of A in halves: 64 depths + 8 padding
+constexpr int kHPB = 104; // LDS pitch of B in halves: 96 depths + 8 padding
 
 __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo, bool& bad) {
     bad = bad || !(::fabsf(x) < 65504.0f);
@@ -502,15 +519,36 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo, boo
     lo = (_Float16)(x - (float)hi);
 }
 
+// XCD-aware tile order: workgroups are dealt to the 8 XCDs in turn, so
+// workgroup w runs on XCD w % 8; XCD x takes the contiguous range of tiles
+// [x per, (x + 1) per) of the (item, column tile, row tile) order, which
+// keeps each item's operand tiles in one XCD's L2.
+constexpr int kXcd = 8;
+__device__ __forceinline__ bool xcd_tile(int64_t ntr, int64_t ntc, int64_t ntiles, int64_t& item, int64_t& ti,
+                                         int64_t& tj) {
+    const int64_t w = blockIdx.x;
+    const int64_t per = (ntiles + kXcd - 1) / kXcd;
+    const int64_t t = (w % kXcd) * per + w / kXcd;
+    if (t >= ntiles) return false;
+    item = t / (ntr * ntc);
+    const int64_t rem = t % (ntr * ntc);
+    tj = rem / ntr;
+    ti = rem % ntr;
+    return true;
+}
+
 __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int64_t ld, int64_t stride,
                                                     int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
-                                                    int64_t k_lo, int kw, int* __restrict__ info) {
-    float2* M = A + (int64_t)blockIdx.z * stride;
-    const int64_t tr = r_lo + (int64_t)blockIdx.x * kBT;
-    const int64_t tc = c_lo + (int64_t)blockIdx.y * kBT;
-    // [hi, lo][tile row][depth: re 0..15, im 16..31] and
-    // [hi, lo][tile column][depth: re 0..15, im 16..31, -im 32..47]
-    __shared__ __attribute__((aligned(16))) _Float16 sA[2][kBT][kHP];
+                                                    int64_t k_lo, int kw, int* __restrict__ info, int64_t ntr,
+                                                    int64_t ntc, int64_t ntiles) {
+    int64_t item, ti, tj;
+    if (!xcd_tile(ntr, ntc, ntiles, item, ti, tj)) return;
+    float2* M = A + item * stride;
+    const int64_t tr = r_lo + ti * kBT;
+    const int64_t tc = c_lo + tj * kBT;
+    // [hi, lo][tile row][depth: re 0..31, im 32..63] and
+    // [hi, lo][tile column][depth: re 0..31, im 32..63, -im 64..95]
+    __shared__ __attribute__((aligned(16))) _Float16 sA[2][kBT][kHPA];
     __shared__ __attribute__((aligned(16))) _Float16 sB[2][kBT][kHPB];
     const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
@@ -522,23 +560,27 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
             accr[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
             acci[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
         }
-    // global loads of a 16-deep chunk: A by (row, 4 depths), B by (column, 4
-    // depths); raw buffer loads with a per-lane byte offset and a
-    // wave-uniform one, offsets of entries outside the ranges replaced by 0
-    // and the values by zero (the host checks the extent fits 31 bits)
+    // global loads of a chunk: A by (row, 8 depths), B by (column, 8 depths);
+    // raw buffer loads with a per-lane byte offset and a wave-uniform one,
+    // offsets of entries outside the ranges replaced by 0 and the values by
+    // zero (the host checks the extent fits 31 bits)
     const int ra = tid & (kBT - 1), ga = __builtin_amdgcn_readfirstlane(tid >> 7);
     const int cb = tid >> 2, gb = tid & 3;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         M, (short)0, (int)(((r_hi - 1) + (c_hi - 1) * ld + 1) * 8), 0x00020000);
     const bool rok = tr + ra < r_hi, cok = tc + cb < c_hi;
     const uint32_t voa = rok ? (uint32_t)(tr + ra) * 8u : 0u;
-    const uint32_t vob = cok ? (uint32_t)((tc + cb) * ld + k_lo + 4 * gb) * 8u : 0u;
-    float2 pa[4], pb[4];
+    const uint32_t vob = cok ? (uint32_t)((tc + cb) * ld + k_lo + 8 * gb) * 8u : 0u;
+    // the whole depth (at most kNBOuter = 4 chunks) is loaded up front, so
+    // every load is in flight at once
+    constexpr int kMaxChunks = kNBOuter / kHK;
+    float2 pa[kMaxChunks][8], pb[kMaxChunks][8];
     bool bad = false;
-    auto gload = [&](int kc) {
+    auto gload = [&](int c) {
+        const int kc = kHK * c;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int ka = kc + 4 * ga + e, kb = kc + 4 * gb + e;
+        for (int e = 0; e < 8; ++e) {
+            const int ka = kc + 8 * ga + e, kb = kc + 8 * gb + e;
             const bool ina = rok && ka < kw, inb = cok && kb < kw;
             const uint32_t soa = ka < kw ? (uint32_t)((k_lo + ka) * ld) * 8u : 0u;
             const uint32_t sob = (uint32_t)(kc + e) * 8u;
@@ -546,94 +588,105 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
                 __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voa, (int)soa, 0));
             const float2 vb = __builtin_bit_cast(
                 float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(inb ? vob : 0u), (int)(inb ? sob : 0u), 0));
-            pa[e] = ina ? va : make_float2(0.f, 0.f);
-            pb[e] = inb ? vb : make_float2(0.f, 0.f);
+            pa[c][e] = ina ? va : make_float2(0.f, 0.f);
+            pb[c][e] = inb ? vb : make_float2(0.f, 0.f);
         }
     };
-    auto lstore = [&]() {
-        h4 rh, rl, ih, il;
+    auto lstore = [&](int c) {
+        h8 rh, rl, ih, il;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < 8; ++e) {
             _Float16 a0, a1, a2, a3;
-            split_h(pa[e].x, a0, a1, bad);
-            split_h(pa[e].y, a2, a3, bad);
+            split_h(pa[c][e].x, a0, a1, bad);
+            split_h(pa[c][e].y, a2, a3, bad);
             rh[e] = a0;
             rl[e] = a1;
             ih[e] = a2;
             il[e] = a3;
         }
-        *reinterpret_cast<h4*>(&sA[0][ra][4 * ga]) = rh;
-        *reinterpret_cast<h4*>(&sA[1][ra][4 * ga]) = rl;
-        *reinterpret_cast<h4*>(&sA[0][ra][16 + 4 * ga]) = ih;
-        *reinterpret_cast<h4*>(&sA[1][ra][16 + 4 * ga]) = il;
+        *reinterpret_cast<h8*>(&sA[0][ra][8 * ga]) = rh;
+        *reinterpret_cast<h8*>(&sA[1][ra][8 * ga]) = rl;
+        *reinterpret_cast<h8*>(&sA[0][ra][32 + 8 * ga]) = ih;
+        *reinterpret_cast<h8*>(&sA[1][ra][32 + 8 * ga]) = il;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < 8; ++e) {
             _Float16 a0, a1, a2, a3;
-            split_h(pb[e].x, a0, a1, bad);
-            split_h(pb[e].y, a2, a3, bad);
+            split_h(pb[c][e].x, a0, a1, bad);
+            split_h(pb[c][e].y, a2, a3, bad);
             rh[e] = a0;
             rl[e] = a1;
             ih[e] = a2;
             il[e] = a3;
         }
-        *reinterpret_cast<h4*>(&sB[0][cb][4 * gb]) = rh;
-        *reinterpret_cast<h4*>(&sB[1][cb][4 * gb]) = rl;
-        *reinterpret_cast<h4*>(&sB[0][cb][16 + 4 * gb]) = ih;
-        *reinterpret_cast<h4*>(&sB[1][cb][16 + 4 * gb]) = il;
-        *reinterpret_cast<h4*>(&sB[0][cb][32 + 4 * gb]) = -ih;
-        *reinterpret_cast<h4*>(&sB[1][cb][32 + 4 * gb]) = -il;
+        *reinterpret_cast<h8*>(&sB[0][cb][8 * gb]) = rh;
+        *reinterpret_cast<h8*>(&sB[1][cb][8 * gb]) = rl;
+        *reinterpret_cast<h8*>(&sB[0][cb][32 + 8 * gb]) = ih;
+        *reinterpret_cast<h8*>(&sB[1][cb][32 + 8 * gb]) = il;
+        *reinterpret_cast<h8*>(&sB[0][cb][64 + 8 * gb]) = -ih;
+        *reinterpret_cast<h8*>(&sB[1][cb][64 + 8 * gb]) = -il;
     };
-    const int dq = 8 * (l >> 4);                 // this lane's depths of [Ar|Ai]
-    const int dq1 = l < 32 ? dq : dq + 16;       // of [Br|-Bi]
-    const int dq2 = l < 32 ? dq + 16 : dq - 16;  // of [Bi|Br]
-    gload(0);
-    lstore();
-    __syncthreads();
-    for (int kc = 0; kc < kw; kc += 16) {
-        const bool more = kc + 16 < kw;
-        if (more) gload(kc + 16);
+    const int dq = 8 * (l >> 4);  // this lane's depths within a 32-deep step
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int col = 32 * wc + 16 * b + (l & 15);
-            const h8 b1h = *reinterpret_cast<const h8*>(&sB[0][col][dq1]);
-            const h8 b1l = *reinterpret_cast<const h8*>(&sB[1][col][dq1]);
-            const h8 b2h = *reinterpret_cast<const h8*>(&sB[0][col][dq2]);
-            const h8 b2l = *reinterpret_cast<const h8*>(&sB[1][col][dq2]);
+    for (int c = 0; c < kMaxChunks; ++c)
+        if (kHK * c < kw) gload(c);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int row = 64 * wr + 16 * a + (l & 15);
-                const h8 ah = *reinterpret_cast<const h8*>(&sA[0][row][dq]);
-                const h8 al = *reinterpret_cast<const h8*>(&sA[1][row][dq]);
-                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, ah, accr[a][b], 0, 0, 0);
-                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, ah, acci[a][b], 0, 0, 0);
-                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1l, ah, accr[a][b], 0, 0, 0);
-                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2l, ah, acci[a][b], 0, 0, 0);
-                accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, al, accr[a][b], 0, 0, 0);
-                acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, al, acci[a][b], 0, 0, 0);
+    for (int c = 0; c < kMaxChunks; ++c) {
+        if (kHK * c >= kw) break;
+        if (c > 0) __syncthreads();  // previous chunk's operands consumed
+        lstore(c);
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {  // real depths: re (st 0), im (st 1)
+            // B operands: [Br ; -Bi] for Re C and [Bi ; Br] for Im C
+            const int o1 = st == 0 ? dq : 64 + dq;
+            const int o2 = st == 0 ? 32 + dq : dq;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int col = 32 * wc + 16 * b + (l & 15);
+                const h8 b1h = *reinterpret_cast<const h8*>(&sB[0][col][o1]);
+                const h8 b1l = *reinterpret_cast<const h8*>(&sB[1][col][o1]);
+                const h8 b2h = *reinterpret_cast<const h8*>(&sB[0][col][o2]);
+                const h8 b2l = *reinterpret_cast<const h8*>(&sB[1][col][o2]);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int row = 64 * wr + 16 * a + (l & 15);
+                    const h8 ah = *reinterpret_cast<const h8*>(&sA[0][row][32 * st + dq]);
+                    const h8 al = *reinterpret_cast<const h8*>(&sA[1][row][32 * st + dq]);
+                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, ah, accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, ah, acci[a][b], 0, 0, 0);
+                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1l, ah, accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2l, ah, acci[a][b], 0, 0, 0);
+                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, al, accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, al, acci[a][b], 0, 0, 0);
+                }
             }
         }
-        __syncthreads();
-        if (more) {
-            lstore();
-            __syncthreads();
-        }
     }
-    if (bad) atomicOr(info + blockIdx.z, 2);
+    if (bad) atomicOr(info + item, 2);
+    // C -= acc: every entry of this lane read before any is written (a store
+    // between two loads would make each load wait for the store as well)
+    float2 cv[4][2][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
-            if (gr >= r_hi) continue;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
                 const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
-                if (gc >= c_hi) continue;
-                float2* o = M + gr + gc * ld;
-                const float2 v = *o;
-                *o = make_float2(v.x - accr[a][b][r], v.y - acci[a][b][r]);
+                cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : make_float2(0.f, 0.f);
             }
-        }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
+                const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
+                if (gr < r_hi && gc < c_hi)
+                    M[gr + gc * ld] = make_float2(cv[a][b][r].x - accr[a][b][r], cv[a][b][r].y - acci[a][b][r]);
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -671,9 +724,12 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
     if (big) {
         const dim3 grid((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch);
-        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores
-            hipLaunchKernelGGL(k_lu_gemm_h3, grid, dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi,
-                               c_lo, c_hi, k_lo, kw, P.info);
+        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores (kw <= 128)
+            const int64_t ntr = grid.x, ntc = grid.y, ntiles = ntr * ntc * P.batch;
+            const int64_t nwg = (ntiles + kXcd - 1) / kXcd * kXcd;
+            MH_REQUIRE(nwg <= 0x7fffffff, "lu: too many tiles for one launch");
+            hipLaunchKernelGGL(k_lu_gemm_h3, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride,
+                               r_lo, r_hi, c_lo, c_hi, k_lo, kw, P.info, ntr, ntc, ntiles);
         } else {
             hipLaunchKernelGGL(k_lu_gemm_big<T>, grid, dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo,
                                r_hi, c_lo, c_hi, k_lo, kw);
