@@ -571,12 +571,14 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
     const bool rok = tr + ra < r_hi, cok = tc + cb < c_hi;
     const uint32_t voa = rok ? (uint32_t)(tr + ra) * 8u : 0u;
     const uint32_t vob = cok ? (uint32_t)((tc + cb) * ld + k_lo + 8 * gb) * 8u : 0u;
-    // the whole depth (at most kNBOuter = 4 chunks) is loaded up front, so
-    // every load is in flight at once
+    // operand chunks go through a ring of three register slots: the first
+    // three are loaded up front and chunk c + 3 is loaded as soon as chunk c
+    // has been staged, so two chunks of loads stay in flight
+    constexpr int kSlots = 3;
     constexpr int kMaxChunks = kNBOuter / kHK;
-    float2 pa[kMaxChunks][8], pb[kMaxChunks][8];
+    float2 pa[kSlots][8], pb[kSlots][8];
     bool bad = false;
-    auto gload = [&](int c) {
+    auto gload = [&](int slot, int c) {
         const int kc = kHK * c;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -588,17 +590,17 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
                 __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voa, (int)soa, 0));
             const float2 vb = __builtin_bit_cast(
                 float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(inb ? vob : 0u), (int)(inb ? sob : 0u), 0));
-            pa[c][e] = ina ? va : make_float2(0.f, 0.f);
-            pb[c][e] = inb ? vb : make_float2(0.f, 0.f);
+            pa[slot][e] = ina ? va : make_float2(0.f, 0.f);
+            pb[slot][e] = inb ? vb : make_float2(0.f, 0.f);
         }
     };
-    auto lstore = [&](int c) {
+    auto lstore = [&](int slot) {
         h8 rh, rl, ih, il;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             _Float16 a0, a1, a2, a3;
-            split_h(pa[c][e].x, a0, a1, bad);
-            split_h(pa[c][e].y, a2, a3, bad);
+            split_h(pa[slot][e].x, a0, a1, bad);
+            split_h(pa[slot][e].y, a2, a3, bad);
             rh[e] = a0;
             rl[e] = a1;
             ih[e] = a2;
@@ -611,8 +613,8 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             _Float16 a0, a1, a2, a3;
-            split_h(pb[c][e].x, a0, a1, bad);
-            split_h(pb[c][e].y, a2, a3, bad);
+            split_h(pb[slot][e].x, a0, a1, bad);
+            split_h(pb[slot][e].y, a2, a3, bad);
             rh[e] = a0;
             rl[e] = a1;
             ih[e] = a2;
@@ -626,14 +628,28 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
         *reinterpret_cast<h8*>(&sB[1][cb][64 + 8 * gb]) = -il;
     };
     const int dq = 8 * (l >> 4);  // this lane's depths within a 32-deep step
+    float2 cv[4][2][4];           // this lane's C entries (half read during the last chunk)
 #pragma unroll
-    for (int c = 0; c < kMaxChunks; ++c)
-        if (kHK * c < kw) gload(c);
+    for (int c = 0; c < kSlots; ++c)
+        if (kHK * c < kw) gload(c, c);
 #pragma unroll
     for (int c = 0; c < kMaxChunks; ++c) {
         if (kHK * c >= kw) break;
         if (c > 0) __syncthreads();  // previous chunk's operands consumed
-        lstore(c);
+        lstore(c % kSlots);
+        if (kHK * (c + kSlots) < kw) gload(c % kSlots, c + kSlots);
+        if (kHK * (c + 1) >= kw) {  // last chunk: read half of this lane's C entries meanwhile
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
+                        const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
+                        cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : make_float2(0.f, 0.f);
+                    }
+        }
         __syncthreads();
 #pragma unroll
         for (int st = 0; st < 2; ++st) {  // real depths: re (st 0), im (st 1)
@@ -665,9 +681,8 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
     if (bad) atomicOr(info + item, 2);
     // C -= acc: every entry of this lane read before any is written (a store
     // between two loads would make each load wait for the store as well)
-    float2 cv[4][2][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 2; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -724,7 +739,7 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
     if (big) {
         const dim3 grid((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch);
-        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores (kw <= 128)
+        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores (kw <= kNBOuter)
             const int64_t ntr = grid.x, ntc = grid.y, ntiles = ntr * ntc * P.batch;
             const int64_t nwg = (ntiles + kXcd - 1) / kXcd * kXcd;
             MH_REQUIRE(nwg <= 0x7fffffff, "lu: too many tiles for one launch");
