@@ -492,11 +492,13 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
 }
 
 // FP32 trailing updates of the mixed-precision LU on half-precision matrix
-// cores.  Every FP32 operand x is split as x = hi + lo (hi = fp16(x), lo =
-// fp16(x - hi): 22 significant bits together) and each real product is
-// formed as hi*hi + hi*lo + lo*hi with FP32 accumulation; the dropped lo*lo
-// term is below 2^-22 of the product, the size of the FP32 accumulation
-// error over a 128-deep update.  The complex product folds into real
+// cores.  Every FP32 operand is scaled by 2^12 and split as x = hi + lo (hi =
+// fp16(x), lo = fp16(x - hi): 22 significant bits together; the scaling keeps
+// lo out of the fp16 subnormal range for |x| down to 3e-5, below which the
+// absolute error is under 2^-36) and each real product is formed as hi*hi +
+// hi*lo + lo*hi with FP32 accumulation, the sum scaled back by 2^-24; the
+// dropped lo*lo term is below 2^-22 of the product, the size of the FP32
+// accumulation error over a 128-deep update.  The complex product folds into real
 // products of doubled depth: with the 32 complex depths of a chunk laid out
 // as [re (32) | im (32)],
 //   Re C = [Ar | Ai] . [Br ; -Bi],   Im C = [Ar | Ai] . [Bi ; Br]
@@ -507,16 +509,20 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
 // + e], B[k = 8 (l/16) + e][n = l%16], result register r is row 4 (l/16) + r.
 // As in k_lu_gemm_big the tile's columns are the instruction's rows, so the
 // results run along columns of C.  An operand outside the half-precision
-// range (|x| >= 65504) sets bit 1 of info[item].
+// range after scaling (|x| >= 65504 / 4096 = 15.99) sets bit 1 of
+// info[item].
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 constexpr int kHK = 32;   // complex depths per chunk
 constexpr int kHPA = 72;  // LDS pitch of A in halves: 64 depths + 8 padding
 constexpr int kHPB = 104; // LDS pitch of B in halves: 96 depths + 8 padding
 
+constexpr float kHScale = 4096.0f;      // 2^12 per operand
+constexpr float kHUnscale = 0x1p-24f;   // 2^-24 per product
 __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo, bool& bad) {
-    bad = bad || !(::fabsf(x) < 65504.0f);
-    hi = (_Float16)x;
-    lo = (_Float16)(x - (float)hi);
+    const float xs = x * kHScale;  // exact
+    bad = bad || !(::fabsf(xs) < 65504.0f);
+    hi = (_Float16)xs;
+    lo = (_Float16)(xs - (float)hi);
 }
 
 // XCD-aware tile order: workgroups are dealt to the 8 XCDs in turn, so
@@ -700,7 +706,8 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
                 const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
                 const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
                 if (gr < r_hi && gc < c_hi)
-                    M[gr + gc * ld] = make_float2(cv[a][b][r].x - accr[a][b][r], cv[a][b][r].y - acci[a][b][r]);
+                    M[gr + gc * ld] = make_float2(cv[a][b][r].x - kHUnscale * accr[a][b][r],
+                                                  cv[a][b][r].y - kHUnscale * acci[a][b][r]);
             }
 }
 
