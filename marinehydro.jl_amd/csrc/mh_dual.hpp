@@ -16,6 +16,18 @@ struct Dual {
     double d[P];
 };
 
+// 1/x for the tangent arithmetic: the hardware reciprocal estimate refined
+// by two Newton steps (error within a few ulp for normal, finite x; the
+// quotient's special-case handling of a full division is not needed for
+// derivative terms).
+__device__ __forceinline__ double drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ double val(double x) { return x; }
 template <int P>
 __device__ __forceinline__ double val(const Dual<P>& x) {
@@ -91,7 +103,7 @@ __device__ __forceinline__ Dual<P> operator*(const Dual<P>& a, const Dual<P>& b)
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(const Dual<P>& a, const Dual<P>& b) {
     Dual<P> r;
-    const double ib = 1.0 / b.v;
+    const double ib = drcp(b.v);
     const double q = a.v * ib;
     r.v = q;
 #pragma unroll
@@ -138,7 +150,7 @@ __device__ __forceinline__ Dual<P> operator*(double a, const Dual<P>& b) {
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(const Dual<P>& a, double b) {
     Dual<P> r;
-    const double ib = 1.0 / b;
+    const double ib = drcp(b);
     r.v = a.v * ib;
 #pragma unroll
     for (int m = 0; m < P; ++m) r.d[m] = a.d[m] * ib;
@@ -146,7 +158,7 @@ __device__ __forceinline__ Dual<P> operator/(const Dual<P>& a, double b) {
 }
 template <int P>
 __device__ __forceinline__ Dual<P> operator/(double a, const Dual<P>& b) {
-    const double ib = 1.0 / b.v;
+    const double ib = drcp(b.v);
     const double q = a * ib;
     return chain(b, q, -q * ib);
 }
@@ -172,7 +184,7 @@ __device__ __forceinline__ double atan2(double y, double x) { return ::atan2(y, 
 template <int P>
 __device__ __forceinline__ Dual<P> sqrt(const Dual<P>& x) {
     const double s = ::sqrt(x.v);
-    return chain(x, s, 0.5 / s);
+    return chain(x, s, 0.5 * drcp(s));
 }
 template <int P>
 __device__ __forceinline__ Dual<P> exp(const Dual<P>& x) {
@@ -181,7 +193,7 @@ __device__ __forceinline__ Dual<P> exp(const Dual<P>& x) {
 }
 template <int P>
 __device__ __forceinline__ Dual<P> log(const Dual<P>& x) {
-    return chain(x, ::log(x.v), 1.0 / x.v);
+    return chain(x, ::log(x.v), drcp(x.v));
 }
 template <int P>
 __device__ __forceinline__ Dual<P> fabs(const Dual<P>& x) {
@@ -192,7 +204,7 @@ template <int P>
 __device__ __forceinline__ Dual<P> atan2(const Dual<P>& y, const Dual<P>& x) {
     Dual<P> r;
     r.v = ::atan2(y.v, x.v);
-    const double den = 1.0 / (x.v * x.v + y.v * y.v);
+    const double den = drcp(x.v * x.v + y.v * y.v);
 #pragma unroll
     for (int m = 0; m < P; ++m) r.d[m] = (x.v * y.d[m] - y.v * x.d[m]) * den;
     return r;
@@ -213,13 +225,13 @@ template <int P>
 __device__ __forceinline__ void bessel_j01(const Dual<P>& x, Dual<P>& J0, Dual<P>& J1) {
     const double a = ::j0(x.v), b = ::j1(x.v);
     J0 = chain(x, a, -b);
-    J1 = chain(x, b, (x.v != 0.0) ? a - b / x.v : 0.5);
+    J1 = chain(x, b, (x.v != 0.0) ? a - b * drcp(x.v) : 0.5);
 }
 template <int P>
 __device__ __forceinline__ void bessel_y01(const Dual<P>& x, Dual<P>& Y0, Dual<P>& Y1) {
     const double a = ::y0(x.v), b = ::y1(x.v);
     Y0 = chain(x, a, -b);
-    Y1 = chain(x, b, a - b / x.v);
+    Y1 = chain(x, b, a - b * drcp(x.v));
 }
 
 // lift a double with given tangents
