@@ -26,7 +26,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "marinehydro.jl_amd"))
 
 METRIC = "freq-solves/s incl. adjoint grads at 2k panels, mixed vs FP64 at ≤1e-6 rel err"
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense matrix peaks, TFLOP/s
+PEAK = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0}  # MI355X dense matrix peaks, TFLOP/s
+# matrix-core kernels of the sweep: timer name -> (peak key, executed products
+# per algorithmic complex-GEMM flop).  The mixed LU's trailing updates run on
+# fp16 matrix cores as three split products per real product.
+MFMA_KERNELS = {"lu_gemm_big": ("fp64", 1), "lu_gemm": ("fp64", 1), "lu_gemm_f32": ("fp32", 1),
+                "lu_gemm_f16x3": ("fp16", 3)}
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -153,20 +158,36 @@ def main():
     step()
     stats = mh._lib.profile_report()
     mh._lib.profile_enable(False)
-    dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    # The roofline is the dominant matrix-core kernel's (the LU trailing
+    # update); the pair kernels are FP64 VALU-bound with no algorithmic flop
+    # count (Bessel, log, exp), so the longest-running kernel is named and,
+    # when it is one of them, its VALU issue utilisation from rocprofv3
+    # counters is attached (profiles/valu.json).
+    top_name, top = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    mfma = {k: v for k, v in stats.items() if k in MFMA_KERNELS and v["flops"] > 0}
+    dom_name, dom = max(mfma.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dom["ms"] / dom["launches"]
-    if dom["flops"] > 0:
-        achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
-        peak = PEAK["fp64" if args.precision == "fp64" else "fp32"]
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None}
-    else:
-        achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK, 4), "traffic": None}
+    pkey, mult = MFMA_KERNELS[dom_name]
+    alg = dom["flops"] / (dom["ms"] * 1e-3) / 1e12  # complex-GEMM flops (8 per complex MAC)
+    achieved = mult * alg
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[pkey], "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK[pkey], 4), "traffic": None}
     roof["kernel"] = dom_name
     roof["avg_launch_ms"] = round(avg_ms, 4)
     roof["launches_per_step"] = dom["launches"]
+    if mult != 1:
+        roof["complex_gemm_tflops"] = round(alg, 3)
+        roof["note"] = (f"{mult} fp16 products per real product (split FP32 operands); achieved counts "
+                        f"the fp16 matrix-core work, complex_gemm_tflops the FP32-equivalent rate")
+    roof["longest_kernel"] = {"name": top_name, "ms_per_step": round(top["ms"], 3)}
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if top_name != dom_name and os.path.exists(vpath):
+        with open(vpath) as f:
+            vj = json.load(f)
+        ent = vj.get(f"{top_name}|{args.precision}|{n}x{args.nomega}")
+        if ent:
+            roof["longest_kernel"]["valu_issue_utilisation"] = ent["valu_issue_utilisation"]
+            roof["longest_kernel"]["source"] = ent["summary"]
     # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of
     # the same workload, scaled by the counters' reading for a copy of known
     # size (profiles/traffic.json, written from those runs)

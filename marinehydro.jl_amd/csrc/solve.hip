@@ -741,7 +741,7 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
     if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
     const bool big = mr >= kBT && mc >= kBT;
-    const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_big_f32") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
+    const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_f16x3") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
     ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
                  2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
     if (big) {
