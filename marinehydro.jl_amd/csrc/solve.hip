@@ -24,13 +24,80 @@ constexpr int kNBOuter = 128;  // block-column width of the trailing updates
 constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
 // ---------------------------------------------------------------------------
+// Wave reductions on DPP lane moves.  Within each row of 16 lanes the moves
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror and row_mirror
+// (gfx9 DPP control codes below) pair every lane with one in the other half
+// of a group of 2, 4, 8 and 16 lanes, so four steps leave the row's
+// reduction in every lane of the row; the four rows are then combined from
+// lanes 0, 16, 32 and 48 by readlane (a wave-uniform result).  Every lane of
+// the wave must be active.
+// ---------------------------------------------------------------------------
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x) {
+    return __int_as_float(dpp_i32<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_i32<CTRL>((int)b), hi = dpp_i32<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int x) {
+    return dpp_i32<CTRL>(x);
+}
+__device__ __forceinline__ float read_lane(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double read_lane(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ int read_lane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+// maximum (MAX) or minimum over each row of 16 lanes; x never NaN
+template <bool MAX, class V>
+__device__ __forceinline__ V row_reduce(V x) {
+    V y = dpp_move<kDppXor1>(x);
+    x = MAX ? (y > x ? y : x) : (y < x ? y : x);
+    y = dpp_move<kDppXor2>(x);
+    x = MAX ? (y > x ? y : x) : (y < x ? y : x);
+    y = dpp_move<kDppHalfMirror>(x);
+    x = MAX ? (y > x ? y : x) : (y < x ? y : x);
+    y = dpp_move<kDppMirror>(x);
+    x = MAX ? (y > x ? y : x) : (y < x ? y : x);
+    return x;
+}
+template <bool MAX, class V>
+__device__ __forceinline__ V wave_reduce(V x) {
+    x = row_reduce<MAX>(x);
+    V r = read_lane(x, 0);
+#pragma unroll
+    for (int l = 16; l < 64; l += 16) {
+        const V y = read_lane(x, l);
+        r = MAX ? (y > r ? y : r) : (y < r ? y : r);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= kLeaf, held in
-// registers.  Thread t owns local rows t + 1024 q, q < R.  The leaf's row
-// interchanges are then applied to the other columns of the enclosing block
-// column [pc_lo, pc_hi) (at most kNBOuter wide), which replaces the separate
-// swap passes of the recursion: the left columns hold final L values whose
-// rows are only re-ordered, the right columns are not updated before the
-// interchanges of every earlier leaf have reached them.
+// registers.  Thread t owns local rows t + 1024 q, q < R.  Per column: every
+// wave finds its largest |re|+|im| (first row on ties) and the lane owning
+// that row publishes the row with the wave's maximum; after one barrier
+// every wave picks the winner among the 16 candidates and reads its row.
+// The leaf's row interchanges are then applied to the other columns of the
+// enclosing block column [pc_lo, pc_hi) (at most kNBOuter wide), which
+// replaces the separate swap passes of the recursion: the left columns hold
+// final L values whose rows are only re-ordered, the right columns are not
+// updated before the interchanges of every earlier leaf have reached them.
 // ---------------------------------------------------------------------------
 template <class T, int R, int kLeaf>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
@@ -42,10 +109,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = (int)(n - k0);
-    // double-buffered by column parity: two barriers per column
+    // double-buffered by column parity: one barrier per column (a wave writes
+    // buffer c & 1 again only after every wave has passed the barrier of
+    // column c + 1, i.e. finished reading column c)
     __shared__ T s_v[2][16];
     __shared__ int s_i[2][16];
-    __shared__ C2 s_crow[2][kLeaf], s_prow[2][kLeaf];
+    __shared__ C2 s_cand[2][16][kLeaf];           // each wave's candidate pivot row
+    __shared__ C2 s_crow[2][kLeaf];
     __shared__ int s_p[kLeaf];                    // local pivot rows
     __shared__ int s_rows[2 * kLeaf], s_src[2 * kLeaf];
     C2 a[R][kLeaf];
@@ -73,63 +143,54 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
                 }
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            s_v[bf][wave] = best;
-            s_i[bf][wave] = bi;
-        }
-        __syncthreads();
-        // every wave reduces the 16 wave maxima itself (first index on ties)
-        T b = s_v[bf][lane & 15];
-        int p = s_i[bf][lane & 15];
-        for (int off = 8; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(b, off);
-            const int oi = __shfl_xor(p, off);
-            if (ov > b || (ov == b && oi < p)) {
-                b = ov;
-                p = oi;
-            }
-        }
-        const bool zero = !(b > T(0));  // exact zero pivot: record, no interchange, no update
-        if (zero) p = c;
-        if (tid == 0) {
-            if (zero) atomicOr(info + blockIdx.x, 1);
-            piv[k0 + c] = k0 + p;
-            s_p[c] = p;
-        }
-        if (tid == c) {
-#pragma unroll
-            for (int j = 0; j < kLeaf; ++j) s_crow[bf][j] = a[0][j];
-        }
-        {  // the pivot row's owner publishes it (selects keep register indices static)
+        // the wave's maximum, then the first row that attains it
+        const T wv = wave_reduce<true>(best);
+        const int wi = wave_reduce<false>(best == wv ? bi : 0x7fffffff);
+        {  // the owner of row wi publishes it (selects keep register indices static)
             bool own = false;
 #pragma unroll
-            for (int q = 0; q < R; ++q) own = own || (tid + 1024 * q == p);
+            for (int q = 0; q < R; ++q) own = own || (tid + 1024 * q == wi);
             if (own) {
 #pragma unroll
                 for (int j = 0; j < kLeaf; ++j) {
                     C2 v = a[0][j];
 #pragma unroll
                     for (int q = 1; q < R; ++q) {
-                        const bool o = tid + 1024 * q == p;
+                        const bool o = tid + 1024 * q == wi;
                         v = mkc<C2>(o ? a[q][j].x : v.x, o ? a[q][j].y : v.y);
                     }
-                    s_prow[bf][j] = v;
+                    s_cand[bf][wave][j] = v;
                 }
             }
         }
+        if (lane == 0) {
+            s_v[bf][wave] = wv;
+            s_i[bf][wave] = wi;
+        }
+        if (tid == c) {
+#pragma unroll
+            for (int j = 0; j < kLeaf; ++j) s_crow[bf][j] = a[0][j];
+        }
         __syncthreads();
+        // every wave reduces the 16 wave maxima itself (first row on ties)
+        const T bv = s_v[bf][lane & 15];
+        const int biv = s_i[bf][lane & 15];
+        const T bm = row_reduce<true>(bv);
+        const int pm = row_reduce<false>(bv == bm ? biv : 0x7fffffff);
+        const T b = read_lane(bm, 0);
+        int p = read_lane(pm, 0);
+        const bool zero = !(b > T(0));  // exact zero pivot: record, no interchange, no update
+        if (zero) p = c;
+        const C2* prow = s_cand[bf][(p & 1023) >> 6];  // the wave that owns row p
+        if (tid == 0) {
+            if (zero) atomicOr(info + blockIdx.x, 1);
+            piv[k0 + c] = k0 + p;
+            s_p[c] = p;
+        }
         if (p != c) {
             if (tid == c) {
 #pragma unroll
-                for (int j = 0; j < kLeaf; ++j) a[0][j] = s_prow[bf][j];
+                for (int j = 0; j < kLeaf; ++j) a[0][j] = prow[j];
             }
 #pragma unroll
             for (int q = 0; q < R; ++q) {
@@ -142,7 +203,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
             }
         }
         if (!zero) {
-            const C2 inv = crecip(s_prow[bf][c]);
+            const C2 inv = crecip(prow[c]);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int r = tid + 1024 * q;
@@ -151,7 +212,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
                     a[q][c] = l;
 #pragma unroll
                     for (int j = 1; j < kLeaf; ++j)  // constant trip count: unrolled before c is known
-                        if (j > c) a[q][j] = cfms(a[q][j], l, s_prow[bf][j]);
+                        if (j > c) a[q][j] = cfms(a[q][j], l, prow[j]);
                 }
             }
         }
@@ -786,7 +847,7 @@ template <class T>
 static int leaf_width(int64_t m) {
     const int R = m <= 1024 ? 1 : (m <= 2048 ? 2 : 4);
     if (sizeof(T) == 4) return R <= 2 ? 16 : 8;
-    return R == 1 ? 16 : 8;
+    return R == 1 ? 16 : (R == 2 ? 8 : 4);  // panels of 64 VGPRs per thread
 }
 
 template <class T>
@@ -804,7 +865,8 @@ static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
         if constexpr (sizeof(T) == 4) LEAF(2, 16);
         else LEAF(2, 8);
     } else {
-        LEAF(4, 8);
+        if constexpr (sizeof(T) == 4) LEAF(4, 8);
+        else LEAF(4, 4);
     }
 #undef LEAF
     return MH_OK;

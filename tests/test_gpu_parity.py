@@ -177,6 +177,56 @@ def test_lu_matches_oracle_pivots(mh, oracle, gpu):
     assert relmax(dA.cpu().numpy().T, LUo) < 1e-12
 
 
+@pytest.mark.parametrize("n", [700, 1100, 2100])
+def test_lu_exact_factors_with_pivot_ties(mh, gpu, n):
+    # A = L0 U0 with L0 unit lower in {-1, 0, 1} and U0 upper with entries in
+    # {0, +-1, +-i} and diagonal in {+-1, +-i, +-1+-i}: every step of the
+    # elimination is exact in FP64 (Gaussian integers, reciprocals of the
+    # pivots are exact), and column c of the active matrix is
+    # L0[c:, c] U0[c, c], so every row with L0[i, c] != 0 ties with row c in
+    # |re| + |im|.  Partial pivoting that takes the first row of a tie (LAPACK's
+    # izamax) keeps the diagonal: ipiv = 0..n-1 and LU = L0 + U0 - I exactly.
+    # The sizes take the three leaf shapes (1, 2 and 4 rows per thread).
+    torch = gpu
+    rng = np.random.default_rng(n)
+    L0 = np.tril(rng.integers(-1, 2, (n, n)), -1).astype(np.complex128) + np.eye(n)
+    units = np.array([1, -1, 1j, -1j, 1 + 1j, 1 - 1j, -1 + 1j, -1 - 1j])
+    U0 = np.triu(np.array([0, 1, -1, 1j, -1j])[rng.integers(0, 5, (n, n))], 1) + np.diag(units[rng.integers(0, 8, n)])
+    A = L0 @ U0
+    assert np.abs(A.real).max() < 2 ** 20  # products and sums stay exact
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    mh._lib.check(mh._lib.lib().mh_lu_factor(mh._lib.context(0), n, 1, dA.data_ptr(), n, n * n,
+                                             piv.data_ptr()))
+    assert np.array_equal(piv.cpu().numpy(), np.arange(n))
+    LU = dA.cpu().numpy().T
+    assert np.array_equal(LU, np.tril(L0, -1) + U0)
+
+
+def test_lu_pivot_tie_across_waves_and_slots(mh, gpu):
+    # equal |re| + |im| = 3 in four rows of the first column of a 1100 x 1100
+    # matrix (panel rows 1030 and 1090 lie in the second register slot of
+    # threads 6 and 66, rows 300 and 700 in the first slot of threads in
+    # later waves); the first row, 300, is the pivot
+    torch = gpu
+    n = 1100
+    rng = np.random.default_rng(11)
+    A = 0.05 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A[:, 0] *= 0.1
+    A[1030, 0], A[300, 0], A[700, 0], A[1090, 0] = 2.0 + 1.0j, -1.0 - 2.0j, 3.0j, -3.0
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    ctx = mh._lib.context(0)
+    L = mh._lib.lib()
+    mh._lib.check(L.mh_lu_factor(ctx, n, 1, dA.data_ptr(), n, n * n, piv.data_ptr()))
+    assert int(piv[0]) == 300
+    b = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    dB = torch.from_numpy(np.ascontiguousarray(b.T)).cuda()
+    mh._lib.check(L.mh_lu_solve(ctx, n, 1, dA.data_ptr(), n, n * n, piv.data_ptr(), 0, 2,
+                                dB.data_ptr(), n, 2 * n))
+    assert relmax(dB.cpu().numpy().T, np.linalg.solve(A, b)) < 1e-10
+
+
 def test_singular_matrix_reports_error(mh, gpu):
     torch = gpu
     n = 40
