@@ -665,6 +665,16 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         double2* Rb = reinterpret_cast<double2*>(ws + o_R);
         float2* R32 = reinterpret_cast<float2*>(ws + o_R32);
         double* d_nrm = reinterpret_cast<double*>(ws + o_nrm);
+        // the frequency-independent Rankine parts of the gradient (first
+        // parameter block) depend only on the mesh: side stream, while the LU runs
+        const bool early_rk = ga && ndof && rank_mask;
+        if (early_rk) {
+            TRY(fork_side(c));
+            OnSide on(c);
+            double* tan = reinterpret_cast<double*>(ws + o_tan);
+            TRY(launch_pack_tangents(c, mesh, Pg, ga->nparam, 0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
+            TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, reinterpret_cast<double*>(ws + o_dR)));
+        }
         if (mixed) {
             // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64.
             // The forward right-hand side S bc (direct) is formed on the side
@@ -750,9 +760,15 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         double2* Fw = F ? reinterpret_cast<double2*>(F) + w0 * nbeta * ndof : nullptr;
         TRY(launch_forces(c, mesh, nw, d_om + w0, rho, ndof, dof_fields, nbeta, phi, ldb, bstride, Aw,
                           Bw, Fw));
-        TRY(tri_check(c));
-        TRY(check_info(c, info, nw));
-        if (!ga || ndof == 0) continue;
+        // solve failures (a wait that timed out, a zero pivot) are reported
+        // at the end of the chunk, after the gradient kernels: one host
+        // synchronisation per chunk
+        if (!ga || ndof == 0) {
+            TRY(tri_check(c));
+            TRY(check_info(c, info, nw));
+            continue;
+        }
+        if (early_rk) TRY(join_side(c));
         // ---- gradients (grad.hip) ----
         double2* cw = reinterpret_cast<double2*>(ws + o_cw);
         double2* lam = reinterpret_cast<double2*>(ws + o_lam);
@@ -778,26 +794,25 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         } else {
             aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
         }
-        // outputs of this chunk, host side: F of every (dof, rhs) pair
-        std::vector<double> hA(nw * ndof * ndof), hB(nw * ndof * ndof);
-        if (ndof) {
-            MH_HIP_CHECK(hipMemcpyAsync(hA.data(), Aw, sizeof(double) * hA.size(), hipMemcpyDeviceToHost, c->stream));
-            MH_HIP_CHECK(hipMemcpyAsync(hB.data(), Bw, sizeof(double) * hB.size(), hipMemcpyDeviceToHost, c->stream));
-        }
         const int64_t np = ga->nparam;
-        std::vector<double> gA(nw * np * ndof * ndof), gB(nw * np * ndof * ndof), gF(2 * nw * np * nbeta * ndof);
-        std::vector<double> hdk(nw * Pg), hdom(nw * Pg);
-        std::vector<double2> hT1(nw * nout), hT2(nw * nout);
+        // omega and wavenumber tangents of every parameter block; each block
+        // copies from its own slice, which lives until the chunk's final
+        // synchronisation
+        const int64_t nblocks = (np + Pg - 1) / Pg;
+        std::vector<double> hdk(nblocks * nw * Pg), hdom(nblocks * nw * Pg);
         for (int64_t m0 = 0; m0 < np; m0 += Pg) {
+            double* bdk = hdk.data() + (m0 / Pg) * nw * Pg;
+            double* bdom = hdom.data() + (m0 / Pg) * nw * Pg;
             for (int64_t w = 0; w < nw; ++w)
                 for (int m = 0; m < Pg; ++m) {
                     const double dw = (ga->domega && m0 + m < np) ? ga->domega[(m0 + m) * nomega + w0 + w] : 0.0;
-                    hdom[w * Pg + m] = dw;
-                    hdk[w * Pg + m] = 2.0 * omegas[w0 + w] / g * dw;
+                    bdom[w * Pg + m] = dw;
+                    bdk[w * Pg + m] = 2.0 * omegas[w0 + w] / g * dw;
                 }
-            MH_HIP_CHECK(hipMemcpyAsync(d_dk, hdk.data(), sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
-            MH_HIP_CHECK(hipMemcpyAsync(d_dom, hdom.data(), sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
-            TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
+            MH_HIP_CHECK(hipMemcpyAsync(d_dk, bdk, sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(d_dom, bdom, sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
+            const bool done = early_rk && m0 == 0;  // tangents and Rankine parts already there
+            if (!done) TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
             int64_t nblk = 0;
             if (wave_mask)
                 TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
@@ -806,45 +821,19 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             if (rank_mask) {
                 double* dR = reinterpret_cast<double*>(ws + o_dR);
                 double2* Yb = reinterpret_cast<double2*>(ws + o_Y);
-                TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, dR));
+                if (!done) TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, dR));
                 TRY(launch_rankine_contract(c, n, Pg, nw, ndof, nr, dR, aS, aDw, sS, sD, wa, wss, Yb, T2));
             }
             if (direct) TRY(join_aux(c));
             TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
-            MH_HIP_CHECK(hipMemcpyAsync(hT1.data(), T1, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));
-            MH_HIP_CHECK(hipMemcpyAsync(hT2.data(), T2, sizeof(double2) * nw * ndof * nr * Pg, hipMemcpyDeviceToHost, c->stream));
-            MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-            for (int64_t w = 0; w < nw; ++w) {
-                const double om = omegas[w0 + w];
-                for (int m = 0; m < Pg && m0 + m < np; ++m) {
-                    const double dw = hdom[w * Pg + m];
-                    for (int64_t d = 0; d < ndof; ++d)
-                        for (int64_t r = 0; r < nr; ++r) {
-                            const int64_t o = ((w * ndof + d) * nr + r) * Pg + m;
-                            const double fr = hT1[o].x + hT2[o].x, fi = hT1[o].y + hT2[o].y;
-                            if (r < ndof) {
-                                // A = Re F / omega^2, B = Im F / omega  (waves.jl:36)
-                                const int64_t ia = (w * ndof + r) * ndof + d;
-                                const double Fre = hA[ia] * om * om, Fim = hB[ia] * om;
-                                const int64_t og = (((w * np) + m0 + m) * ndof + r) * ndof + d;
-                                gA[og] = fr / (om * om) - 2.0 * Fre * dw / (om * om * om);
-                                gB[og] = fi / om - Fim * dw / (om * om);
-                            } else {
-                                const int64_t og = (((w * np) + m0 + m) * nbeta + (r - ndof)) * ndof + d;
-                                gF[2 * og] = fr;
-                                gF[2 * og + 1] = fi;
-                            }
-                        }
-                }
-            }
+            TRY(launch_grad_finish(c, nw, np, m0, Pg, ndof, nr, nbeta, d_om + w0, d_dom, Aw, Bw, T1, T2,
+                                   ga->dAout ? ga->dAout + w0 * np * ndof * ndof : nullptr,
+                                   ga->dBout ? ga->dBout + w0 * np * ndof * ndof : nullptr,
+                                   (ga->dFout && nbeta) ? ga->dFout + 2 * w0 * np * nbeta * ndof : nullptr));
         }
-        if (ga->dAout)
-            MH_HIP_CHECK(hipMemcpy(ga->dAout + w0 * np * ndof * ndof, gA.data(), sizeof(double) * gA.size(), hipMemcpyHostToDevice));
-        if (ga->dBout)
-            MH_HIP_CHECK(hipMemcpy(ga->dBout + w0 * np * ndof * ndof, gB.data(), sizeof(double) * gB.size(), hipMemcpyHostToDevice));
-        if (ga->dFout && nbeta)
-            MH_HIP_CHECK(hipMemcpy(ga->dFout + 2 * w0 * np * nbeta * ndof, gF.data(), sizeof(double) * gF.size(), hipMemcpyHostToDevice));
+        TRY(tri_check(c));
+        TRY(check_info(c, info, nw));
     }
     return MH_OK;
 }

@@ -918,4 +918,45 @@ int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout,
     return MH_OK;
 }
 
+// Gradients of A, B and F from the contracted pair terms T1 + T2 (dF of every
+// (dof d, right-hand side r) pair per parameter), the forces A, B and the
+// parameter's omega tangent dw: A = Re F / omega^2, B = Im F / omega
+// (src/waves.jl:36), so dA = Re dF / omega^2 - 2 A dw / omega and
+// dB = Im dF / omega - B dw / omega; right-hand sides r >= ndof are the
+// headings, whose dF is written as is.  One thread per (w, m, d, r).
+__global__ void k_grad_finish(int64_t nw, int64_t np, int64_t m0, int P, int64_t ndof, int64_t nr,
+                              int64_t nbeta, const double* __restrict__ om, const double* __restrict__ dom,
+                              const double* __restrict__ A, const double* __restrict__ B,
+                              const double2* __restrict__ T1, const double2* __restrict__ T2,
+                              double* __restrict__ gA, double* __restrict__ gB, double* __restrict__ gF) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw * P * ndof * nr) return;
+    const int64_t r = t % nr, d = (t / nr) % ndof, m = (t / (nr * ndof)) % P, w = t / (nr * ndof * P);
+    if (m0 + m >= np) return;
+    const int64_t o = ((w * ndof + d) * nr + r) * P + m;
+    const double fr = T1[o].x + T2[o].x, fi = T1[o].y + T2[o].y;
+    if (r < ndof) {
+        const double w_ = om[w], dw = dom[w * P + m];
+        const int64_t ia = (w * ndof + r) * ndof + d;
+        const int64_t og = ((w * np + m0 + m) * ndof + r) * ndof + d;
+        if (gA) gA[og] = fr / (w_ * w_) - 2.0 * A[ia] * dw / w_;
+        if (gB) gB[og] = fi / w_ - B[ia] * dw / w_;
+    } else if (gF) {
+        const int64_t og = ((w * np + m0 + m) * nbeta + (r - ndof)) * ndof + d;
+        gF[2 * og] = fr;
+        gF[2 * og + 1] = fi;
+    }
+}
+
+int launch_grad_finish(Context* ctx, int64_t nw, int64_t np, int64_t m0, int P, int64_t ndof, int64_t nr,
+                       int64_t nbeta, const double* d_om, const double* d_dom, const double* A, const double* B,
+                       const double2* T1, const double2* T2, double* gA, double* gB, double* gF) {
+    const int64_t tot = nw * P * ndof * nr;
+    if (tot == 0) return MH_OK;
+    hipLaunchKernelGGL(k_grad_finish, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, nw, np, m0,
+                       P, ndof, nr, nbeta, d_om, d_dom, A, B, T1, T2, gA, gB, gF);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
 }  // namespace mh

@@ -39,6 +39,26 @@ __device__ __forceinline__ C2 mkc(decltype(C2().x) a, decltype(C2().x) b) {
     return o;
 }
 
+// Raw buffer loads and stores of one complex entry: a per-lane byte offset
+// plus a wave-uniform one (scalar register), so strided accesses need no
+// 64-bit address registers, and a run of stores writes no address register
+// between them (a register that still holds a store's data is not rewritten,
+// so the stores are not made to wait for each other's completion).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float2) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, double2) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)vo, (int)so, 0);
+}
+
 
 template <class C2>
 __device__ __forceinline__ C2 cmul(C2 a, C2 b) {

@@ -209,5 +209,12 @@ int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan
                       const double2* rhoW, int64_t wstride, double2* out);
 int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout, const double2* part,
                            double2* out, int accumulate);
+// dA, dB (nw x nparam x ndof x ndof) and dF (complex, nw x nparam x nbeta x
+// ndof) of parameters m0 .. m0+P-1 from the pair terms T1 + T2, the forces A,
+// B of the same frequencies and the omega tangents dom (nw x P); null
+// outputs are skipped.
+int launch_grad_finish(Context* ctx, int64_t nw, int64_t np, int64_t m0, int P, int64_t ndof, int64_t nr,
+                       int64_t nbeta, const double* d_om, const double* d_dom, const double* A, const double* B,
+                       const double2* T1, const double2* T2, double* gA, double* gB, double* gF);
 
 }  // namespace mh

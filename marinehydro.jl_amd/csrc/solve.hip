@@ -1,12 +1,12 @@
-// Batched dense complex LU (partial pivoting) and triangular solves, FP64.
+// Batched dense complex LU (partial pivoting) and triangular solves, FP64 and FP32.
 //
 // Right-looking LU with block columns of NB = 128 (one batch item per
 // frequency).  Each block column is factored recursively (left half, update
 // of the right half, right half; as in LAPACK's zgetrf2) down to leaves of
-// 8 columns, and every update is a trailing GEMM on FP64 MFMA:
-//   k_lu_leaf      register-resident factorisation of an (n-k0) x <=8 panel,
-//                  1024 threads, pivot = max |re|+|im| (izamax semantics,
-//                  first index on ties)
+// 4-16 columns, and every update is a trailing GEMM on the matrix cores:
+//   k_lu_leaf      register-resident factorisation of an (n-k0) x <=16 panel,
+//                  one workgroup per matrix, pivot = max |re|+|im| (izamax
+//                  semantics, first index on ties)
 //   k_lu_swap      row interchanges of a pivot block on a column range
 //   k_lu_trsm      unit-lower L11 X = B on a <=32-row block, column range
 //   k_lu_gemm      C -= A B on arbitrary row/column/depth ranges, complex
@@ -89,7 +89,7 @@ __device__ __forceinline__ V wave_reduce(V x) {
 
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= kLeaf, held in
-// registers.  Thread t owns local rows t + 1024 q, q < R.  Per column: every
+// registers.  Thread t owns local rows t + NT q, q < R.  Per column: every
 // wave finds its largest |re|+|im| (first row on ties) and the lane owning
 // that row publishes the row with the wave's maximum; after one barrier
 // every wave picks the winner among the 16 candidates and reads its row.
@@ -99,12 +99,13 @@ __device__ __forceinline__ V wave_reduce(V x) {
 // final L values whose rows are only re-ordered, the right columns are not
 // updated before the interchanges of every earlier leaf have reached them.
 // ---------------------------------------------------------------------------
-template <class T, int R, int kLeaf>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
-                                                  int64_t stride, int64_t k0, int w,
-                                                  int64_t* __restrict__ ipiv,
-                                                  int* __restrict__ info, int64_t pc_lo, int64_t pc_hi) {
+template <class T, int R, int kLeaf, int NT>
+__global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
+                                                int64_t stride, int64_t k0, int w, int64_t* __restrict__ ipiv,
+                                                int* __restrict__ info, int64_t pc_lo, int64_t pc_hi) {
     typedef typename Cx<T>::t C2;
+    static_assert(NT == 256 || NT == 512 || NT == 1024, "4, 8 or 16 waves");
+    constexpr int NW = NT / 64;
     C2* M = A + (int64_t)blockIdx.x * stride;
     int64_t* piv = ipiv + (int64_t)blockIdx.x * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -112,20 +113,35 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
     // double-buffered by column parity: one barrier per column (a wave writes
     // buffer c & 1 again only after every wave has passed the barrier of
     // column c + 1, i.e. finished reading column c)
-    __shared__ T s_v[2][16];
-    __shared__ int s_i[2][16];
-    __shared__ C2 s_cand[2][16][kLeaf];           // each wave's candidate pivot row
+    __shared__ T s_v[2][NW];
+    __shared__ int s_i[2][NW];
+    __shared__ C2 s_cand[2][NW][kLeaf];           // each wave's candidate pivot row
     __shared__ C2 s_crow[2][kLeaf];
     __shared__ int s_p[kLeaf];                    // local pivot rows
     __shared__ int s_rows[2 * kLeaf], s_src[2 * kLeaf];
+    // buffer addressing from the panel's first row M + k0: entry (k0 + r,
+    // col) at byte (r + col ld) sizeof(C2) (the host checks the extent fits
+    // 31 bits); rows outside the panel load from offset 0 and are zeroed
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        M + k0, (short)0, (int)(((int64_t)(m - 1) + (n - 1) * ld + 1) * (int64_t)sizeof(C2)), 0x00020000);
     C2 a[R][kLeaf];
+    bool any_zero = false;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int r = tid + 1024 * q;
+        const int r = tid + NT * q;
+        const uint32_t vo = r < m ? (uint32_t)r * (uint32_t)sizeof(C2) : 0u;
 #pragma unroll
-        for (int c = 0; c < kLeaf; ++c)
-            a[q][c] = (r < m && c < w) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
+        for (int c = 0; c < kLeaf; ++c) {
+            a[q][c] = mkc<C2>(T(0), T(0));
+            if (c < w) {  // uniform
+                const C2 v = buf_load(rs, vo, (uint32_t)((k0 + c) * ld) * (uint32_t)sizeof(C2), C2());
+                if (r < m) a[q][c] = v;
+            }
+        }
     }
+    // the panel is in registers: nothing of it stays outstanding in the
+    // vector memory counter, which later stores share
+    __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
     for (int c = 0; c < kLeaf; ++c) {
         if (c < w) {  // uniform
@@ -134,7 +150,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
         int bi = 0x7fffffff;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-            const int r = tid + 1024 * q;
+            const int r = tid + NT * q;
             if (r >= c && r < m) {
                 const T v = fabs(a[q][c].x) + fabs(a[q][c].y);
                 if (v > best) {
@@ -143,26 +159,16 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
                 }
             }
         }
-        // the wave's maximum, then the first row that attains it
+        // the wave's maximum, then the first row that attains it; the owner
+        // of that row publishes it
         const T wv = wave_reduce<true>(best);
         const int wi = wave_reduce<false>(best == wv ? bi : 0x7fffffff);
-        {  // the owner of row wi publishes it (selects keep register indices static)
-            bool own = false;
 #pragma unroll
-            for (int q = 0; q < R; ++q) own = own || (tid + 1024 * q == wi);
-            if (own) {
+        for (int q = 0; q < R; ++q)
+            if (tid + NT * q == wi) {
 #pragma unroll
-                for (int j = 0; j < kLeaf; ++j) {
-                    C2 v = a[0][j];
-#pragma unroll
-                    for (int q = 1; q < R; ++q) {
-                        const bool o = tid + 1024 * q == wi;
-                        v = mkc<C2>(o ? a[q][j].x : v.x, o ? a[q][j].y : v.y);
-                    }
-                    s_cand[bf][wave][j] = v;
-                }
+                for (int j = 0; j < kLeaf; ++j) s_cand[bf][wave][j] = a[q][j];
             }
-        }
         if (lane == 0) {
             s_v[bf][wave] = wv;
             s_i[bf][wave] = wi;
@@ -172,58 +178,73 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
             for (int j = 0; j < kLeaf; ++j) s_crow[bf][j] = a[0][j];
         }
         __syncthreads();
-        // every wave reduces the 16 wave maxima itself (first row on ties)
-        const T bv = s_v[bf][lane & 15];
-        const int biv = s_i[bf][lane & 15];
+        // every wave reduces the wave maxima itself (first row on ties)
+        const T bv = s_v[bf][lane % NW];
+        const int biv = s_i[bf][lane % NW];
         const T bm = row_reduce<true>(bv);
         const int pm = row_reduce<false>(bv == bm ? biv : 0x7fffffff);
         const T b = read_lane(bm, 0);
         int p = read_lane(pm, 0);
         const bool zero = !(b > T(0));  // exact zero pivot: record, no interchange, no update
         if (zero) p = c;
-        const C2* prow = s_cand[bf][(p & 1023) >> 6];  // the wave that owns row p
-        if (tid == 0) {
-            if (zero) atomicOr(info + blockIdx.x, 1);
-            piv[k0 + c] = k0 + p;
-            s_p[c] = p;
-        }
-        if (p != c) {
+        const int po = p % NT, pq = p / NT;  // owner thread and register slot of row p
+        const C2* prow = s_cand[bf][po >> 6];
+        // pivots and the zero-pivot flag stay on chip until the loop ends: a
+        // global store here would be waited for (the vector memory counter
+        // also counts stores) wherever a register still waits for its load
+        if (tid == 0) s_p[c] = p;
+        any_zero = any_zero || zero;
+        if (p != c) {  // uniform: rows c and p exchange
             if (tid == c) {
 #pragma unroll
                 for (int j = 0; j < kLeaf; ++j) a[0][j] = prow[j];
             }
+            if (tid == po) {
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const bool own = tid + 1024 * q == p;
+                for (int q = 0; q < R; ++q)
+                    if (q == pq) {
 #pragma unroll
-                for (int j = 0; j < kLeaf; ++j) {
-                    const C2 cr = s_crow[bf][j];
-                    a[q][j] = mkc<C2>(own ? cr.x : a[q][j].x, own ? cr.y : a[q][j].y);
-                }
+                        for (int j = 0; j < kLeaf; ++j) a[q][j] = s_crow[bf][j];
+                    }
             }
         }
         if (!zero) {
             const C2 inv = crecip(prow[c]);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const int r = tid + 1024 * q;
+                const int r = tid + NT * q;
                 if (r > c && r < m) {
                     const C2 l = cmul(a[q][c], inv);
                     a[q][c] = l;
 #pragma unroll
                     for (int j = 1; j < kLeaf; ++j)  // constant trip count: unrolled before c is known
-                        if (j > c) a[q][j] = cfms(a[q][j], l, prow[j]);
+                        if (j > c) {  // a -= l u as four fused multiply-adds
+                            const C2 u = prow[j];
+                            T x = fma(-l.x, u.x, a[q][j].x), y = fma(-l.x, u.y, a[q][j].y);
+                            a[q][j] = mkc<C2>(fma(l.y, u.y, x), fma(-l.y, u.x, y));
+                        }
                 }
             }
         }
         }
     }
+    if (tid < w) piv[k0 + tid] = k0 + s_p[tid];  // s_p written by lane 0 of this wave
+    if (tid == 0 && any_zero) atomicOr(info + blockIdx.x, 1);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int r = tid + 1024 * q;
+        const int r = tid + NT * q;
+        if (r < m) {
+            const uint32_t vo = (uint32_t)r * (uint32_t)sizeof(C2);
+            if (w == kLeaf) {  // uniform; straight-line stores
 #pragma unroll
-        for (int c = 0; c < kLeaf; ++c)
-            if (r < m && c < w) M[(k0 + r) + (k0 + c) * ld] = a[q][c];
+                for (int c = 0; c < kLeaf; ++c)
+                    buf_store(a[q][c], rs, vo, (uint32_t)((k0 + c) * ld) * (uint32_t)sizeof(C2));
+            } else {
+#pragma unroll
+                for (int c = 0; c < kLeaf; ++c)
+                    if (c < w) buf_store(a[q][c], rs, vo, (uint32_t)((k0 + c) * ld) * (uint32_t)sizeof(C2));
+            }
+        }
     }
     // interchanges on the block column's other columns: the net permutation
     // of the rows they touch (value of row s_rows[s] <- row s_src[s]; slot
@@ -262,31 +283,32 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(1, 4))) vo
     __syncthreads();
     const int nr = 2 * w;
     const int items = (nleft + nright) * nr;  // <= kNBOuter * 2 * kLeaf
-    constexpr int kItems = (kNBOuter * 2 * kLeaf + 1023) / 1024;
+    constexpr int kItems = (kNBOuter * 2 * kLeaf + NT - 1) / NT;
     C2 g[kItems];
+    uint32_t vd[kItems];  // destination byte offsets, computed before any store
 #pragma unroll
     for (int e = 0; e < kItems; ++e) {
-        const int it = tid + 1024 * e;
+        const int it = tid + NT * e;
+        vd[e] = 0xffffffffu;  // nothing to store
+        g[e] = mkc<C2>(T(0), T(0));
         if (it < items) {
             const int ci = it / nr, si = it % nr;
             const int64_t col = ci < nleft ? pc_lo + ci : k0 + w + (ci - nleft);
-            const int sr = s_src[si];
-            if (sr >= 0) g[e] = M[(k0 + sr) + col * ld];
+            const int sr = s_src[si], rr = s_rows[si];
+            const uint32_t cb = (uint32_t)(col * ld) * (uint32_t)sizeof(C2);
+            if (sr >= 0 && rr >= 0) {
+                g[e] = buf_load(rs, (uint32_t)sr * (uint32_t)sizeof(C2) + cb, 0u, C2());
+                vd[e] = (uint32_t)rr * (uint32_t)sizeof(C2) + cb;
+            }
         }
     }
     __builtin_amdgcn_s_waitcnt(0);  // every gather has returned before any row is overwritten
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < kItems; ++e) {
-        const int it = tid + 1024 * e;
-        if (it < items) {
-            const int ci = it / nr, si = it % nr;
-            const int64_t col = ci < nleft ? pc_lo + ci : k0 + w + (ci - nleft);
-            const int rr = s_rows[si];
-            if (rr >= 0) M[(k0 + rr) + col * ld] = g[e];
-        }
-    }
+    for (int e = 0; e < kItems; ++e)
+        if (vd[e] != 0xffffffffu) buf_store(g[e], rs, vd[e], 0u);
 }
+
 
 // ---------------------------------------------------------------------------
 // row interchanges of pivots [k0, k0+npiv) on columns [c_lo, c_hi)
@@ -840,33 +862,33 @@ static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t 
     return MH_OK;
 }
 
-// Leaf width for a panel of m rows: the leaf keeps R x width complex values
-// in registers per thread (R rows of 1024 threads, so at most 128 VGPRs per
-// thread): 64 VGPRs of panel values and no spills.
+// Leaf shape for a panel of m rows: 512 threads (two waves per SIMD, so the
+// per-wave work of a column, its reductions and reciprocal, is paid by 8
+// waves instead of 16), R = 2, 4 or 8 rows per thread and a width that keeps
+// the panel in 128 VGPRs per thread (no spills).
+constexpr int kLeafNT = 512;
 template <class T>
 static int leaf_width(int64_t m) {
-    const int R = m <= 1024 ? 1 : (m <= 2048 ? 2 : 4);
-    if (sizeof(T) == 4) return R <= 2 ? 16 : 8;
-    return R == 1 ? 16 : (R == 2 ? 8 : 4);  // panels of 64 VGPRs per thread
+    const int R = m <= 1024 ? 2 : (m <= 2048 ? 4 : 8);
+    if (sizeof(T) == 4) return R <= 4 ? 16 : 8;
+    return R == 2 ? 16 : (R == 4 ? 8 : 4);
 }
 
 template <class T>
 static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
     const int64_t m = P.n - k0;
     ProfScope ps(P.ctx, "lu_leaf", 8.0 * (double)m * w * w / 2.0 * P.batch, 32.0 * (double)m * w * P.batch);
-    const int W = leaf_width<T>(m);
-#define LEAF(RR, WW)                                                                                        \
-    hipLaunchKernelGGL((k_lu_leaf<T, RR, WW>), dim3((unsigned)P.batch), dim3(1024), 0, P.ctx->stream, P.n, P.A, \
-                       P.ld, P.stride, k0, w, P.ipiv, P.info, P.pc_lo, P.pc_hi)
-    (void)W;
+#define LEAF(RR, WW)                                                                                   \
+    hipLaunchKernelGGL((k_lu_leaf<T, RR, WW, kLeafNT>), dim3((unsigned)P.batch), dim3(kLeafNT), 0,     \
+                       P.ctx->stream, P.n, P.A, P.ld, P.stride, k0, w, P.ipiv, P.info, P.pc_lo, P.pc_hi)
     if (m <= 1024) {
-        LEAF(1, 16);
+        LEAF(2, 16);
     } else if (m <= 2048) {
-        if constexpr (sizeof(T) == 4) LEAF(2, 16);
-        else LEAF(2, 8);
+        if constexpr (sizeof(T) == 4) LEAF(4, 16);
+        else LEAF(4, 8);
     } else {
-        if constexpr (sizeof(T) == 4) LEAF(4, 8);
-        else LEAF(4, 4);
+        if constexpr (sizeof(T) == 4) LEAF(8, 8);
+        else LEAF(8, 4);
     }
 #undef LEAF
     return MH_OK;
@@ -906,6 +928,7 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
     MH_REQUIRE(sizeof(T) == 8 || (ncols * ld) * 8 <= 0x7fffffff, "lu: FP32 matrix extent above 2 GiB");
+    MH_REQUIRE(n * ld * (int64_t)sizeof(typename Cx<T>::t) <= 0x7fffffff, "lu: matrix extent above 2 GiB");
     LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info, 0, 0};
     const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
     TRY_RC(ensure_aux(ctx));

@@ -87,16 +87,6 @@ __device__ __forceinline__ void pub_store(double2* p, double2 v) {
     st_pub(reinterpret_cast<double*>(p) + 1, b);
 }
 
-// Raw buffer loads of one complex entry: a per-lane byte offset plus a
-// wave-uniform one (scalar register), so a row's 32 strided loads need no
-// 64-bit address registers.
-__device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float2) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
-}
-__device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, double2) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0));
-}
-
 // A row's slice of block c (logical columns 32c .. 32c+31) for rows after
 // the block, and its application with the block's solution x.  Every load
 // stays inside the matrix (offsets of rows/columns outside it are replaced

@@ -405,6 +405,32 @@ def FroudeKrylovForce(mesh, omega, dof, beta=0.0):
 # ---------------------------------------------------------------------------
 # batched end-to-end sweep and the reference's two physics entry points
 # ---------------------------------------------------------------------------
+class _Outputs:
+    """Device outputs of a sweep in one zero-filled float64 buffer: one fill
+    and one read-back instead of one per array.  Every segment starts at an
+    even offset, so complex segments are 16-byte aligned."""
+
+    def __init__(self, torch, dev, parts):
+        self.offs, tot = {}, 0
+        for name, shape, cplx in parts:
+            cnt = int(np.prod(shape)) * (2 if cplx else 1)
+            self.offs[name] = (tot, shape, cplx)
+            tot += cnt + (cnt & 1)
+        self.buf = torch.zeros(max(tot, 2), dtype=torch.float64, device=dev)
+        self.host = None
+
+    def ptr(self, name):
+        return self.buf.data_ptr() + 8 * self.offs[name][0]
+
+    def get(self, name):
+        if self.host is None:
+            self.host = self.buf.cpu().numpy()
+        off, shape, cplx = self.offs[name]
+        cnt = int(np.prod(shape)) * (2 if cplx else 1)
+        a = self.host[off:off + cnt]
+        return (a.view(np.complex128) if cplx else a).reshape(shape)
+
+
 def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=True,
                precision="fp64", device=0, g=None, rho=None):
     """All frequencies in one batched pass (one LU per frequency, shared by
@@ -426,20 +452,20 @@ def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=
     dev = f"cuda:{dm.device}"
     dofs = torch.from_numpy(np.ascontiguousarray(
         np.stack([f.T for f in fields]) if ndof else np.zeros((1, 3, n)))).to(dev)
-    A = torch.zeros((nw, max(ndof, 1), max(ndof, 1)), dtype=torch.float64, device=dev)
-    B = torch.zeros_like(A)
-    F = torch.zeros((nw, max(nbeta, 1), max(ndof, 1)), dtype=torch.complex128, device=dev)
+    nd = max(ndof, 1)
+    out = _Outputs(torch, dev, [("A", (nw, nd, nd), False), ("B", (nw, nd, nd), False),
+                                ("F", (nw, max(nbeta, 1), nd), True)])
     prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
     ctx = L.context(dm.device)
     L.check(L.lib().mh_freq_sweep(ctx, C.byref(dm.struct), gf_mask(gfs), int(bool(direct)), prec,
                                   nw, om.ctypes.data, float(g or SETTINGS.g),
                                   float(rho or SETTINGS.rho), ndof, dofs.data_ptr(), nbeta,
-                                  be.ctypes.data if nbeta else None, A.data_ptr(), B.data_ptr(),
-                                  F.data_ptr()), "freq_sweep")
+                                  be.ctypes.data if nbeta else None, out.ptr("A"), out.ptr("B"),
+                                  out.ptr("F")), "freq_sweep")
     # library layout A[w][j][i] -> A[w, i, j]
-    A_ = A.transpose(1, 2).cpu().numpy()[:, :ndof, :ndof]
-    B_ = B.transpose(1, 2).cpu().numpy()[:, :ndof, :ndof]
-    F_ = F.cpu().numpy()[:, :nbeta, :ndof]
+    A_ = np.ascontiguousarray(out.get("A").transpose(0, 2, 1)[:, :ndof, :ndof])
+    B_ = np.ascontiguousarray(out.get("B").transpose(0, 2, 1)[:, :ndof, :ndof])
+    F_ = np.ascontiguousarray(out.get("F")[:, :nbeta, :ndof])
     return A_, B_, F_
 
 
@@ -544,12 +570,10 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
     ddof = up(T.ddof_fields, (npar, ndof, n, 3))
     dom = None if T.domegas is None else np.ascontiguousarray(np.asarray(T.domegas, dtype=np.float64).reshape(npar, nw))
     dofs = torch.from_numpy(np.ascontiguousarray(np.stack([f.T for f in fields]))).to(dev)
-    A = torch.zeros((nw, ndof, ndof), dtype=torch.float64, device=dev)
-    B = torch.zeros_like(A)
-    F = torch.zeros((nw, max(nbeta, 1), ndof), dtype=torch.complex128, device=dev)
-    gA = torch.zeros((nw, max(npar, 1), ndof, ndof), dtype=torch.float64, device=dev)
-    gB = torch.zeros_like(gA)
-    gF = torch.zeros((nw, max(npar, 1), max(nbeta, 1), ndof), dtype=torch.complex128, device=dev)
+    mp, nb = max(npar, 1), max(nbeta, 1)
+    out = _Outputs(torch, dev, [("A", (nw, ndof, ndof), False), ("B", (nw, ndof, ndof), False),
+                                ("F", (nw, nb, ndof), True), ("gA", (nw, mp, ndof, ndof), False),
+                                ("gB", (nw, mp, ndof, ndof), False), ("gF", (nw, mp, nb, ndof), True)])
     ptr = lambda t: t.data_ptr() if t is not None else None
     prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
     ctx = L.context(dm.device)
@@ -557,12 +581,12 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
         ctx, C.byref(dm.struct), gf_mask(gfs), int(bool(direct)), prec, nw, om.ctypes.data,
         float(g or SETTINGS.g), float(rho or SETTINGS.rho), ndof, dofs.data_ptr(), nbeta,
         be.ctypes.data if nbeta else None, npar, ptr(dV), ptr(dC), ptr(dN), ptr(dA_), ptr(ddof),
-        dom.ctypes.data if dom is not None else None, A.data_ptr(), B.data_ptr(), F.data_ptr(),
-        gA.data_ptr(), gB.data_ptr(), gF.data_ptr()), "freq_sweep_grad")
-    A_ = A.transpose(1, 2).cpu().numpy()
-    B_ = B.transpose(1, 2).cpu().numpy()
-    F_ = F.cpu().numpy()[:, :nbeta, :]
-    gA_ = gA.transpose(2, 3).cpu().numpy()[:, :npar]
-    gB_ = gB.transpose(2, 3).cpu().numpy()[:, :npar]
-    gF_ = gF.cpu().numpy()[:, :npar, :nbeta, :]
+        dom.ctypes.data if dom is not None else None, out.ptr("A"), out.ptr("B"), out.ptr("F"),
+        out.ptr("gA"), out.ptr("gB"), out.ptr("gF")), "freq_sweep_grad")
+    A_ = np.ascontiguousarray(out.get("A").transpose(0, 2, 1))
+    B_ = np.ascontiguousarray(out.get("B").transpose(0, 2, 1))
+    F_ = np.ascontiguousarray(out.get("F")[:, :nbeta, :])
+    gA_ = np.ascontiguousarray(out.get("gA").transpose(0, 1, 3, 2)[:, :npar])
+    gB_ = np.ascontiguousarray(out.get("gB").transpose(0, 1, 3, 2)[:, :npar])
+    gF_ = np.ascontiguousarray(out.get("gF")[:, :npar, :nbeta, :])
     return A_, B_, F_, gA_, gB_, gF_
