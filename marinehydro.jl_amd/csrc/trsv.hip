@@ -119,6 +119,33 @@ __device__ __forceinline__ void trsv_apply(const C2 (&m)[kTsCB], C2 (&v)[NR], in
     }
 }
 
+// Inverse of the diagonal block of logical block cc (physical block K,
+// element (a, b) at Inv[((item * nsubp + K) * 32 + b) * 32 + a]): fetched
+// into registers one block ahead, staged into LDS when its turn comes.
+// The parts are held as separate real arrays: an array of HIP's vector
+// structs carried around the block loop stays in scratch memory.
+constexpr int kTsIE = kTsCB * kTsCB / kTsRows;
+template <bool REV, class T, class C2>
+__device__ __forceinline__ void fetch_inv(T (&ibx)[kTsIE], T (&iby)[kTsIE], const C2* __restrict__ Inv, int64_t item,
+                                          int nsubp, int cc, int tid) {
+    const int64_t K = REV ? nsubp - 1 - cc : cc;
+    const C2* src = Inv + (item * nsubp + K) * kTsCB * kTsCB;
+#pragma unroll
+    for (int e = 0; e < kTsIE; ++e) {
+        const C2 v = src[tid + kTsRows * e];
+        ibx[e] = v.x;
+        iby[e] = v.y;
+    }
+}
+template <class T, class C2>
+__device__ __forceinline__ void stash_inv(C2 (*dst)[kTsCB + 1], const T (&ibx)[kTsIE], const T (&iby)[kTsIE], int tid) {
+#pragma unroll
+    for (int e = 0; e < kTsIE; ++e) {
+        const int x = tid + kTsRows * e;
+        dst[x % kTsCB][x / kTsCB] = mkc<C2>(ibx[e], iby[e]);
+    }
+}
+
 // TR: transposed solve (inverse blocks read transposed); TRM: the matrix is
 // read transposed (false when A is an explicit transposed copy).
 template <class T, bool TR, bool TRM, bool REV, int NR>
@@ -141,7 +168,12 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
         s_bad[1] = 0u;
     }
     __syncthreads();
-    const int64_t tk = (int64_t)s_tk;
+    // values read from LDS are wave-uniform but not known to be so: read them
+    // back as scalars, so the indices derived from them (and the scalar
+    // offsets of the buffer loads) stay in scalar registers
+    const unsigned long long tk_lds = s_tk;
+    const int64_t tk = (int64_t)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(tk_lds >> 32)) << 32) |
+                                 (unsigned)__builtin_amdgcn_readfirstlane((unsigned)tk_lds));
     if (tk >= (int64_t)nslab * batch) return;  // cannot happen with a reset ticket
     const int slab = (int)(tk % nslab);
     const int64_t item = tk / nslab;
@@ -154,7 +186,6 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
     const int* sp = sperm ? sperm + item * n : nullptr;
     const C2* bin = in + item * istride;
     C2* bout = out + item * ostride;
-    const C2 zero = mkc<C2>(T(0), T(0));
     auto phys = [&](int64_t i) -> int64_t { return REV ? np - 1 - i : i; };
     auto dst_of = [&](int64_t p) -> int64_t { return sp ? (int64_t)sp[p] : p; };
 
@@ -169,32 +200,16 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
     const int cown = (int)(i0 / kTsCB);
     const int cend = (cown + kTsOwn) < nsubp ? cown + kTsOwn : nsubp;
 
-    // inverse of own block s (physical block K, element (a, b) at
-    // Inv[((item * nsubp + K) * 32 + b) * 32 + a]): fetched into registers
-    // one block ahead, staged into LDS buffer s & 1 when its turn comes
-    constexpr int kIE = kTsCB * kTsCB / kTsRows;
-    C2 ib[kIE];
-    auto fetch_inv = [&](int s) {
-        const int cc = cown + s;
-        const int64_t K = REV ? nsubp - 1 - cc : cc;
-        const C2* src = Inv + (item * nsubp + K) * kTsCB * kTsCB;
-#pragma unroll
-        for (int e = 0; e < kIE; ++e) ib[e] = src[tid + kTsRows * e];
-    };
-    auto stash_inv = [&](int s) {
-#pragma unroll
-        for (int e = 0; e < kIE; ++e) {
-            const int x = tid + kTsRows * e;
-            sI[s & 1][x % kTsCB][x / kTsCB] = ib[e];
-        }
-    };
-    if (cend > cown) fetch_inv(0);
+    // inverse of own block s: fetched one block ahead, staged into LDS
+    // buffer s & 1 when its turn comes
+    T ibx[kTsIE], iby[kTsIE];
+    if (cend > cown) fetch_inv<REV>(ibx, iby, Inv, item, nsubp, cown, tid);
 
     C2 v[NR];
     {
         const int64_t src = valid ? (gp ? (int64_t)gp[pi] : pi) : 0;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) v[r] = (valid && r < nrhs) ? bin[src + r * ldi] : zero;
+        for (int r = 0; r < NR; ++r) v[r] = (valid && r < nrhs) ? bin[src + r * ldi] : mkc<C2>(T(0), T(0));
     }
 
     // the row's slices are consumed in block order 0, 1, ..., cend-1: apply
@@ -223,7 +238,7 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
                 for (int e = lane; e < kTsCB * NR; e += 64) {
                     const int q = e % kTsCB, r = e / kTsCB;
                     const int64_t pj = phys((int64_t)c * kTsCB + q);
-                    C2 val = zero;
+                    C2 val = mkc<C2>(T(0), T(0));
                     if (r < nrhs && pj < n) ok = pub_load(bout + dst_of(pj) + r * ldo, val) && ok;
                     sx[0][q][r] = val;
                 }
@@ -246,14 +261,14 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
                     continue;
                 }
                 const int64_t pj = phys((int64_t)(c + b) * kTsCB + q);
-                C2 val = zero;
+                C2 val = mkc<C2>(T(0), T(0));
                 if (r < nrhs && pj < n && !pub_load(bout + dst_of(pj) + r * ldo, val)) bad |= 1u << b;
                 sx[b][q][r] = val;
             }
             if (bad) atomicOr(&s_bad[par], bad);
         }
         __syncthreads();
-        const int nb = __builtin_ctz(s_bad[par] | (1u << kTsBatch));
+        const int nb = __builtin_amdgcn_readfirstlane(__builtin_ctz(s_bad[par] | (1u << kTsBatch)));
         for (int b = 0; b < nb; ++b) MH_TRSV_STEP(c + b, sx[b]);
         c += nb;
         par ^= 1;
@@ -269,15 +284,15 @@ __global__ __launch_bounds__(kTsRows) __attribute__((amdgpu_waves_per_eu(2))) vo
 #pragma unroll
             for (int r = 0; r < NR; ++r) sr[tid - base][r] = v[r];
         }
-        stash_inv(s);
-        if (s + 1 < cend - cown) fetch_inv(s + 1);
+        stash_inv(sI[s & 1], ibx, iby, tid);
+        if (s + 1 < cend - cown) fetch_inv<REV>(ibx, iby, Inv, item, nsubp, cown + s + 1, tid);
         __syncthreads();
         if (mine) {
             const int a = tid - base;
             const int pa = REV ? kTsCB - 1 - a : a;
             C2 acc[NR];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] = zero;
+            for (int r = 0; r < NR; ++r) acc[r] = mkc<C2>(T(0), T(0));
 #pragma unroll 8
             for (int b = 0; b < kTsCB; ++b) {
                 const int pb = REV ? kTsCB - 1 - b : b;
