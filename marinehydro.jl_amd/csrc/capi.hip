@@ -488,8 +488,8 @@ struct IrSystem {
 // sweeps typically reach 1e-11 and the results agree with the FP64 path to
 // ~1e-10, well inside the 1e-6 the mixed-precision mode is specified for.
 constexpr double kIrTol = 1e-10;
-// LUt32 (optional): space for the transposed factors; the first transposed
-// system fills it (on its stream) and solves with coalesced reads.
+// LUt32 (optional): the transposed factors (built during the LU); transposed
+// systems solve from it with coalesced reads.
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
                     int nsys, float2* LUt32) {
@@ -501,7 +501,6 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
     for (int k = 0; k < nsys; ++k) {
         OnAux on(c, k == 1);
         const IrSystem& q = sys[k];
-        if (q.trans && LUt32) TRY(tri_transpose_f32(c, n, batch, LU32, n, stride32, LUt32));
         MH_HIP_CHECK(hipMemsetAsync(q.X, 0, sizeof(double2) * (size_t)(q.bstride * batch), c->stream));
         MH_HIP_CHECK(hipMemcpyAsync(q.R, q.Bm, sizeof(double2) * (size_t)(q.bstride * batch),
                                     hipMemcpyDeviceToDevice, c->stream));
@@ -685,7 +684,9 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
             }
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
-            TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info));
+            // with an adjoint system, the transposed factors are built during the LU
+            float2* Dt32 = (ga && ndof) ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr;
+            TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info, Dt32));
             TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
             // forward system: phi = D \ (S bc), or sigma = K \ bc (kept in ybuf)
             IrSystem sys[2];
@@ -729,7 +730,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         }
         if (!mixed) {
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
-            TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info));
+            double2* dDt = (ga && ndof) ? reinterpret_cast<double2*>(ws + o_Dt) : nullptr;
+            TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info, dDt));  // + transposed factors
             TRY(tri_prepare(c, n, nw, dD, ld, stride, piv, (ga && ndof) ? perm : nullptr, inv));
             if (ga && ndof) {  // adjoint solve on aux, alongside the back substitution
                 double2* cw = reinterpret_cast<double2*>(ws + o_cw);
@@ -741,8 +743,6 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                     MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
                 else  // lambda = K^-T S^T c
                     TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
-                double2* dDt = reinterpret_cast<double2*>(ws + o_Dt);
-                TRY(tri_transpose(c, n, nw, dD, ld, stride, dDt));
                 TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa, dDt));
             }
             TRY(tri_solve(c, n, nw, dD, ld, stride, nullptr, inv, 0, 2, nr, aug, ld, stride));

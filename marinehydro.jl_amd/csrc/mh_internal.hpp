@@ -121,8 +121,10 @@ int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const d
                      double2* out);
 
 // ---- launchers (solve.hip) --------------------------------------------------
+// LUt (optional): the transposed factors (n x n, same ld and stride), each
+// block row transposed as soon as it is final, alongside the factorisation.
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
-              int64_t stride, int64_t* ipiv, int* d_info);
+              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt = nullptr);
 // ---- launchers (trsv.hip) ---------------------------------------------------
 constexpr int kMaxSolveN = 4096;
 // After an LU factorisation: perm[b*n + i], the net row permutation of the
@@ -135,11 +137,12 @@ int tri_prepare(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64
 int tri_prepare_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
                     const int64_t* ipiv, int* perm, float2* inv);
 // LUt = LU^T per item (same ld and stride), for transposed solves that read
-// the factors row-wise.
+// the factors row-wise; only rows [r_lo, r_hi) of LU (columns of LUt) when
+// given (r_hi < 0: to n).
 int tri_transpose(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
-                  double2* LUt);
+                  double2* LUt, int64_t r_lo = 0, int64_t r_hi = -1);
 int tri_transpose_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
-                      float2* LUt);
+                      float2* LUt, int64_t r_lo = 0, int64_t r_hi = -1);
 // Solves with the factors of A = P L U, in place on B (n x nrhs per item):
 // trans 0: A X = B, 1: A^T X = B; part 1 = L (with P), 2 = U, 3 = both.
 // LUt (optional): tri_transpose of LU, used by the transposed solves.
@@ -164,7 +167,7 @@ int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
                      double2* Y, int64_t ldy, int64_t ystride, const double2* Z = nullptr);
 // FP32 factorisation and solves (mixed precision)
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
-                  int64_t stride, int64_t* ipiv, int* d_info);
+                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt = nullptr);
 int convert_to_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* src,
                    int64_t lds, int64_t sstride, float2* dst, int64_t ldd, int64_t dstride);
 int add_from_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const float2* d,

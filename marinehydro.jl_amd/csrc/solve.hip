@@ -921,9 +921,22 @@ static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
 // after every update that reads those columns has completed.  With the
 // kernel timer on, the same schedule runs on one stream (each timed kernel
 // alone on the device).
+static int transpose_rows(Context* c, int64_t n, int64_t batch, const double2* A, int64_t ld, int64_t stride,
+                          double2* At, int64_t r_lo, int64_t r_hi) {
+    return tri_transpose(c, n, batch, A, ld, stride, At, r_lo, r_hi);
+}
+static int transpose_rows(Context* c, int64_t n, int64_t batch, const float2* A, int64_t ld, int64_t stride,
+                          float2* At, int64_t r_lo, int64_t r_hi) {
+    return tri_transpose_f32(c, n, batch, A, ld, stride, At, r_lo, r_hi);
+}
+
+//
+// With LUt, block row k of the factors (final once panel k, its narrow and
+// its wide update are done: later interchanges only touch rows below it) is
+// transposed into block column k of LUt on the side stream.
 template <class T>
 static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, typename Cx<T>::t* A,
-                       int64_t ld, int64_t stride, int64_t* ipiv, int* d_info) {
+                       int64_t ld, int64_t stride, int64_t* ipiv, int* d_info, typename Cx<T>::t* LUt) {
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
@@ -932,11 +945,14 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info, 0, 0};
     const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
     TRY_RC(ensure_aux(ctx));
-    TRY_RC(ensure_events(ctx, (size_t)(2 * npan + 1)));
-    hipEvent_t* evF = ctx->ev_pool.data();         // panel k factored (stream)
-    hipEvent_t* evB = ctx->ev_pool.data() + npan;  // wide update k done (aux)
-    hipEvent_t ev0 = ctx->ev_pool[2 * npan];
+    TRY_RC(ensure_events(ctx, (size_t)(3 * npan + 2)));
+    hipEvent_t* evF = ctx->ev_pool.data();             // panel k factored (stream)
+    hipEvent_t* evB = ctx->ev_pool.data() + npan;      // wide update k done (aux)
+    hipEvent_t* evN = ctx->ev_pool.data() + 2 * npan;  // narrow update k done (stream)
+    hipEvent_t ev0 = ctx->ev_pool[3 * npan];
+    hipEvent_t evT = ctx->ev_pool[3 * npan + 1];       // last transposed block row (aux)
     const hipStream_t sA = ctx->stream, sB = ctx->profiling ? ctx->stream : ctx->aux;
+    const hipStream_t sT = ctx->profiling ? ctx->stream : ctx->side;
     MH_HIP_CHECK(hipEventRecord(ev0, sA));  // aux starts after the work already issued
     MH_HIP_CHECK(hipStreamWaitEvent(sB, ev0, 0));
     int rc = MH_OK;
@@ -969,23 +985,35 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
             if (rc == MH_OK) rc = lu_trsm(P, k0, w, kn0, cw0);
             if (rc == MH_OK) rc = lu_gemm(P, kn0, n, kn0, cw0, k0, w);
         }
+        if (LUt && rc == MH_OK) {  // block row k is final: transpose it on the side stream
+            MH_HIP_CHECK(hipEventRecord(evN[k], sA));
+            MH_HIP_CHECK(hipStreamWaitEvent(sT, evN[k], 0));
+            MH_HIP_CHECK(hipStreamWaitEvent(sT, evB[k], 0));
+            ctx->stream = sT;
+            rc = transpose_rows(ctx, n, batch, A, ld, stride, LUt, k0, k0 + w);
+            ctx->stream = sA;
+        }
     }
     ctx->stream = sA;
-    // stream continues after all of aux's updates
+    // stream continues after all of aux's updates (and the transposes)
     MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[npan - 1], 0));
+    if (LUt) {
+        MH_HIP_CHECK(hipEventRecord(evT, sT));
+        MH_HIP_CHECK(hipStreamWaitEvent(sA, evT, 0));
+    }
     if (rc != MH_OK) return rc;
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
 
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
-              int64_t stride, int64_t* ipiv, int* d_info) {
-    return lu_factor_t<double>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info);
+              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt) {
+    return lu_factor_t<double>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt);
 }
 
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
-                  int64_t stride, int64_t* ipiv, int* d_info) {
-    return lu_factor_t<float>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info);
+                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt) {
+    return lu_factor_t<float>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt);
 }
 
 // ---------------------------------------------------------------------------

@@ -515,43 +515,48 @@ static int launch_trsv(Context* c, int64_t n, int64_t batch, const typename Cx<T
     return MH_OK;
 }
 
-// 64 x 64 tiles through LDS: reads and writes both run down columns.
+// Rows [r_lo, r_hi) of each n x n matrix, transposed into columns
+// [r_lo, r_hi) of At: 64 x 64 tiles through LDS, reads and writes both run
+// down columns.
 template <class C2>
 __global__ __launch_bounds__(256) void k_transpose(int64_t n, const C2* __restrict__ A, int64_t ld,
-                                                   int64_t stride, C2* __restrict__ At) {
+                                                   int64_t stride, C2* __restrict__ At, int64_t r_lo,
+                                                   int64_t r_hi) {
     __shared__ C2 t[64][65];
     const int64_t item = blockIdx.z;
-    const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+    const int64_t r0 = r_lo + (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
     const C2* src = A + item * stride;
     C2* dst = At + item * stride;
     const int x = threadIdx.x & 63, y = threadIdx.x >> 6;
     for (int cc = y; cc < 64; cc += 4)
-        if (r0 + x < n && c0 + cc < n) t[cc][x] = src[(r0 + x) + (c0 + cc) * ld];
+        if (r0 + x < r_hi && c0 + cc < n) t[cc][x] = src[(r0 + x) + (c0 + cc) * ld];
     __syncthreads();
     // At[c0 + x, r0 + rr] = A[r0 + rr, c0 + x]
     for (int rr = y; rr < 64; rr += 4)
-        if (c0 + x < n && r0 + rr < n) dst[(c0 + x) + (r0 + rr) * ld] = t[x][rr];
+        if (c0 + x < n && r0 + rr < r_hi) dst[(c0 + x) + (r0 + rr) * ld] = t[x][rr];
 }
 
 template <class T>
 static int tri_transpose_t(Context* c, int64_t n, int64_t batch, const typename Cx<T>::t* LU, int64_t ld,
-                           int64_t stride, typename Cx<T>::t* LUt) {
-    if (n == 0 || batch == 0) return MH_OK;
+                           int64_t stride, typename Cx<T>::t* LUt, int64_t r_lo, int64_t r_hi) {
+    if (r_hi < 0) r_hi = n;
+    if (n == 0 || batch == 0 || r_hi <= r_lo) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu_solve: batch too large");
-    const unsigned t = (unsigned)((n + 63) / 64);
-    ProfScope ps(c, "transpose", 0.0, 2.0 * sizeof(typename Cx<T>::t) * (double)n * n * batch);
-    hipLaunchKernelGGL(k_transpose<typename Cx<T>::t>, dim3(t, t, (unsigned)batch), dim3(256), 0, c->stream, n,
-                       LU, ld, stride, LUt);
+    MH_REQUIRE(r_lo >= 0 && r_hi <= n, "transpose: row range outside the matrix");
+    const unsigned tr = (unsigned)((r_hi - r_lo + 63) / 64), tc = (unsigned)((n + 63) / 64);
+    ProfScope ps(c, "transpose", 0.0, 2.0 * sizeof(typename Cx<T>::t) * (double)(r_hi - r_lo) * n * batch);
+    hipLaunchKernelGGL(k_transpose<typename Cx<T>::t>, dim3(tr, tc, (unsigned)batch), dim3(256), 0, c->stream, n,
+                       LU, ld, stride, LUt, r_lo, r_hi);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
 int tri_transpose(Context* c, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
-                  double2* LUt) {
-    return tri_transpose_t<double>(c, n, batch, LU, ld, stride, LUt);
+                  double2* LUt, int64_t r_lo, int64_t r_hi) {
+    return tri_transpose_t<double>(c, n, batch, LU, ld, stride, LUt, r_lo, r_hi);
 }
 int tri_transpose_f32(Context* c, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
-                      float2* LUt) {
-    return tri_transpose_t<float>(c, n, batch, LU, ld, stride, LUt);
+                      float2* LUt, int64_t r_lo, int64_t r_hi) {
+    return tri_transpose_t<float>(c, n, batch, LU, ld, stride, LUt, r_lo, r_hi);
 }
 
 // One triangle: as stored, or transposed (from the explicit copy LUt when
