@@ -686,8 +686,7 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             // with an adjoint system, the transposed factors are built during the LU
             float2* Dt32 = (ga && ndof) ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr;
-            TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info, Dt32));
-            TRY(tri_prepare_f32(c, n, nw, D32, n, n * n, piv, perm, inv32));
+            TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info, Dt32, perm, inv32));  // + solve data
             // forward system: phi = D \ (S bc), or sigma = K \ bc (kept in ybuf)
             IrSystem sys[2];
             if (direct) {
@@ -731,8 +730,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
         if (!mixed) {
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
             double2* dDt = (ga && ndof) ? reinterpret_cast<double2*>(ws + o_Dt) : nullptr;
-            TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info, dDt));  // + transposed factors
-            TRY(tri_prepare(c, n, nw, dD, ld, stride, piv, (ga && ndof) ? perm : nullptr, inv));
+            TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info, dDt, (ga && ndof) ? perm : nullptr,
+                          inv));  // + transposed factors and solve data
             if (ga && ndof) {  // adjoint solve on aux, alongside the back substitution
                 double2* cw = reinterpret_cast<double2*>(ws + o_cw);
                 double2* lam = reinterpret_cast<double2*>(ws + o_lam);

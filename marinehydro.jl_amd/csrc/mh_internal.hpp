@@ -123,8 +123,11 @@ int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const d
 // ---- launchers (solve.hip) --------------------------------------------------
 // LUt (optional): the transposed factors (n x n, same ld and stride), each
 // block row transposed as soon as it is final, alongside the factorisation.
+// inv (optional, with perm optional): what tri_prepare produces, built block
+// column by block column alongside the factorisation.
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
-              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt = nullptr);
+              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt = nullptr, int* perm = nullptr,
+              double2* inv = nullptr);
 // ---- launchers (trsv.hip) ---------------------------------------------------
 constexpr int kMaxSolveN = 4096;
 // After an LU factorisation: perm[b*n + i], the net row permutation of the
@@ -136,6 +139,14 @@ int tri_prepare(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64
                 const int64_t* ipiv, int* perm, double2* inv);
 int tri_prepare_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
                     const int64_t* ipiv, int* perm, float2* inv);
+// The same for block column [k0, k0+w) of a factorisation in progress (its
+// pivots and diagonal blocks final): its interchanges folded into perm (which
+// starts from the identity at k0 = 0; perm may be null) and the inverses of
+// its diagonal blocks.  Block columns are processed in order.
+int tri_prepare_part(Context* ctx, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+                     const int64_t* ipiv, int* perm, double2* inv, int64_t k0, int w);
+int tri_prepare_part(Context* ctx, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                     const int64_t* ipiv, int* perm, float2* inv, int64_t k0, int w);
 // LUt = LU^T per item (same ld and stride), for transposed solves that read
 // the factors row-wise; only rows [r_lo, r_hi) of LU (columns of LUt) when
 // given (r_hi < 0: to n).
@@ -167,7 +178,8 @@ int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
                      double2* Y, int64_t ldy, int64_t ystride, const double2* Z = nullptr);
 // FP32 factorisation and solves (mixed precision)
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
-                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt = nullptr);
+                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt = nullptr, int* perm = nullptr,
+                  float2* inv = nullptr);
 int convert_to_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* src,
                    int64_t lds, int64_t sstride, float2* dst, int64_t ldd, int64_t dstride);
 int add_from_f32(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const float2* d,

@@ -934,9 +934,13 @@ static int transpose_rows(Context* c, int64_t n, int64_t batch, const float2* A,
 // With LUt, block row k of the factors (final once panel k, its narrow and
 // its wide update are done: later interchanges only touch rows below it) is
 // transposed into block column k of LUt on the side stream.
+// With inv (and perm), the permutation and inverse diagonal blocks the
+// triangular solves use are built per block column on the side stream, once
+// its panel is factored.
 template <class T>
 static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, typename Cx<T>::t* A,
-                       int64_t ld, int64_t stride, int64_t* ipiv, int* d_info, typename Cx<T>::t* LUt) {
+                       int64_t ld, int64_t stride, int64_t* ipiv, int* d_info, typename Cx<T>::t* LUt, int* perm,
+                       typename Cx<T>::t* inv) {
     if (n == 0 || batch == 0) return MH_OK;
     MH_REQUIRE(batch <= 65535, "lu: batch too large for one launch");
     MH_REQUIRE(n <= 4096, "lu: n larger than 4096 is not supported by the register-resident panel");
@@ -968,6 +972,13 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         rc = lu_panel(P, k0, w);
         if (rc != MH_OK) break;
         MH_HIP_CHECK(hipEventRecord(evF[k], sA));
+        if (inv) {  // this block column's pivots and diagonal blocks are final
+            MH_HIP_CHECK(hipStreamWaitEvent(sT, evF[k], 0));
+            ctx->stream = sT;
+            rc = tri_prepare_part(ctx, n, batch, A, ld, stride, ipiv, perm, inv, k0, w);
+            ctx->stream = sA;
+            if (rc != MH_OK) break;
+        }
         // wide update on aux
         ctx->stream = sB;
         MH_HIP_CHECK(hipStreamWaitEvent(sB, evF[k], 0));
@@ -997,7 +1008,7 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     ctx->stream = sA;
     // stream continues after all of aux's updates (and the transposes)
     MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[npan - 1], 0));
-    if (LUt) {
+    if (LUt || inv) {
         MH_HIP_CHECK(hipEventRecord(evT, sT));
         MH_HIP_CHECK(hipStreamWaitEvent(sA, evT, 0));
     }
@@ -1007,13 +1018,13 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
 }
 
 int lu_factor(Context* ctx, int64_t n, int64_t ncols, int64_t batch, double2* A, int64_t ld,
-              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt) {
-    return lu_factor_t<double>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt);
+              int64_t stride, int64_t* ipiv, int* d_info, double2* LUt, int* perm, double2* inv) {
+    return lu_factor_t<double>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt, perm, inv);
 }
 
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
-                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt) {
-    return lu_factor_t<float>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt);
+                  int64_t stride, int64_t* ipiv, int* d_info, float2* LUt, int* perm, float2* inv) {
+    return lu_factor_t<float>(ctx, n, ncols, batch, A, ld, stride, ipiv, d_info, LUt, perm, inv);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,6 +53,7 @@ constexpr int kTsOwn = kTsRows / kTsCB;
 constexpr int kTsBatch = 8;   // ready blocks of earlier slabs applied per round
 constexpr unsigned kSpinLimit = 1u << 22;
 constexpr unsigned long long kUnset = ~0ull;  // "not yet written"
+constexpr int kNBPerm = 128;  // interchanges folded per k_perm_update launch (a block column)
 
 __device__ __forceinline__ unsigned long long ld_pub(const void* p) {
     return __hip_atomic_load(reinterpret_cast<unsigned long long*>(const_cast<void*>(p)), __ATOMIC_RELAXED,
@@ -338,11 +339,11 @@ __global__ void k_fill_unset(int64_t rows, int64_t cols, int64_t batch, C2* __re
 template <class T>
 __global__ __launch_bounds__(64) void k_tri_inv(int64_t n, int64_t batch, const typename Cx<T>::t* __restrict__ A,
                                                 int64_t ld, int64_t stride, int nsubp,
-                                                typename Cx<T>::t* __restrict__ inv) {
+                                                typename Cx<T>::t* __restrict__ inv, int K0) {
     typedef typename Cx<T>::t C2;
     __shared__ C2 sB[kTsCB][kTsCB + 1];
     __shared__ C2 sY[2][kTsCB][kTsCB + 1];  // columns of L^-1 and U^-1
-    const int K = blockIdx.x;
+    const int K = K0 + blockIdx.x;
     const int64_t item = blockIdx.y;
     const int lane = threadIdx.x;
     const C2* M = A + item * stride;
@@ -402,6 +403,42 @@ __global__ __launch_bounds__(256) void k_perm_from_ipiv(int64_t n, const int64_t
     }
     __syncthreads();
     for (int64_t k = threadIdx.x; k < n; k += blockDim.x) perm[item * n + k] = s[k];
+}
+
+// The interchanges ipiv[k0 .. k0+w) folded into the net permutation perm
+// (initialised to the identity when k0 = 0); they only move entries >= k0,
+// which are staged in LDS.  One workgroup per item, one thread applies the
+// interchanges in order.
+__global__ __launch_bounds__(256) void k_perm_update(int64_t n, const int64_t* __restrict__ ipiv,
+                                                     int* __restrict__ perm, int* __restrict__ err, int64_t k0,
+                                                     int w) {
+    __shared__ int s[kMaxSolveN];
+    __shared__ int sp[kNBPerm];
+    const int64_t item = blockIdx.x;
+    const int64_t* pv = ipiv + item * n;
+    int* pm = perm + item * n;
+    for (int64_t k = k0 + threadIdx.x; k < n; k += blockDim.x) s[k - k0] = k0 == 0 ? (int)k : pm[k];
+    for (int k = threadIdx.x; k < w; k += blockDim.x) {
+        const int64_t p = pv[k0 + k];
+        sp[k] = (p < k0 + k || p >= n) ? -1 : (int)(p - k0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < w; ++k) {
+            const int p = sp[k];
+            if (p < 0) {
+                atomicOr(err, 2);
+                continue;
+            }
+            if (p != k) {
+                const int t = s[k];
+                s[k] = s[p];
+                s[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    for (int64_t k = k0 + threadIdx.x; k < n; k += blockDim.x) pm[k] = s[k - k0];
 }
 
 __global__ void k_reset_ticket(unsigned long long* t) { *t = 0ull; }
@@ -469,9 +506,43 @@ static int tri_prepare_t(Context* c, int64_t n, int64_t batch, const typename Cx
     const int nsubp = (int)((n + kTsCB - 1) / kTsCB);
     ProfScope ps(c, sizeof(T) == 8 ? "tri_inv" : "tri_inv_f32", 0.0, 0.0);
     hipLaunchKernelGGL(k_tri_inv<T>, dim3((unsigned)nsubp, (unsigned)batch), dim3(64), 0, c->stream, n, batch,
-                       LU, ld, stride, nsubp, inv);
+                       LU, ld, stride, nsubp, inv, 0);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
+}
+
+// The part of tri_prepare that block column [k0, k0+w) of a factorisation in
+// progress allows (w <= 128; k0 and w multiples of 32 except at the end):
+// its interchanges folded into perm, the inverses of its diagonal blocks.
+template <class T>
+static int tri_prepare_part_t(Context* c, int64_t n, int64_t batch, const typename Cx<T>::t* LU, int64_t ld,
+                              int64_t stride, const int64_t* ipiv, int* perm, typename Cx<T>::t* inv, int64_t k0,
+                              int w) {
+    if (n == 0 || batch == 0 || w <= 0) return MH_OK;
+    MH_REQUIRE(n <= kMaxSolveN, "lu_solve: n larger than 4096 is not supported");
+    MH_REQUIRE(batch <= 65535, "lu_solve: batch too large");
+    MH_REQUIRE(w <= kNBPerm && k0 % kTsCB == 0 && (k0 + w == n || w % kTsCB == 0), "lu_solve: bad block column");
+    TRY_RC(tri_state(c, 0));
+    if (perm) {
+        hipLaunchKernelGGL(k_perm_update, dim3((unsigned)batch), dim3(256), 0, c->stream, n, ipiv, perm, c->tri_err,
+                           k0, w);
+        MH_HIP_CHECK(hipGetLastError());
+    }
+    const int nsubp = (int)((n + kTsCB - 1) / kTsCB);
+    const int K0 = (int)(k0 / kTsCB), K1 = (int)((k0 + w + kTsCB - 1) / kTsCB);
+    ProfScope ps(c, sizeof(T) == 8 ? "tri_inv" : "tri_inv_f32", 0.0, 0.0);
+    hipLaunchKernelGGL(k_tri_inv<T>, dim3((unsigned)(K1 - K0), (unsigned)batch), dim3(64), 0, c->stream, n, batch,
+                       LU, ld, stride, nsubp, inv, K0);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+int tri_prepare_part(Context* c, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,
+                     const int64_t* ipiv, int* perm, double2* inv, int64_t k0, int w) {
+    return tri_prepare_part_t<double>(c, n, batch, LU, ld, stride, ipiv, perm, inv, k0, w);
+}
+int tri_prepare_part(Context* c, int64_t n, int64_t batch, const float2* LU, int64_t ld, int64_t stride,
+                     const int64_t* ipiv, int* perm, float2* inv, int64_t k0, int w) {
+    return tri_prepare_part_t<float>(c, n, batch, LU, ld, stride, ipiv, perm, inv, k0, w);
 }
 
 int tri_prepare(Context* c, int64_t n, int64_t batch, const double2* LU, int64_t ld, int64_t stride,

@@ -280,6 +280,20 @@ def test_mixed_precision_north_star_size(mh, gpu):
     assert relmax(Am, A) < 1e-8 and relmax(Bm, B) < 1e-8 and relmax(Fm, F) < 1e-8
 
 
+def test_mixed_precision_above_2048_panels(mh, gpu):
+    """2100 panels: FP32 leaf panels taller than 2048 rows (8 rows per thread)
+    and a last block column of 52; mixed against FP64, with gradients."""
+    body = mh.hemisphere(1.0, ntheta=70, nphi=60)
+    m = body.mesh
+    assert m.nfaces == 2100
+    omegas = [0.5, 2.0]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=2, omega_param=1)
+    r64 = mh.freq_sweep_grad(m, omegas, [[0, 0, 1.0]], [0.0], T, rho=1025.0)
+    r32 = mh.freq_sweep_grad(m, omegas, [[0, 0, 1.0]], [0.0], T, rho=1025.0, precision="mixed")
+    for a, b in zip(r32, r64):
+        assert relmax(a, b) < 1e-8
+
+
 def test_reference_entry_points(mh, oracle, gpu):
     # calculate_radiation_forces / DiffractionForce (src/waves.jl:29-52)
     m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
