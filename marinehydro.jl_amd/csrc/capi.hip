@@ -492,7 +492,11 @@ constexpr double kIrTol = 1e-10;
 // systems solve from it with coalesced reads.
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
-                    int nsys, float2* LUt32) {
+                    int nsys, float2* LUt32, double2* Pa) {
+    // a forward and a transposed system form their residuals together, in
+    // one pass over A (Pa: workspace of zgemv_pair)
+    const bool pair = nsys == 2 && Pa && sys[0].trans == 0 && sys[1].trans == 1 && sys[0].nrhs <= 4 &&
+                      sys[1].nrhs <= 4;
     size_t hcount = 0;
     for (int k = 0; k < nsys; ++k) hcount += (size_t)(batch * sys[k].nrhs);
     TRY(ensure_pinned(c, hcount));
@@ -506,6 +510,7 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
                                     hipMemcpyDeviceToDevice, c->stream));
     }
     for (int it = 0; it < 12; ++it) {
+        const bool both = pair && !done[0] && !done[1];
         size_t off = 0;
         for (int k = 0; k < nsys; ++k) {
             const IrSystem& q = sys[k];
@@ -517,18 +522,45 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
                 TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, q.trans, 3, q.nrhs, q.R32, n, s32,
                                   q.trans ? LUt32 : nullptr));
                 TRY(add_from_f32(c, n, q.nrhs, batch, q.R32, n, s32, q.X, q.ldb, q.bstride));
+                if (both) {
+                    off += cnt;
+                    continue;
+                }
                 if (q.trans)
                     TRY(zgemm_tn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
                                          q.bstride, q.Bm));
                 else
                     TRY(zgemm_nn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
                                          q.bstride, q.Bm));
-                TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
-                MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + off, q.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost,
-                                            c->stream));
+                if (it > 0) {
+                    TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
+                    MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + off, q.nrm, sizeof(double) * cnt,
+                                                hipMemcpyDeviceToHost, c->stream));
+                }
             }
             off += cnt;
         }
+        if (both) {  // both residuals in one pass over A, on stream
+            TRY(join_aux(c));
+            const IrSystem &f = sys[0], &a = sys[1];
+            TRY(zgemv_pair(c, n, batch, A64, ld, stride, (int)f.nrhs, f.X, f.Bm, f.R, f.ldb, f.bstride, (int)a.nrhs,
+                           a.X, a.Bm, a.R, a.ldb, a.bstride, Pa));
+            if (it > 0) {
+                size_t o2 = 0;
+                for (int k = 0; k < nsys; ++k) {
+                    const IrSystem& q = sys[k];
+                    const size_t cnt = (size_t)(batch * q.nrhs);
+                    TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
+                    MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + o2, q.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                                c->stream));
+                    o2 += cnt;
+                }
+            }
+            TRY(fork_aux(c));  // the next solve on aux starts after the residuals
+        }
+        // one FP32 solve leaves a residual far above the tolerance: the first
+        // correction is always made, without a host round trip for the check
+        if (it == 0) continue;
         MH_HIP_CHECK(hipStreamSynchronize(c->stream));
         if (nsys > 1 && !c->profiling) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
         off = 0;
@@ -598,6 +630,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     const size_t o_Dt = (ga && ndof) ? bump.take(mixed ? sizeof(float2) * (size_t)n * (size_t)n * chunk
                                                        : mat_aug * chunk)
                                      : 0;
+    // partial sums of the paired residuals of the forward and adjoint systems
+    const size_t o_pa = (mixed && ga && ndof) ? bump.take(sizeof(double2) * (size_t)zgemv_pair_ws_elems(n, chunk)) : 0;
     if (mixed) {
         o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * chunk);
         o_y = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
@@ -714,7 +748,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             }
             {
                 const int rc = ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
-                                        nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr);
+                                        nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr,
+                                        o_pa ? reinterpret_cast<double2*>(ws + o_pa) : nullptr);
                 if (rc == MH_ERR_NOT_CONVERGED) TRY(check_info(c, info, nw));  // a more specific cause
                 TRY(rc);
             }

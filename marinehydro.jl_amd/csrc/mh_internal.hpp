@@ -176,6 +176,13 @@ int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const
 int zgemm_nn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,
                      int64_t ld, int64_t stride, const double2* X, int64_t ldx, int64_t xstride,
                      double2* Y, int64_t ldy, int64_t ystride, const double2* Z = nullptr);
+// Rf = Bf - S Xf and Ra = Ba - S^T Xa (1 to 4 columns each, n x cols per
+// item with the given leading dimensions and strides) in one pass over S;
+// Pa: workspace of zgemv_pair_ws_elems(n, batch) complex entries.
+int64_t zgemv_pair_ws_elems(int64_t n, int64_t batch);
+int zgemv_pair(Context* ctx, int64_t n, int64_t batch, const double2* S, int64_t ld, int64_t stride, int nrf,
+               const double2* Xf, const double2* Bf, double2* Rf, int64_t ldf, int64_t fstride, int nra,
+               const double2* Xa, const double2* Ba, double2* Ra, int64_t lda, int64_t astride, double2* Pa);
 // FP32 factorisation and solves (mixed precision)
 int lu_factor_f32(Context* ctx, int64_t n, int64_t ncols, int64_t batch, float2* A, int64_t ld,
                   int64_t stride, int64_t* ipiv, int* d_info, float2* LUt = nullptr, int* perm = nullptr,

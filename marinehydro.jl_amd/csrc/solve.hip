@@ -1158,6 +1158,185 @@ __global__ __launch_bounds__(256) void k_zgemv_tn(int64_t n, int64_t nrhs, const
     }
 }
 
+// Residuals of a forward and a transposed system with the same matrix, in
+// one pass over it: Rf = Bf - A Xf (nrf <= NRF columns) and Ra = Ba - A^T Xa
+// (nra <= NRA columns).  One thread per row of a 256-row tile, columns in
+// chunks of 64 with Xf staged in LDS and 16 column loads in flight.  For
+// A^T Xa every lane forms A[i, j] Xa[i] for 16 columns; recursive halving
+// across the wave (lanes exchanging half of their columns at distances 32,
+// 16, 8, 4, then summing at 2 and 1) leaves each column's wave sum in 4
+// lanes, the 4 waves are added in LDS and the tile's partial sums are
+// written to Pa[item][tile][r][j] for k_pair_finish.
+constexpr int kPairRows = 256;
+template <int NRF, int NRA>
+__global__ __launch_bounds__(256) void k_zgemv_pair(int64_t n, const double2* __restrict__ A, int64_t ld,
+                                                    int64_t stride, int nrf, const double2* __restrict__ Xf,
+                                                    const double2* __restrict__ Bf, double2* __restrict__ Rf,
+                                                    int64_t ldf, int64_t fstride, int nra,
+                                                    const double2* __restrict__ Xa, int64_t lda, int64_t astride,
+                                                    double2* __restrict__ Pa, int ntile) {
+    const int64_t item = blockIdx.y;
+    const int tile = blockIdx.x;
+    const double2* Am = A + item * stride;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t i = (int64_t)tile * kPairRows + tid;
+    const bool ok = i < n;
+    __shared__ double2 sx[64][NRF];
+    __shared__ double2 scol[4][NRA][64];  // per wave: column sums of the current 64-column chunk
+    double2 xa[NRA];
+#pragma unroll
+    for (int r = 0; r < NRA; ++r)
+        xa[r] = (ok && r < nra) ? Xa[item * astride + i + r * lda] : make_double2(0.0, 0.0);
+    double2 acc[NRF];
+#pragma unroll
+    for (int r = 0; r < NRF; ++r) acc[r] = make_double2(0.0, 0.0);
+    const double2* row = Am + (ok ? i : 0);
+    for (int64_t j0 = 0; j0 < n; j0 += 64) {
+        const int jn = (int)((n - j0) < 64 ? (n - j0) : 64);
+        __syncthreads();  // previous chunk's sx and scol consumed
+        for (int q = tid; q < 64 * NRF; q += 256) {
+            const int jj = q % 64, r = q / 64;
+            sx[jj][r] = (jj < jn && r < nrf) ? Xf[item * fstride + (j0 + jj) + r * ldf] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int jb = 0; jb < 64; jb += 16) {
+            double2 av[16];  // 16 column loads in flight (zero outside the matrix)
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                av[u] = (ok && jb + u < jn) ? row[(j0 + jb + u) * ld] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) zmac<NRF>(acc, av[u], sx[jb + u]);
+#pragma unroll
+            for (int r = 0; r < NRA; ++r) {
+                // p[u] = A[i, j0+jb+u] Xa[i, r], reduced over the wave's 64 rows
+                double px[16], py[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    px[u] = av[u].x * xa[r].x - av[u].y * xa[r].y;
+                    py[u] = av[u].x * xa[r].y + av[u].y * xa[r].x;
+                }
+                // distance 32: keep columns 0-7 (lanes < 32) or 8-15
+                const bool h32 = (lane & 32) != 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const double sx_ = h32 ? px[u] : px[u + 8], sy_ = h32 ? py[u] : py[u + 8];
+                    const double ox = __shfl_xor(sx_, 32), oy = __shfl_xor(sy_, 32);
+                    px[u] = (h32 ? px[u + 8] : px[u]) + ox;
+                    py[u] = (h32 ? py[u + 8] : py[u]) + oy;
+                }
+                const bool h16 = (lane & 16) != 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double sx_ = h16 ? px[u] : px[u + 4], sy_ = h16 ? py[u] : py[u + 4];
+                    const double ox = __shfl_xor(sx_, 16), oy = __shfl_xor(sy_, 16);
+                    px[u] = (h16 ? px[u + 4] : px[u]) + ox;
+                    py[u] = (h16 ? py[u + 4] : py[u]) + oy;
+                }
+                const bool h8 = (lane & 8) != 0;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const double sx_ = h8 ? px[u] : px[u + 2], sy_ = h8 ? py[u] : py[u + 2];
+                    const double ox = __shfl_xor(sx_, 8), oy = __shfl_xor(sy_, 8);
+                    px[u] = (h8 ? px[u + 2] : px[u]) + ox;
+                    py[u] = (h8 ? py[u + 2] : py[u]) + oy;
+                }
+                const bool h4 = (lane & 4) != 0;
+                {
+                    const double sx_ = h4 ? px[0] : px[1], sy_ = h4 ? py[0] : py[1];
+                    const double ox = __shfl_xor(sx_, 4), oy = __shfl_xor(sy_, 4);
+                    px[0] = (h4 ? px[1] : px[0]) + ox;
+                    py[0] = (h4 ? py[1] : py[0]) + oy;
+                }
+                px[0] += __shfl_xor(px[0], 2);
+                py[0] += __shfl_xor(py[0], 2);
+                px[0] += __shfl_xor(px[0], 1);
+                py[0] += __shfl_xor(py[0], 1);
+                // this lane holds column 8 b5 + 4 b4 + 2 b3 + b2 of the 16
+                const int u = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+                if ((lane & 3) == 0) scol[wave][r][jb + u] = make_double2(px[0], py[0]);
+            }
+        }
+        __syncthreads();
+        if (tid < 64) {  // the tile's partial sums of this chunk's columns
+#pragma unroll
+            for (int r = 0; r < NRA; ++r) {
+                if (r >= nra || tid >= jn) continue;
+                double2 t = scol[0][r][tid];
+#pragma unroll
+                for (int w = 1; w < 4; ++w) {
+                    t.x += scol[w][r][tid].x;
+                    t.y += scol[w][r][tid].y;
+                }
+                Pa[((item * ntile + tile) * NRA + r) * n + j0 + tid] = t;
+            }
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int r = 0; r < NRF; ++r)
+            if (r < nrf) {
+                const double2 b = Bf[item * fstride + i + r * ldf];
+                Rf[item * fstride + i + r * ldf] = make_double2(b.x - acc[r].x, b.y - acc[r].y);
+            }
+    }
+}
+
+// Ra[:, r] = Ba[:, r] - sum over tiles of the partial sums
+template <int NRA>
+__global__ __launch_bounds__(256) void k_pair_finish(int64_t n, int nra, int ntile, const double2* __restrict__ Pa,
+                                                     const double2* __restrict__ Ba, double2* __restrict__ Ra,
+                                                     int64_t lda, int64_t astride) {
+    const int64_t item = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    for (int r = 0; r < nra; ++r) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int tl = 0; tl < ntile; ++tl) {
+            const double2 p = Pa[((item * ntile + tl) * NRA + r) * n + j];
+            t.x += p.x;
+            t.y += p.y;
+        }
+        const double2 b = Ba[item * astride + j + r * lda];
+        Ra[item * astride + j + r * lda] = make_double2(b.x - t.x, b.y - t.y);
+    }
+}
+
+int64_t zgemv_pair_ws_elems(int64_t n, int64_t batch) {
+    return batch * ((n + kPairRows - 1) / kPairRows) * 4 * n;
+}
+
+int zgemv_pair(Context* ctx, int64_t n, int64_t batch, const double2* A, int64_t ld, int64_t stride, int nrf,
+               const double2* Xf, const double2* Bf, double2* Rf, int64_t ldf, int64_t fstride, int nra,
+               const double2* Xa, const double2* Ba, double2* Ra, int64_t lda, int64_t astride, double2* Pa) {
+    if (n == 0 || batch == 0) return MH_OK;
+    MH_REQUIRE(batch <= 65535, "matvec: batch too large");
+    MH_REQUIRE(nrf >= 1 && nrf <= 4 && nra >= 1 && nra <= 4, "matvec pair: 1 to 4 columns per system");
+    const int ntile = (int)((n + kPairRows - 1) / kPairRows);
+    ProfScope ps(ctx, "zgemv_pair", 8.0 * (double)n * n * (nrf + nra) * batch, 16.0 * (double)n * n * batch);
+    const dim3 grid((unsigned)ntile, (unsigned)batch), fgrid((unsigned)((n + 255) / 256), (unsigned)batch);
+#define MH_PAIR(F, R)                                                                                            \
+    do {                                                                                                         \
+        hipLaunchKernelGGL((k_zgemv_pair<F, R>), grid, dim3(256), 0, ctx->stream, n, A, ld, stride, nrf, Xf, Bf, \
+                           Rf, ldf, fstride, nra, Xa, lda, astride, Pa, ntile);                                  \
+        hipLaunchKernelGGL((k_pair_finish<R>), fgrid, dim3(256), 0, ctx->stream, n, nra, ntile, Pa, Ba, Ra, lda,  \
+                           astride);                                                                             \
+    } while (0)
+    const int f = nrf <= 1 ? 1 : (nrf <= 2 ? 2 : 4), r = nra <= 1 ? 1 : (nra <= 2 ? 2 : 4);
+    if (f == 1 && r == 1) MH_PAIR(1, 1);
+    else if (f == 1 && r == 2) MH_PAIR(1, 2);
+    else if (f == 1) MH_PAIR(1, 4);
+    else if (f == 2 && r == 1) MH_PAIR(2, 1);
+    else if (f == 2 && r == 2) MH_PAIR(2, 2);
+    else if (f == 2) MH_PAIR(2, 4);
+    else if (r == 1) MH_PAIR(4, 1);
+    else if (r == 2) MH_PAIR(4, 2);
+    else MH_PAIR(4, 4);
+#undef MH_PAIR
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
 #define MH_ZGEMV_DISPATCH(KERNEL, GRID)                                                                  \
     do {                                                                                                 \
         if (nr <= 1)                                                                                     \
