@@ -847,17 +847,23 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             MH_HIP_CHECK(hipMemcpyAsync(d_dom, bdom, sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
             const bool done = early_rk && m0 == 0;  // tangents and Rankine parts already there
             if (!done) TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
-            int64_t nblk = 0;
-            if (wave_mask)
-                TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
-                                         aDw, sS, sD, wa, wss, part, &nblk));
-            TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 0));
+            // the Rankine parts are contracted into T2 on the side stream while
+            // the wave-term pair kernel runs; its partial sums are added after
+            MH_HIP_CHECK(hipMemsetAsync(T2, 0, sizeof(double2) * (size_t)(nw * ndof * nr * Pg), c->stream));
             if (rank_mask) {
+                TRY(fork_side(c));
+                OnSide on(c);
                 double* dR = reinterpret_cast<double*>(ws + o_dR);
                 double2* Yb = reinterpret_cast<double2*>(ws + o_Y);
                 if (!done) TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, dR));
                 TRY(launch_rankine_contract(c, n, Pg, nw, ndof, nr, dR, aS, aDw, sS, sD, wa, wss, Yb, T2));
             }
+            int64_t nblk = 0;
+            if (wave_mask)
+                TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
+                                         aDw, sS, sD, wa, wss, part, &nblk));
+            if (rank_mask) TRY(join_side(c));
+            TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 1));
             if (direct) TRY(join_aux(c));
             TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
