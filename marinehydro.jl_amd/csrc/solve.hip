@@ -210,20 +210,33 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
         }
         if (!zero) {
             const C2 inv = crecip(prow[c]);
+            // the pivot row's remaining entries, read once for all R rows
+            T ux[kLeaf], uy[kLeaf];  // scalar arrays: an array of vector structs would go to scratch
+#pragma unroll
+            for (int j = 1; j < kLeaf; ++j)  // constant trip count: unrolled before c is known
+                if (j > c) {
+                    const C2 v = prow[j];
+                    ux[j] = v.x;
+                    uy[j] = v.y;
+                }
+            // rows above the pivot and outside the panel take multiplier 0, so
+            // every row runs the same straight-line update (a - 0 u = a)
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int r = tid + NT * q;
-                if (r > c && r < m) {
-                    const C2 l = cmul(a[q][c], inv);
-                    a[q][c] = l;
+                const bool act = r > c && r < m;
+                const C2 lv = cmul(a[q][c], inv);
+                // selects per component: a conditional on the vector structs
+                // selects between addresses and puts the panel in scratch
+                const T lx = act ? lv.x : T(0), ly = act ? lv.y : T(0);
+                a[q][c] = mkc<C2>(act ? lv.x : a[q][c].x, act ? lv.y : a[q][c].y);
+                const C2 l = mkc<C2>(lx, ly);
 #pragma unroll
-                    for (int j = 1; j < kLeaf; ++j)  // constant trip count: unrolled before c is known
-                        if (j > c) {  // a -= l u as four fused multiply-adds
-                            const C2 u = prow[j];
-                            T x = fma(-l.x, u.x, a[q][j].x), y = fma(-l.x, u.y, a[q][j].y);
-                            a[q][j] = mkc<C2>(fma(l.y, u.y, x), fma(-l.y, u.x, y));
-                        }
-                }
+                for (int j = 1; j < kLeaf; ++j)
+                    if (j > c) {  // a -= l u as four fused multiply-adds
+                        T x = fma(-l.x, ux[j], a[q][j].x), y = fma(-l.x, uy[j], a[q][j].y);
+                        a[q][j] = mkc<C2>(fma(l.y, uy[j], x), fma(-l.y, ux[j], y));
+                    }
             }
         }
         }
