@@ -87,6 +87,32 @@ __device__ __forceinline__ V wave_reduce(V x) {
     return r;
 }
 
+// 1/p for a leaf pivot without a division: p scaled by a power of two so its
+// larger component lies in [0.5, 1) (exact), 1/|p|^2 from the hardware
+// reciprocal estimate refined by Newton steps (one in FP32, two in FP64),
+// then scaled back (exact).  Within an ulp or two of Smith's algorithm;
+// reciprocals of powers of two and of +-1 +-i stay exact.
+__device__ __forceinline__ float rcp_refined(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);
+}
+__device__ __forceinline__ double rcp_refined(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ int frexp_exponent(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ __forceinline__ int frexp_exponent(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ __forceinline__ float scale2(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+__device__ __forceinline__ double scale2(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+template <class C2>
+__device__ __forceinline__ C2 pivot_recip(C2 p) {
+    const int e = frexp_exponent(fmax(fabs(p.x), fabs(p.y)));
+    const auto x = scale2(p.x, -e), y = scale2(p.y, -e);
+    const auto r = rcp_refined(fma(x, x, y * y));
+    return mkc<C2>(scale2(x * r, -e), scale2(-y * r, -e));
+}
+
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= kLeaf, held in
 // registers.  Thread t owns local rows t + NT q, q < R.  Per column: every
@@ -209,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
             }
         }
         if (!zero) {
-            const C2 inv = crecip(prow[c]);
+            const C2 inv = pivot_recip(prow[c]);
             // the pivot row's remaining entries, read once for all R rows
             T ux[kLeaf], uy[kLeaf];  // scalar arrays: an array of vector structs would go to scratch
 #pragma unroll
