@@ -61,7 +61,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep is "
               "independent per (body, frequency) and would shard by frequency, but multi-GPU runs "
-              "are not part of this build", file=sys.stderr)
+              "are not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

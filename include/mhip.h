@@ -38,7 +38,7 @@ enum mh_status {
     MH_ERR_ARGUMENT = 1,        /* bad size, pointer or option            */
     MH_ERR_HIP = 2,             /* HIP runtime error                      */
     MH_ERR_SINGULAR = 3,        /* exact zero pivot in the LU             */
-    MH_ERR_NOT_SUPPORTED = 4,   /* option not available in this build     */
+    MH_ERR_NOT_SUPPORTED = 4,   /* option not supported by the library   */
     MH_ERR_NO_DEVICE = 5,       /* no usable gfx950 device                */
     MH_ERR_NOT_CONVERGED = 6    /* iterative refinement did not converge  */
 };

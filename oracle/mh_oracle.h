@@ -2,7 +2,7 @@
  * FP64 CPU oracle for the MarineHydro boundary-element hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
- * algorithm (MarineHydro.jl, Julia) used as the checker for the HIP library.
+ * algorithm (MarineHydro.jl, Julia) that the HIP library is tested against.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product path (libmhip.so) never calls into it.
  *

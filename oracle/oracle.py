@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 CPU oracle (oracle/mh_oracle.c).
 
 TEST INFRASTRUCTURE ONLY.  The oracle is a plain-C restatement of the
-reference algorithm (MarineHydro.jl); it is the checker for the HIP library.
+reference algorithm (MarineHydro.jl); the HIP library is tested against it.
 Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module; the product package never does.
 

@@ -270,7 +270,7 @@ def test_freq_sweep_matches_oracle(mh, oracle, gpu, direct, precision):
         assert relmax(F[w].T, Fo) < tol, om
 
 
-def test_mixed_precision_north_star_size(mh, gpu):
+def test_mixed_precision_2048_panels(mh, gpu):
     """2048 panels: mixed-precision sweep against the FP64 sweep; the mode is
     specified to 1e-6 relative, refinement to a 1e-10 residual gives ~1e-10."""
     m = mh.hemisphere(1.0, ntheta=64, nphi=64).mesh
@@ -356,7 +356,7 @@ def test_bad_mesh_index_is_rejected(mh, gpu):
         mh.assemble_matrix_wu(dm, 1.0)
 
 
-def test_north_star_size_properties(mh, oracle, gpu):
+def test_2048_panel_assembly_and_solve_properties(mh, oracle, gpu):
     """N = 2048 (64 x 64 hemisphere): assembly of sampled columns against the
     oracle, and the batched LU solve through its residual."""
     torch = gpu
