@@ -404,6 +404,97 @@ __global__ __launch_bounds__(256) void k_lu_trsm(typename Cx<T>::t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// X = L11^{-1} X in one launch for a whole block column: L11 = unit lower
+// block at (k0, k0) of size w <= kNBOuter, X = rows [k0, k0+w) of columns
+// [c_lo, c_hi).  A workgroup solves kTbCols columns of one matrix: the 16
+// lanes of a DPP row hold one column, lane g rows g + 16 i (i < 8), in
+// registers.  Forward substitution one unknown at a time, fully unrolled so
+// every register index is a constant: x_c is broadcast within the row
+// (row_newbcast), then every lane subtracts L(r, c) x_c from its rows below
+// c; L11's columns are staged in LDS kTbStage at a time (zero on and above
+// the diagonal and beyond w).  Replaces the 32-row blocked TRSM + GEMM
+// sequence (7 launches for a 128-wide block column).
+// ---------------------------------------------------------------------------
+constexpr int kTbCols = 16, kTbStage = 32, kTbSlots = kNBOuter / 16;
+template <int L>
+__device__ __forceinline__ float row_bcast(float x) {
+    static_assert(L >= 0 && L < 16, "lane within a row of 16");
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + L, 0xF, 0xF, false));
+}
+template <int C>
+struct TbStep {  // unknown C of the substitution
+    __device__ __forceinline__ static void run(float (&xr)[kTbSlots], float (&xi)[kTbSlots],
+                                               const float2 (*sL)[kNBOuter + 1], int g) {
+        constexpr int ic = C / 16, gc = C % 16, cc = C % kTbStage;
+        const float br = row_bcast<gc>(xr[ic]), bi = row_bcast<gc>(xi[ic]);
+#pragma unroll
+        for (int i = ic; i < kTbSlots; ++i) {
+            const float2 l = sL[cc][g + 16 * i];  // zero for rows r <= C (and r >= w)
+            xr[i] = fmaf(-l.x, br, fmaf(l.y, bi, xr[i]));
+            xi[i] = fmaf(-l.x, bi, fmaf(-l.y, br, xi[i]));
+        }
+    }
+};
+template <int C0, int N>
+struct TbStage {  // unknowns C0 .. C0 + N - 1 (one staged block), uniform bound w
+    __device__ __forceinline__ static void run(float (&xr)[kTbSlots], float (&xi)[kTbSlots],
+                                               const float2 (*sL)[kNBOuter + 1], int g, int w) {
+        if (C0 < w) {
+            TbStep<C0>::run(xr, xi, sL, g);
+            TbStage<C0 + 1, N - 1>::run(xr, xi, sL, g, w);
+        }
+    }
+};
+template <int C0>
+struct TbStage<C0, 0> {
+    __device__ __forceinline__ static void run(float (&)[kTbSlots], float (&)[kTbSlots],
+                                               const float2 (*)[kNBOuter + 1], int, int) {}
+};
+template <int S>
+__device__ __forceinline__ void tb_stage(float (&xr)[kTbSlots], float (&xi)[kTbSlots], float2 (*sL)[kNBOuter + 1],
+                                         const float2* __restrict__ M, int64_t ld, int64_t k0, int w, int tid,
+                                         int g) {
+    if (S >= w) return;  // uniform
+    __syncthreads();     // the previous block's reads of sL are done
+    for (int q = tid; q < kTbStage * kNBOuter; q += 256) {
+        const int cc = q / kNBOuter, r = q % kNBOuter, c = S + cc;
+        sL[cc][r] = (c < w && r < w && r > c) ? M[(k0 + r) + (k0 + c) * ld] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    TbStage<S, kTbStage>::run(xr, xi, sL, g, w);
+}
+__global__ __launch_bounds__(256) void k_lu_trsm_blk(float2* __restrict__ A, int64_t ld, int64_t stride, int64_t k0,
+                                                     int w, int64_t c_lo, int64_t c_hi) {
+    static_assert(kNBOuter == 128 && kTbStage == 32, "four staged blocks of 32 unknowns");
+    float2* M = A + (int64_t)blockIdx.y * stride;
+    __shared__ float2 sL[kTbStage][kNBOuter + 1];  // sL[c - S][r] = L11(r, c)
+    const int tid = threadIdx.x, g = tid & 15;
+    const int64_t j = c_lo + (int64_t)blockIdx.x * kTbCols + (tid >> 4);
+    const bool ok = j < c_hi;
+    float2* col = M + (ok ? j : c_lo) * ld + k0;
+    float xr[kTbSlots], xi[kTbSlots];
+#pragma unroll
+    for (int i = 0; i < kTbSlots; ++i) {
+        const int r = g + 16 * i;
+        const float2 v = (ok && r < w) ? col[r] : make_float2(0.f, 0.f);
+        xr[i] = v.x;
+        xi[i] = v.y;
+    }
+    // every lane takes part in every step (the row broadcasts read all 16 lanes)
+    tb_stage<0>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<32>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<64>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<96>(xr, xi, sL, M, ld, k0, w, tid, g);
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < kTbSlots; ++i) {
+            const int r = g + 16 * i;
+            if (r < w) col[r] = make_float2(xr[i], xi[i]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // C[r_lo:r_hi, c_lo:c_hi] -= A[r_lo:r_hi, k_lo:k_lo+kw] * A[k_lo:k_lo+kw, c_lo:c_hi]
 // 64 x 64 complex tile per 256-thread workgroup; wave (wr, wc) owns a 32 x 32
 // quarter as 2 x 2 MFMA tiles of 16 x 16, real and imaginary accumulators.
@@ -889,6 +980,13 @@ template <class T>
 static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
     if (c_hi <= c_lo) return MH_OK;
     const int64_t nc = c_hi - c_lo;
+    if constexpr (sizeof(T) == 4) {  // FP32: the whole block in one launch
+        MH_REQUIRE(w <= kNBOuter, "lu: TRSM block wider than a block column");
+        ProfScope ps(P.ctx, "lu_trsm", 4.0 * w * w * (double)nc * P.batch, 16.0 * w * (double)nc * P.batch);
+        hipLaunchKernelGGL(k_lu_trsm_blk, dim3((unsigned)((nc + kTbCols - 1) / kTbCols), (unsigned)P.batch), dim3(256),
+                           0, P.ctx->stream, P.A, P.ld, P.stride, k0, w, c_lo, c_hi);
+        return MH_OK;
+    }
     for (int s = 0; s < w; s += kTrsmNB) {
         const int t = (w - s) < kTrsmNB ? (w - s) : kTrsmNB;
         {
