@@ -8,7 +8,7 @@
 //                  one workgroup per matrix, pivot = max |re|+|im| (izamax
 //                  semantics, first index on ties)
 //   k_lu_swap      row interchanges of a pivot block on a column range
-//   k_lu_trsm      unit-lower L11 X = B on a <=32-row block, column range
+//   k_lu_trsm_blk  unit-lower L11 X = B on a whole <=128-row block, column range
 //   k_lu_gemm      C -= A B on arbitrary row/column/depth ranges, complex
 //                  product as four real v_mfma_f64_16x16x4 products
 // Extra columns beyond n (ncols > n) are carried through every step, so a
@@ -21,7 +21,6 @@
 namespace mh {
 
 constexpr int kNBOuter = 128;  // block-column width of the trailing updates
-constexpr int kTrsmNB = 32;    // rows per triangular-solve block
 
 // ---------------------------------------------------------------------------
 // Wave reductions on DPP lane moves.  Within each row of 16 lanes the moves
@@ -375,35 +374,6 @@ __global__ __launch_bounds__(256) void k_lu_swap(C2* __restrict__ A, int64_t ld,
 }
 
 // ---------------------------------------------------------------------------
-// X = L11^{-1} X, L11 = unit lower block at (k0, k0) of size nb <= 32,
-// X = rows [k0, k0+nb) of columns [c_lo, c_hi).  32 lanes per column (lane r
-// holds row r, coalesced 512-byte column reads); forward substitution by
-// broadcasting x_c within the 32-lane group.
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_lu_trsm(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
-                                                 int64_t k0, int nb, int64_t c_lo, int64_t c_hi) {
-    typedef typename Cx<T>::t C2;
-    C2* M = A + (int64_t)blockIdx.y * stride;
-    __shared__ C2 sL[kTrsmNB][kTrsmNB + 1];  // sL[c][r] = L11(r, c)
-    for (int q = threadIdx.x; q < kTrsmNB * kTrsmNB; q += 256) {
-        const int r = q % kTrsmNB, c = q / kTrsmNB;
-        sL[c][r] = (r < nb && c < nb) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
-    }
-    __syncthreads();
-    const int r = threadIdx.x & 31;
-    const int64_t j = c_lo + (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
-    const bool ok = j < c_hi;
-    C2* col = M + (ok ? j : c_lo) * ld;
-    C2 x = (ok && r < nb) ? col[k0 + r] : mkc<C2>(T(0), T(0));
-    for (int c = 0; c < nb; ++c) {
-        const C2 xc = mkc<C2>(__shfl(x.x, c, 32), __shfl(x.y, c, 32));
-        if (r > c) x = cfms(x, sL[c][r], xc);
-    }
-    if (ok && r < nb) col[k0 + r] = x;
-}
-
-// ---------------------------------------------------------------------------
 // X = L11^{-1} X in one launch for a whole block column: L11 = unit lower
 // block at (k0, k0) of size w <= kNBOuter, X = rows [k0, k0+w) of columns
 // [c_lo, c_hi).  A workgroup solves kTbCols columns of one matrix: the 16
@@ -412,8 +382,8 @@ __global__ __launch_bounds__(256) void k_lu_trsm(typename Cx<T>::t* __restrict__
 // every register index is a constant: x_c is broadcast within the row
 // (row_newbcast), then every lane subtracts L(r, c) x_c from its rows below
 // c; L11's columns are staged in LDS kTbStage at a time (zero on and above
-// the diagonal and beyond w).  Replaces the 32-row blocked TRSM + GEMM
-// sequence (7 launches for a 128-wide block column).
+// the diagonal and beyond w).  One launch instead of a 32-row blocked
+// TRSM + GEMM sequence (7 launches for a 128-wide block column).
 // ---------------------------------------------------------------------------
 constexpr int kTbCols = 16, kTbStage = 32, kTbSlots = kNBOuter / 16;
 template <int L>
@@ -421,75 +391,89 @@ __device__ __forceinline__ float row_bcast(float x) {
     static_assert(L >= 0 && L < 16, "lane within a row of 16");
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + L, 0xF, 0xF, false));
 }
-template <int C>
+template <int L>
+__device__ __forceinline__ double row_bcast(double x) {
+    static_assert(L >= 0 && L < 16, "lane within a row of 16");
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150 + L, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + L, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <class T, int C>
 struct TbStep {  // unknown C of the substitution
-    __device__ __forceinline__ static void run(float (&xr)[kTbSlots], float (&xi)[kTbSlots],
-                                               const float2 (*sL)[kNBOuter + 1], int g) {
+    typedef typename Cx<T>::t C2;
+    __device__ __forceinline__ static void run(T (&xr)[kTbSlots], T (&xi)[kTbSlots], const C2 (*sL)[kNBOuter + 1],
+                                               int g) {
         constexpr int ic = C / 16, gc = C % 16, cc = C % kTbStage;
-        const float br = row_bcast<gc>(xr[ic]), bi = row_bcast<gc>(xi[ic]);
+        const T br = row_bcast<gc>(xr[ic]), bi = row_bcast<gc>(xi[ic]);
 #pragma unroll
         for (int i = ic; i < kTbSlots; ++i) {
-            const float2 l = sL[cc][g + 16 * i];  // zero for rows r <= C (and r >= w)
-            xr[i] = fmaf(-l.x, br, fmaf(l.y, bi, xr[i]));
-            xi[i] = fmaf(-l.x, bi, fmaf(-l.y, br, xi[i]));
+            const C2 l = sL[cc][g + 16 * i];  // zero for rows r <= C (and r >= w)
+            xr[i] = fma(-l.x, br, fma(l.y, bi, xr[i]));
+            xi[i] = fma(-l.x, bi, fma(-l.y, br, xi[i]));
         }
     }
 };
-template <int C0, int N>
+template <class T, int C0, int N>
 struct TbStage {  // unknowns C0 .. C0 + N - 1 (one staged block), uniform bound w
-    __device__ __forceinline__ static void run(float (&xr)[kTbSlots], float (&xi)[kTbSlots],
-                                               const float2 (*sL)[kNBOuter + 1], int g, int w) {
+    typedef typename Cx<T>::t C2;
+    __device__ __forceinline__ static void run(T (&xr)[kTbSlots], T (&xi)[kTbSlots], const C2 (*sL)[kNBOuter + 1],
+                                               int g, int w) {
         if (C0 < w) {
-            TbStep<C0>::run(xr, xi, sL, g);
-            TbStage<C0 + 1, N - 1>::run(xr, xi, sL, g, w);
+            TbStep<T, C0>::run(xr, xi, sL, g);
+            TbStage<T, C0 + 1, N - 1>::run(xr, xi, sL, g, w);
         }
     }
 };
-template <int C0>
-struct TbStage<C0, 0> {
-    __device__ __forceinline__ static void run(float (&)[kTbSlots], float (&)[kTbSlots],
-                                               const float2 (*)[kNBOuter + 1], int, int) {}
+template <class T, int C0>
+struct TbStage<T, C0, 0> {
+    typedef typename Cx<T>::t C2;
+    __device__ __forceinline__ static void run(T (&)[kTbSlots], T (&)[kTbSlots], const C2 (*)[kNBOuter + 1], int,
+                                               int) {}
 };
-template <int S>
-__device__ __forceinline__ void tb_stage(float (&xr)[kTbSlots], float (&xi)[kTbSlots], float2 (*sL)[kNBOuter + 1],
-                                         const float2* __restrict__ M, int64_t ld, int64_t k0, int w, int tid,
-                                         int g) {
+template <class T, int S>
+__device__ __forceinline__ void tb_stage(T (&xr)[kTbSlots], T (&xi)[kTbSlots], typename Cx<T>::t (*sL)[kNBOuter + 1],
+                                         const typename Cx<T>::t* __restrict__ M, int64_t ld, int64_t k0, int w,
+                                         int tid, int g) {
+    typedef typename Cx<T>::t C2;
     if (S >= w) return;  // uniform
     __syncthreads();     // the previous block's reads of sL are done
     for (int q = tid; q < kTbStage * kNBOuter; q += 256) {
         const int cc = q / kNBOuter, r = q % kNBOuter, c = S + cc;
-        sL[cc][r] = (c < w && r < w && r > c) ? M[(k0 + r) + (k0 + c) * ld] : make_float2(0.f, 0.f);
+        sL[cc][r] = (c < w && r < w && r > c) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
     }
     __syncthreads();
-    TbStage<S, kTbStage>::run(xr, xi, sL, g, w);
+    TbStage<T, S, kTbStage>::run(xr, xi, sL, g, w);
 }
-__global__ __launch_bounds__(256) void k_lu_trsm_blk(float2* __restrict__ A, int64_t ld, int64_t stride, int64_t k0,
-                                                     int w, int64_t c_lo, int64_t c_hi) {
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_trsm_blk(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
+                                                     int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+    typedef typename Cx<T>::t C2;
     static_assert(kNBOuter == 128 && kTbStage == 32, "four staged blocks of 32 unknowns");
-    float2* M = A + (int64_t)blockIdx.y * stride;
-    __shared__ float2 sL[kTbStage][kNBOuter + 1];  // sL[c - S][r] = L11(r, c)
+    C2* M = A + (int64_t)blockIdx.y * stride;
+    __shared__ C2 sL[kTbStage][kNBOuter + 1];  // sL[c - S][r] = L11(r, c)
     const int tid = threadIdx.x, g = tid & 15;
     const int64_t j = c_lo + (int64_t)blockIdx.x * kTbCols + (tid >> 4);
     const bool ok = j < c_hi;
-    float2* col = M + (ok ? j : c_lo) * ld + k0;
-    float xr[kTbSlots], xi[kTbSlots];
+    C2* col = M + (ok ? j : c_lo) * ld + k0;
+    T xr[kTbSlots], xi[kTbSlots];
 #pragma unroll
     for (int i = 0; i < kTbSlots; ++i) {
         const int r = g + 16 * i;
-        const float2 v = (ok && r < w) ? col[r] : make_float2(0.f, 0.f);
+        const C2 v = (ok && r < w) ? col[r] : mkc<C2>(T(0), T(0));
         xr[i] = v.x;
         xi[i] = v.y;
     }
     // every lane takes part in every step (the row broadcasts read all 16 lanes)
-    tb_stage<0>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<32>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<64>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<96>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<T, 0>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<T, 32>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<T, 64>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<T, 96>(xr, xi, sL, M, ld, k0, w, tid, g);
     if (ok) {
 #pragma unroll
         for (int i = 0; i < kTbSlots; ++i) {
             const int r = g + 16 * i;
-            if (r < w) col[r] = make_float2(xr[i], xi[i]);
+            if (r < w) col[r] = mkc<C2>(xr[i], xi[i]);
         }
     }
 }
@@ -975,27 +959,15 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
     return MH_OK;
 }
 
-// X = L[k0:k0+w, k0:k0+w]^{-1} X on columns [c_lo, c_hi), blocked by 32 rows
+// X = L[k0:k0+w, k0:k0+w]^{-1} X on columns [c_lo, c_hi), one launch
 template <class T>
 static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
     if (c_hi <= c_lo) return MH_OK;
     const int64_t nc = c_hi - c_lo;
-    if constexpr (sizeof(T) == 4) {  // FP32: the whole block in one launch
-        MH_REQUIRE(w <= kNBOuter, "lu: TRSM block wider than a block column");
-        ProfScope ps(P.ctx, "lu_trsm", 4.0 * w * w * (double)nc * P.batch, 16.0 * w * (double)nc * P.batch);
-        hipLaunchKernelGGL(k_lu_trsm_blk, dim3((unsigned)((nc + kTbCols - 1) / kTbCols), (unsigned)P.batch), dim3(256),
-                           0, P.ctx->stream, P.A, P.ld, P.stride, k0, w, c_lo, c_hi);
-        return MH_OK;
-    }
-    for (int s = 0; s < w; s += kTrsmNB) {
-        const int t = (w - s) < kTrsmNB ? (w - s) : kTrsmNB;
-        {
-            ProfScope ps(P.ctx, "lu_trsm", 4.0 * t * t * (double)nc * P.batch, 32.0 * t * (double)nc * P.batch);
-            hipLaunchKernelGGL(k_lu_trsm<T>, dim3((unsigned)((nc + 7) / 8), (unsigned)P.batch), dim3(256), 0,
-                               P.ctx->stream, P.A, P.ld, P.stride, k0 + s, t, c_lo, c_hi);
-        }
-        if (s + t < w) TRY_RC(lu_gemm(P, k0 + s + t, k0 + w, c_lo, c_hi, k0 + s, t));
-    }
+    MH_REQUIRE(w <= kNBOuter, "lu: TRSM block wider than a block column");
+    ProfScope ps(P.ctx, "lu_trsm", 4.0 * w * w * (double)nc * P.batch, 4.0 * sizeof(T) * w * (double)nc * P.batch);
+    hipLaunchKernelGGL(k_lu_trsm_blk<T>, dim3((unsigned)((nc + kTbCols - 1) / kTbCols), (unsigned)P.batch), dim3(256),
+                       0, P.ctx->stream, P.A, P.ld, P.stride, k0, w, c_lo, c_hi);
     return MH_OK;
 }
 
