@@ -382,7 +382,8 @@ __global__ __launch_bounds__(256) void k_lu_swap(C2* __restrict__ A, int64_t ld,
 // every register index is a constant: x_c is broadcast within the row
 // (row_newbcast), then every lane subtracts L(r, c) x_c from its rows below
 // c; L11's columns are staged in LDS kTbStage at a time (zero on and above
-// the diagonal and beyond w).  One launch instead of a 32-row blocked
+// the diagonal and beyond w), the next block's loads in flight while the
+// current block's steps run.  One launch instead of a 32-row blocked
 // TRSM + GEMM sequence (7 launches for a 128-wide block column).
 // ---------------------------------------------------------------------------
 constexpr int kTbCols = 16, kTbStage = 32, kTbSlots = kNBOuter / 16;
@@ -431,18 +432,36 @@ struct TbStage<T, C0, 0> {
     __device__ __forceinline__ static void run(T (&)[kTbSlots], T (&)[kTbSlots], const C2 (*)[kNBOuter + 1], int,
                                                int) {}
 };
+// L11 columns S .. S + kTbStage - 1 (rows below the diagonal, rows and
+// columns below w; zero elsewhere) into registers: kTbPer entries per thread,
+// every load issued before any is waited for (addresses outside the block
+// clamped to its first entry, values replaced by zero)
+constexpr int kTbPer = kTbStage * kNBOuter / 256;
+template <class T, int S>
+__device__ __forceinline__ void tb_load(T (&tx)[kTbPer], T (&ty)[kTbPer], const typename Cx<T>::t* __restrict__ M,
+                                        int64_t ld, int64_t k0, int w, int tid) {
+    typedef typename Cx<T>::t C2;
+    const int r = tid % kNBOuter;
+#pragma unroll
+    for (int e = 0; e < kTbPer; ++e) {
+        const int c = S + 2 * e + tid / kNBOuter;  // tid + 256 e = (c - S) kNBOuter + r
+        const bool in = c < w && r < w && r > c;
+        const C2 v = M[(k0 + (in ? r : 0)) + (k0 + (in ? c : 0)) * ld];
+        tx[e] = in ? v.x : T(0);
+        ty[e] = in ? v.y : T(0);
+    }
+}
 template <class T, int S>
 __device__ __forceinline__ void tb_stage(T (&xr)[kTbSlots], T (&xi)[kTbSlots], typename Cx<T>::t (*sL)[kNBOuter + 1],
-                                         const typename Cx<T>::t* __restrict__ M, int64_t ld, int64_t k0, int w,
-                                         int tid, int g) {
+                                         T (&tx)[kTbPer], T (&ty)[kTbPer], const typename Cx<T>::t* __restrict__ M,
+                                         int64_t ld, int64_t k0, int w, int tid, int g) {
     typedef typename Cx<T>::t C2;
     if (S >= w) return;  // uniform
     __syncthreads();     // the previous block's reads of sL are done
-    for (int q = tid; q < kTbStage * kNBOuter; q += 256) {
-        const int cc = q / kNBOuter, r = q % kNBOuter, c = S + cc;
-        sL[cc][r] = (c < w && r < w && r > c) ? M[(k0 + r) + (k0 + c) * ld] : mkc<C2>(T(0), T(0));
-    }
+#pragma unroll
+    for (int e = 0; e < kTbPer; ++e) sL[2 * e + tid / kNBOuter][tid % kNBOuter] = mkc<C2>(tx[e], ty[e]);
     __syncthreads();
+    if (S + kTbStage < w) tb_load<T, S + kTbStage>(tx, ty, M, ld, k0, w, tid);  // next block, during this one
     TbStage<T, S, kTbStage>::run(xr, xi, sL, g, w);
 }
 template <class T>
@@ -464,11 +483,13 @@ __global__ __launch_bounds__(256) void k_lu_trsm_blk(typename Cx<T>::t* __restri
         xr[i] = v.x;
         xi[i] = v.y;
     }
+    T tx[kTbPer], ty[kTbPer];
+    tb_load<T, 0>(tx, ty, M, ld, k0, w, tid);
     // every lane takes part in every step (the row broadcasts read all 16 lanes)
-    tb_stage<T, 0>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<T, 32>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<T, 64>(xr, xi, sL, M, ld, k0, w, tid, g);
-    tb_stage<T, 96>(xr, xi, sL, M, ld, k0, w, tid, g);
+    tb_stage<T, 0>(xr, xi, sL, tx, ty, M, ld, k0, w, tid, g);
+    tb_stage<T, 32>(xr, xi, sL, tx, ty, M, ld, k0, w, tid, g);
+    tb_stage<T, 64>(xr, xi, sL, tx, ty, M, ld, k0, w, tid, g);
+    tb_stage<T, 96>(xr, xi, sL, tx, ty, M, ld, k0, w, tid, g);
     if (ok) {
 #pragma unroll
         for (int i = 0; i < kTbSlots; ++i) {
