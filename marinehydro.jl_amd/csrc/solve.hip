@@ -530,21 +530,35 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
             acci[a][b] = (acc_t){T(0), T(0), T(0), T(0)};
         }
     for (int kc = 0; kc < kw; kc += kKC) {
+        // every operand load issued before any is waited for: entries outside
+        // the ranges load from a clamped address (the first entry of the
+        // operand) and are replaced by zero; scalar arrays stay in registers
+        T ax[4], ay[4], bx[4], by[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int q = tid + 256 * t;
             {
                 const int r = q & 63, k = q >> 6;
                 const int64_t gr = tr + r;
-                sA[k][r] = (gr < r_hi && kc + k < kw) ? M[gr + (k_lo + kc + k) * ld]
-                                                      : mkc<C2>(T(0), T(0));
+                const bool in = gr < r_hi && kc + k < kw;
+                const C2 v = M[(in ? gr : r_lo) + (k_lo + (in ? kc + k : 0)) * ld];
+                ax[t] = in ? v.x : T(0);
+                ay[t] = in ? v.y : T(0);
             }
             {
                 const int k = q & 15, c = q >> 4;
                 const int64_t gc = tc + c;
-                sB[k][c] = (gc < c_hi && kc + k < kw) ? M[(k_lo + kc + k) + gc * ld]
-                                                      : mkc<C2>(T(0), T(0));
+                const bool in = gc < c_hi && kc + k < kw;
+                const C2 v = M[(k_lo + (in ? kc + k : 0)) + (in ? gc : c_lo) * ld];
+                bx[t] = in ? v.x : T(0);
+                by[t] = in ? v.y : T(0);
             }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = tid + 256 * t;
+            sA[q >> 6][q & 63] = mkc<C2>(ax[t], ay[t]);
+            sB[q & 15][q >> 4] = mkc<C2>(bx[t], by[t]);
         }
         __syncthreads();
 #pragma unroll
@@ -580,7 +594,8 @@ __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__
             for (int r = 0; r < 4; ++r) {
                 const int64_t gr = tr + 32 * wr + 16 * a + (l & 15);
                 const int64_t gc = tc + 32 * wc + 16 * b + Cx<T>::row(r, l);
-                cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : mkc<C2>(T(0), T(0));
+                const bool in = gr < r_hi && gc < c_hi;  // clamped address outside: no branch
+                cv[a][b][r] = M[(in ? gr : r_lo) + (in ? gc : c_lo) * ld];
             }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
