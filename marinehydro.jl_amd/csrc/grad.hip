@@ -509,15 +509,28 @@ __global__ __launch_bounds__(256) void k_rc_gemm(int64_t n, int64_t ncol, int64_
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
     for (int64_t k0 = 0; k0 < n; k0 += kRcK) {
+        // all eight loads issued before any is waited for (one workgroup per
+        // CU leaves nothing else to hide their latency): entries outside the
+        // ranges load the first element and are replaced by zero
+        double va[4], vbx[4], vby[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int rr = tid % 64, kk = tid / 64 + 4 * t;
             const int64_t row = r0 + rr, k = k0 + kk;
-            sA[kk][rr] = (row < n && k < n) ? A[row + k * n] : 0.0;
+            const bool ina = row < n && k < n;
+            const double a = A[ina ? row + k * n : 0];
+            va[t] = ina ? a : 0.0;
             const int kb = tid % 16, cc = tid / 16 + 16 * t;
             const int64_t col = c0 + cc, q = k0 + kb;
-            sB[kb][cc] = (col < ncol && q < n) ? B[(col / nr) * wstride + (col % nr) * n + q]
-                                               : make_double2(0.0, 0.0);
+            const bool inb = col < ncol && q < n;
+            const double2 b = B[inb ? (col / nr) * wstride + (col % nr) * n + q : 0];
+            vbx[t] = inb ? b.x : 0.0;
+            vby[t] = inb ? b.y : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            sA[tid / 64 + 4 * t][tid % 64] = va[t];
+            sB[tid % 16][tid / 16 + 16 * t] = make_double2(vbx[t], vby[t]);
         }
         __syncthreads();
 #pragma unroll 4
