@@ -23,6 +23,7 @@
 // differences of the forward path converge to.
 #include <vector>
 
+#include "mh_cx.hpp"
 #include "mh_internal.hpp"
 
 namespace mh {
@@ -490,75 +491,89 @@ __global__ __launch_bounds__(256) void k_rankine_tangents(int64_t n, const doubl
 // Y[z] (n x ncol complex, column c at Y + z * n * ncol + c * n) =
 //   dR[z] (n x n real) * B_z, z = mat * P + m; B_z column c = (w, r) =
 //   (c / nr, c % nr) at Bmat + w * wstride + r * n (mat 0: B0, mat 1: B1).
-// 64 x 64 tiles, 256 threads, each a 4 x 4 block of complex outputs.
+// 64 x 64 tiles, 256 threads, on the FP64 matrix cores.
 constexpr int kRcK = 16;
 __global__ __launch_bounds__(256) void k_rc_gemm(int64_t n, int64_t ncol, int64_t nr, int P,
                                                  const double* __restrict__ dR, const double2* __restrict__ B0,
                                                  const double2* __restrict__ B1, int64_t wstride,
                                                  double2* __restrict__ Y) {
-    __shared__ double sA[kRcK][64 + 1];
-    __shared__ double2 sB[kRcK][64 + 1];
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    // 64 x 64 tile per workgroup on the FP64 matrix cores: wave (wr, wc) owns
+    // a 32 x 32 quarter as 2 x 2 v_mfma_f64_16x16x4f64 tiles, real and
+    // imaginary parts of the complex B as two real products.  Operands are
+    // swapped (the MFMA forms the transposed block) so the 16 lanes of a
+    // result row hold 16 consecutive rows of Y (coalesced column stores).
+    __shared__ double sA[kRcK][64];   // sA[k][row]
+    __shared__ double2 sB[kRcK][64];  // sB[k][col]
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wc = wave >> 1;
     const int z = blockIdx.z, mat = z / P;
     const double* A = dR + (int64_t)z * n * n;
     const double2* B = mat ? B1 : B0;
     const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
-    double2 acc[4][4];
+    typedef Cx<double>::acc acc_t;
+    acc_t accr[2][2], acci[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = (acc_t){0.0, 0.0, 0.0, 0.0};
+            acci[a][b] = (acc_t){0.0, 0.0, 0.0, 0.0};
+        }
     for (int64_t k0 = 0; k0 < n; k0 += kRcK) {
-        // all eight loads issued before any is waited for (one workgroup per
-        // CU leaves nothing else to hide their latency): entries outside the
-        // ranges load the first element and are replaced by zero
+        // all eight loads issued before any is waited for: entries outside
+        // the ranges load the first element and are replaced by zero
         double va[4], vbx[4], vby[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int rr = tid % 64, kk = tid / 64 + 4 * t;
+            const int q = tid + 256 * t;
+            const int rr = q & 63, kk = q >> 6;
             const int64_t row = r0 + rr, k = k0 + kk;
             const bool ina = row < n && k < n;
             const double a = A[ina ? row + k * n : 0];
             va[t] = ina ? a : 0.0;
-            const int kb = tid % 16, cc = tid / 16 + 16 * t;
-            const int64_t col = c0 + cc, q = k0 + kb;
-            const bool inb = col < ncol && q < n;
-            const double2 b = B[inb ? (col / nr) * wstride + (col % nr) * n + q : 0];
+            const int kb = q & 15, cc = q >> 4;
+            const int64_t col = c0 + cc, qq = k0 + kb;
+            const bool inb = col < ncol && qq < n;
+            const double2 b = B[inb ? (col / nr) * wstride + (col % nr) * n + qq : 0];
             vbx[t] = inb ? b.x : 0.0;
             vby[t] = inb ? b.y : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            sA[tid / 64 + 4 * t][tid % 64] = va[t];
-            sB[tid % 16][tid / 16 + 16 * t] = make_double2(vbx[t], vby[t]);
+            const int q = tid + 256 * t;
+            sA[q >> 6][q & 63] = va[t];
+            sB[q & 15][q >> 4] = make_double2(vbx[t], vby[t]);
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < kRcK; ++kk) {
-            double av[4];
-            double2 bv[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = sA[kk][tx + 16 * a];
+        for (int kk = 0; kk < kRcK; kk += 4) {
+            double av[2];
+            double2 bv[2];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][ty + 16 * b];
+            for (int a = 0; a < 2; ++a) av[a] = sA[kk + (l >> 4)][32 * wr + 16 * a + (l & 15)];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 2; ++b) bv[b] = sB[kk + (l >> 4)][32 * wc + 16 * b + (l & 15)];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    acc[a][b].x += av[a] * bv[b].x;
-                    acc[a][b].y += av[a] * bv[b].y;
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = Cx<double>::mfma(bv[b].x, av[a], accr[a][b]);
+                    acci[a][b] = Cx<double>::mfma(bv[b].y, av[a], acci[a][b]);
                 }
         }
         __syncthreads();
     }
     double2* Yz = Y + (int64_t)z * n * ncol;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int64_t row = r0 + tx + 16 * a, col = c0 + ty + 16 * b;
-            if (row < n && col < ncol) Yz[col * n + row] = acc[a][b];
-        }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * wr + 16 * a + (l & 15);
+                const int64_t col = c0 + 32 * wc + 16 * b + Cx<double>::row(r, l);
+                if (row < n && col < ncol) Yz[col * n + row] = make_double2(accr[a][b][r], acci[a][b][r]);
+            }
 }
 
 // out[((w * ndof + d) * nr + r) * P + m] +=
