@@ -294,6 +294,22 @@ def test_mixed_precision_above_2048_panels(mh, gpu):
         assert relmax(a, b) < 1e-8
 
 
+@pytest.mark.parametrize("shape", [(10, 26), (12, 43), (10, 77)])
+def test_mixed_precision_narrow_last_block_column(mh, gpu, shape):
+    """130, 258 and 385 panels: the last block column of the FP32 LU is 2, 2
+    and 1 wide (one-column leaves, triangular solves of width 1-2 and of the
+    full 128); mixed against FP64, with gradients."""
+    body = mh.hemisphere(1.0, ntheta=shape[0], nphi=shape[1])
+    m = body.mesh
+    assert m.nfaces % 128 in (1, 2)
+    omegas = [0.4, 1.7]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=2, omega_param=1)
+    r64 = mh.freq_sweep_grad(m, omegas, [[0, 0, 1.0]], [0.0], T, rho=1025.0)
+    r32 = mh.freq_sweep_grad(m, omegas, [[0, 0, 1.0]], [0.0], T, rho=1025.0, precision="mixed")
+    for a, b in zip(r32, r64):
+        assert relmax(a, b) < 1e-8
+
+
 def test_reference_entry_points(mh, oracle, gpu):
     # calculate_radiation_forces / DiffractionForce (src/waves.jl:29-52)
     m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
