@@ -112,6 +112,44 @@ __device__ __forceinline__ C2 pivot_recip(C2 p) {
     return mkc<C2>(scale2(x * r, -e), scale2(-y * r, -e));
 }
 
+// x_c broadcast from lane L of each row of 16 lanes to the whole row (DPP
+// row_newbcast); every lane of the row must be active
+template <int L>
+__device__ __forceinline__ float row_bcast(float x) {
+    static_assert(L >= 0 && L < 16, "lane within a row of 16");
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + L, 0xF, 0xF, false));
+}
+template <int L>
+__device__ __forceinline__ double row_bcast(double x) {
+    static_assert(L >= 0 && L < 16, "lane within a row of 16");
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150 + L, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + L, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// One substitution step of the leaf's optional solve: every lane of a DPP
+// row holds row g of one column, x_g -= L11(g, C) x_C (L11 from LDS, zero on
+// and above the diagonal).
+template <class T, int C, int K>
+struct LeafXStep {
+    typedef typename Cx<T>::t C2;
+    __device__ __forceinline__ static void run(T& xr, T& xi, const C2 (*sL)[16 + 1], int g, int w) {
+        if (C < w) {  // uniform
+            const T br = row_bcast<C>(xr), bi = row_bcast<C>(xi);
+            const C2 l = sL[C][g];
+            xr = fma(-l.x, br, fma(l.y, bi, xr));
+            xi = fma(-l.x, bi, fma(-l.y, br, xi));
+            LeafXStep<T, C + 1, K>::run(xr, xi, sL, g, w);
+        }
+    }
+};
+template <class T, int K>
+struct LeafXStep<T, K, K> {
+    typedef typename Cx<T>::t C2;
+    __device__ __forceinline__ static void run(T&, T&, const C2 (*)[16 + 1], int, int) {}
+};
+constexpr int kLeafX = 16;  // columns the leaf's optional solve covers (one per DPP row of waves 0-3)
+
 // ---------------------------------------------------------------------------
 // leaf panel: columns [k0, k0+w), rows [k0, n), w <= kLeaf, held in
 // registers.  Thread t owns local rows t + NT q, q < R.  Per column: every
@@ -127,7 +165,8 @@ __device__ __forceinline__ C2 pivot_recip(C2 p) {
 template <class T, int R, int kLeaf, int NT>
 __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __restrict__ A, int64_t ld,
                                                 int64_t stride, int64_t k0, int w, int64_t* __restrict__ ipiv,
-                                                int* __restrict__ info, int64_t pc_lo, int64_t pc_hi) {
+                                                int* __restrict__ info, int64_t pc_lo, int64_t pc_hi, int64_t x_lo,
+                                                int64_t x_hi) {
     typedef typename Cx<T>::t C2;
     static_assert(NT == 256 || NT == 512 || NT == 1024, "4, 8 or 16 waves");
     constexpr int NW = NT / 64;
@@ -144,6 +183,7 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
     __shared__ C2 s_crow[2][kLeaf];
     __shared__ int s_p[kLeaf];                    // local pivot rows
     __shared__ int s_rows[2 * kLeaf], s_src[2 * kLeaf];
+    __shared__ C2 s_L[kLeaf <= 16 ? kLeaf : 16][16 + 1];  // L11 for the optional solve, s_L[c][r]
     // buffer addressing from the panel's first row M + k0: entry (k0 + r,
     // col) at byte (r + col ld) sizeof(C2) (the host checks the extent fits
     // 31 bits); rows outside the panel load from offset 0 and are zeroed
@@ -266,6 +306,11 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
         }
         }
     }
+    if (x_hi > x_lo && tid < 16) {  // uniform condition; L11 strictly below the diagonal, zero elsewhere
+#pragma unroll
+        for (int j = 0; j < kLeaf && j < 16; ++j)
+            s_L[j][tid] = mkc<C2>(j < tid && tid < w ? a[0][j].x : T(0), j < tid && tid < w ? a[0][j].y : T(0));
+    }
     if (tid < w) piv[k0 + tid] = k0 + s_p[tid];  // s_p written by lane 0 of this wave
     if (tid == 0 && any_zero) atomicOr(info + blockIdx.x, 1);
 #pragma unroll
@@ -345,6 +390,24 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
 #pragma unroll
     for (int e = 0; e < kItems; ++e)
         if (vd[e] != 0xffffffffu) buf_store(g[e], rs, vd[e], 0u);
+    // optional: X = L11^{-1} X on rows [k0, k0+w) of the (at most kLeafX)
+    // columns [x_lo, x_hi) right of the leaf, once the interchanges above have
+    // reached them -- the recursion's TRSM between two leaves, without a launch
+    if (kLeaf <= 16 && x_hi > x_lo) {  // uniform
+        __syncthreads();  // the rows written above are visible to the workgroup
+        if (tid < 16 * kLeafX) {  // waves 0-3: one column per row of 16 lanes
+            const int gr = tid & 15;
+            const int64_t col = x_lo + (tid >> 4);
+            const bool ok = col < x_hi && gr < w;
+            // per-lane byte offset (row + column; the column differs between
+            // the rows of 16 lanes, so it cannot be the wave-uniform offset)
+            const uint32_t vo = ok ? (uint32_t)(gr + col * ld) * (uint32_t)sizeof(C2) : 0u;
+            const C2 v = buf_load(rs, vo, 0u, C2());
+            T xr = ok ? v.x : T(0), xi = ok ? v.y : T(0);
+            LeafXStep<T, 0, (kLeaf < 16 ? kLeaf : 16)>::run(xr, xi, s_L, gr, w);
+            if (ok) buf_store(mkc<C2>(xr, xi), rs, vo, 0u);
+        }
+    }
 }
 
 
@@ -387,19 +450,6 @@ __global__ __launch_bounds__(256) void k_lu_swap(C2* __restrict__ A, int64_t ld,
 // TRSM + GEMM sequence (7 launches for a 128-wide block column).
 // ---------------------------------------------------------------------------
 constexpr int kTbCols = 16, kTbStage = 32, kTbSlots = kNBOuter / 16;
-template <int L>
-__device__ __forceinline__ float row_bcast(float x) {
-    static_assert(L >= 0 && L < 16, "lane within a row of 16");
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x150 + L, 0xF, 0xF, false));
-}
-template <int L>
-__device__ __forceinline__ double row_bcast(double x) {
-    static_assert(L >= 0 && L < 16, "lane within a row of 16");
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150 + L, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + L, 0xF, 0xF, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
 template <class T, int C>
 struct TbStep {  // unknown C of the substitution
     typedef typename Cx<T>::t C2;
@@ -1020,12 +1070,12 @@ static int leaf_width(int64_t m) {
 }
 
 template <class T>
-static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w) {
+static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w, int64_t x_lo = 0, int64_t x_hi = 0) {
     const int64_t m = P.n - k0;
     ProfScope ps(P.ctx, "lu_leaf", 8.0 * (double)m * w * w / 2.0 * P.batch, 32.0 * (double)m * w * P.batch);
 #define LEAF(RR, WW)                                                                                   \
     hipLaunchKernelGGL((k_lu_leaf<T, RR, WW, kLeafNT>), dim3((unsigned)P.batch), dim3(kLeafNT), 0,     \
-                       P.ctx->stream, P.n, P.A, P.ld, P.stride, k0, w, P.ipiv, P.info, P.pc_lo, P.pc_hi)
+                       P.ctx->stream, P.n, P.A, P.ld, P.stride, k0, w, P.ipiv, P.info, P.pc_lo, P.pc_hi, x_lo, x_hi)
     if (m <= 1024) {
         LEAF(2, 16);
     } else if (m <= 2048) {
@@ -1048,8 +1098,13 @@ static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
     if (w <= L) return lu_leaf(P, k0, w);
     int h = ((w / 2) + L - 1) / L * L;
     if (h >= w) h = w - L;
-    TRY_RC(lu_panel(P, k0, h));
-    TRY_RC(lu_trsm(P, k0, h, k0 + h, k0 + w));
+    if (h <= L && w - h <= kLeafX && L <= 16) {  // the left part is one leaf: it also solves the right part's top rows
+        MH_REQUIRE(k0 + w <= P.pc_hi, "lu: leaf solve beyond the block column");
+        TRY_RC(lu_leaf(P, k0, h, k0 + h, k0 + w));
+    } else {
+        TRY_RC(lu_panel(P, k0, h));
+        TRY_RC(lu_trsm(P, k0, h, k0 + h, k0 + w));
+    }
     TRY_RC(lu_gemm(P, k0 + h, P.n, k0 + h, k0 + w, k0, h));
     TRY_RC(lu_panel(P, k0 + h, w - h));
     return MH_OK;
