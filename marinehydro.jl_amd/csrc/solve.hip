@@ -1264,10 +1264,23 @@ __global__ __launch_bounds__(256) void k_zgemv_nn(int64_t n, int64_t nrhs, const
     for (int r = 0; r < NR; ++r) acc[r] = make_double2(0.0, 0.0);
     for (int64_t j0 = 0; j0 < n; j0 += 64) {
         const int64_t jn = (n - j0) < 64 ? (n - j0) : 64;
+        // this block of X: every load issued before the barrier, none waited
+        // for alone (entries outside load element 0 and count zero)
+        constexpr int kPer = (64 * NR + 255) / 256;
+        double tvx[kPer], tvy[kPer];
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = (int)threadIdx.x + 256 * e, jj = q % 64, r = q / 64;
+            const bool in = q < 64 * NR && jj < jn && r < nr;
+            const double2 v = Xm[in ? (j0 + jj) + r * ldx : 0];
+            tvx[e] = in ? v.x : 0.0;
+            tvy[e] = in ? v.y : 0.0;
+        }
         __syncthreads();
-        for (int q = threadIdx.x; q < 64 * NR; q += 256) {
-            const int jj = q % 64, r = q / 64;
-            sx[jj][r] = (jj < jn && r < nr) ? Xm[(j0 + jj) + r * ldx] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = (int)threadIdx.x + 256 * e;
+            if (q < 64 * NR) sx[q % 64][q / 64] = make_double2(tvx[e], tvy[e]);
         }
         __syncthreads();
         if (i < n) {
@@ -1328,7 +1341,10 @@ __global__ __launch_bounds__(256) void k_zgemv_tn(int64_t n, int64_t nrhs, const
         for (int u = 0; u < 8; ++u) {
             double2 xv[NR];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) xv[r] = (r < nr) ? Xm[j0 + u * 64 + lane + r * ldx] : make_double2(0.0, 0.0);
+            for (int r = 0; r < NR; ++r) {  // unused columns load column 0 (no branch) and count zero
+                const double2 v = Xm[j0 + u * 64 + lane + (r < nr ? r : 0) * ldx];
+                xv[r] = make_double2(r < nr ? v.x : 0.0, r < nr ? v.y : 0.0);
+            }
             zmac<NR>(acc, sv[u], xv);
         }
     }
@@ -1336,7 +1352,10 @@ __global__ __launch_bounds__(256) void k_zgemv_tn(int64_t n, int64_t nrhs, const
         const double2 sj = col[j];
         double2 xv[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) xv[r] = (r < nr) ? Xm[j + r * ldx] : make_double2(0.0, 0.0);
+        for (int r = 0; r < NR; ++r) {
+            const double2 v = Xm[j + (r < nr ? r : 0) * ldx];
+            xv[r] = make_double2(r < nr ? v.x : 0.0, r < nr ? v.y : 0.0);
+        }
         zmac<NR>(acc, sj, xv);
     }
 #pragma unroll
