@@ -349,9 +349,23 @@ __global__ __launch_bounds__(64) void k_tri_inv(int64_t n, int64_t batch, const 
     const C2* M = A + item * stride;
     const int64_t r0 = (int64_t)K * kTsCB;
     const int jn = (int)((n - r0) < kTsCB ? (n - r0) : kTsCB);
-    for (int e = lane; e < kTsCB * kTsCB; e += 64) {
-        const int a = e % kTsCB, b = e / kTsCB;
-        sB[a][b] = (a < jn && b < jn) ? M[(r0 + a) + (r0 + b) * ld] : mkc<C2>(T(a == b ? 1 : 0), T(0));
+    {  // the diagonal block, every load issued before any is waited for
+       // (outside the matrix: element 0 loaded, identity entries used)
+        constexpr int kPer = kTsCB * kTsCB / 64;
+        T vx[kPer], vy[kPer];
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = lane + 64 * e, ra = q % kTsCB, cb = q / kTsCB;
+            const bool in = ra < jn && cb < jn;
+            const C2 v = M[in ? (r0 + ra) + (r0 + cb) * ld : 0];
+            vx[e] = in ? v.x : T(ra == cb ? 1 : 0);
+            vy[e] = in ? v.y : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int q = lane + 64 * e;
+            sB[q % kTsCB][q / kTsCB] = mkc<C2>(vx[e], vy[e]);
+        }
     }
     __syncthreads();
     const int b = lane & 31, u = lane >> 5;
