@@ -611,22 +611,35 @@ __global__ __launch_bounds__(256) void k_rc_contract(int64_t n, int64_t ndof, in
 }
 
 // out[w][d][r][m] (+)= sum over blocks of part
-__global__ void k_reduce_partials(int64_t nw, int64_t nblk, int64_t nout, const double2* __restrict__ part,
-                                  double2* __restrict__ out, int accumulate) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nw * nout) return;
+__global__ __launch_bounds__(256) void k_reduce_partials(int64_t nw, int64_t nblk, int64_t nout,
+                                                         const double2* __restrict__ part, double2* __restrict__ out,
+                                                         int accumulate) {
+    // one workgroup per output (w, o): the threads take the blocks in turn,
+    // then the sums of the four waves are added
+    const int64_t t = blockIdx.x;
     const int64_t w = t / nout, o = t % nout;
     double sx = 0.0, sy = 0.0;
-    for (int64_t b = 0; b < nblk; ++b) {
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) {
         const double2 v = part[(w * nblk + b) * nout + o];
         sx += v.x;
         sy += v.y;
     }
-    if (accumulate) {
-        out[t].x += sx;
-        out[t].y += sy;
-    } else {
-        out[t] = make_double2(sx, sy);
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+    }
+    __shared__ double2 s_w[4];
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = make_double2(sx, sy);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double rx = (s_w[0].x + s_w[1].x) + (s_w[2].x + s_w[3].x);
+        const double ry = (s_w[0].y + s_w[1].y) + (s_w[2].y + s_w[3].y);
+        if (accumulate) {
+            out[t].x += rx;
+            out[t].y += ry;
+        } else {
+            out[t] = make_double2(rx, ry);
+        }
     }
 }
 
@@ -940,8 +953,9 @@ int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout,
                            double2* out, int accumulate) {
     const int64_t tot = nw * nout;
     if (tot == 0) return MH_OK;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, nw,
-                       nblk, nout, part, out, accumulate);
+    MH_REQUIRE(tot <= 0x7fffffff, "gradients: too many outputs for one launch");
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)tot), dim3(256), 0, ctx->stream, nw, nblk, nout, part, out,
+                       accumulate);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
