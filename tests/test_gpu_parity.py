@@ -310,6 +310,29 @@ def test_mixed_precision_narrow_last_block_column(mh, gpu, shape):
         assert relmax(a, b) < 1e-8
 
 
+@pytest.mark.parametrize("direct", [True, False])
+def test_twelve_dofs_four_headings(mh, oracle, gpu, direct):
+    """Two bodies with six rigid dofs each and four headings: 16 right-hand
+    sides and 12 force weights, which take the widest matrix-vector variants
+    (16 columns, S b and S^T c); FP64 sweep against the oracle."""
+    b1 = mh.hemisphere(1.0, center=(0.0, 0.0, 0.0), ntheta=8, nphi=8)
+    b2 = mh.hemisphere(0.8, center=(3.0, 0.0, 0.0), ntheta=8, nphi=8)
+    m = mh.combine_meshes([b1.mesh, b2.mesh])
+    names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
+    n1, n2 = b1.mesh.nfaces, b2.mesh.nfaces
+    dofs = [np.vstack([d, np.zeros((n2, 3))]) for d in mh.rigid_dofs(b1.mesh, names)]
+    dofs += [np.vstack([np.zeros((n1, 3)), d])
+             for d in mh.rigid_dofs(b2.mesh, names, rotation_center=(3.0, 0.0, 0.0))]
+    betas = [0.0, 0.5, 1.0, 2.0]
+    omegas = [0.8, 1.6]
+    A, B, F = mh.freq_sweep(m, omegas, dofs, betas, direct=direct, rho=1025.0)
+    for w, om in enumerate(omegas):
+        Ao, Bo, Fo = oracle.freq_solve(GF_WU, m, om, dofs, betas, direct=direct, rho=1025.0)
+        assert relmax(A[w], Ao) < 1e-10, om
+        assert relmax(B[w], Bo) < 1e-10, om
+        assert relmax(F[w].T, Fo) < 1e-10, om
+
+
 def test_reference_entry_points(mh, oracle, gpu):
     # calculate_radiation_forces / DiffractionForce (src/waves.jl:29-52)
     m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
