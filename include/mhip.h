@@ -223,6 +223,31 @@ int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
                        double* A, double* B, double* F, double* dA, double* dB,
                        double* dF);
 
+/* ---- Froude-Krylov and excitation forces ---------------------------------
+ * FroudeKrylovForce (src/waves.jl:62-66, :150-164): the incident-wave
+ * pressure i omega rho phi_Airy at the panel centres (airy_waves_pressure,
+ * airy_waves_potential, :112-119, :150-154) integrated by integrate_pressure
+ * (:21-25), for every frequency, heading and influenced dof.  The excitation
+ * force of assemble_hydrodynamic_coefficients
+ * (src/problems_and_results.jl:134-136) is the sweep's F plus FK.
+ * omegas, betas: HOST arrays.  FK (device): nomega x nbeta x ndof complex128,
+ * the layout of mh_freq_sweep's F.  Synchronises. */
+int mh_froude_krylov(mh_context_t ctx, const mh_mesh_t* mesh, int64_t nomega,
+                     const double* omegas, double g, double rho, int64_t ndof,
+                     const double* dof_fields, int64_t nbeta, const double* betas,
+                     double* FK);
+/* The same plus dFK (device, nomega x nparam x nbeta x ndof complex128), the
+ * derivatives along nparam directions; tangents as mh_freq_sweep_grad
+ * (device dcenters, dnormals, dareas, ddof_fields, host domegas; each may be
+ * NULL = zero).  The reference obtains them by Zygote through the same
+ * functions (test/outputs_differentiation.jl).  Synchronises. */
+int mh_froude_krylov_grad(mh_context_t ctx, const mh_mesh_t* mesh, int64_t nomega,
+                          const double* omegas, double g, double rho, int64_t ndof,
+                          const double* dof_fields, int64_t nbeta, const double* betas,
+                          int64_t nparam, const double* dcenters, const double* dnormals,
+                          const double* dareas, const double* ddof_fields,
+                          const double* domegas, double* FK, double* dFK);
+
 /* ---- host-array entry points (one call, synchronous) -------------------
  * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
  * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.

@@ -134,11 +134,18 @@ int mh_context_destroy(mh_context_t ctx) {
     if (!ctx) return MH_OK;
     Context* c = CTX(ctx);
     (void)hipSetDevice(c->device);
+    // every stream of the context drains before any buffer it may still be
+    // reading or writing is released
     (void)hipStreamSynchronize(c->stream);
-    if (c->ws) (void)hipFree(c->ws);
-    if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    for (ProfRec& r : c->prof) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    c->prof.clear();
+    if (c->ws) (void)hipFree(c->ws);
+    if (c->d_flag) (void)hipFree(c->d_flag);
     for (auto& t : c->tri) {
         if (t.sync) (void)hipFree(t.sync);
         if (t.ws) (void)hipFree(t.ws);
@@ -356,8 +363,9 @@ static int check_info(Context* c, const int* info, int64_t batch) {
             return MH_ERR_SINGULAR;
         }
         if (h[b] & 2) {
-            set_error("mixed precision: matrix entries beyond the half-precision split range (|x| >= 65504) in "
-                      "batch item " + std::to_string(b) + "; use the FP64 path");
+            set_error("mixed precision: matrix entries beyond the split range of the FP32 factorisation "
+                      "(|x| >= 65504 / 4096 = 16 after the 2^12 operand scaling) in batch item " +
+                      std::to_string(b) + "; use the FP64 path");
             return MH_ERR_NOT_CONVERGED;
         }
     }
@@ -568,9 +576,11 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
         for (int k = 0; k < nsys; ++k) {
             const size_t cnt = (size_t)(batch * sys[k].nrhs);
             if (!done[k]) {
-                double worst = 0.0;
-                for (size_t e = 0; e < cnt; ++e) worst = c->h_pin[off + e] > worst ? c->h_pin[off + e] : worst;
-                done[k] = worst < kIrTol;
+                // every column below the tolerance; a NaN compares false and
+                // keeps the system unconverged
+                bool ok = true;
+                for (size_t e = 0; e < cnt; ++e) ok = ok && (c->h_pin[off + e] < kIrTol);
+                done[k] = ok;
             }
             all = all && done[k];
             off += cnt;
@@ -931,6 +941,66 @@ int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int
     GradArgs ga{nparam, dvertices, dcenters, dnormals, dareas, ddof_fields, domegas, dA, dB, dF};
     return sweep_impl(CTX(ctx), mesh, gf_mask, direct, precision, nomega, omegas, g, rho, ndof,
                       dof_fields, nbeta, betas, A, B, F, nparam > 0 ? &ga : nullptr);
+}
+
+// ---- Froude-Krylov force ----------------------------------------------------
+static int fk_impl(Context* c, const mh_mesh_t* mesh, int64_t nomega, const double* omegas, double g,
+                   double rho, int64_t ndof, const double* dofs, int64_t nbeta, const double* betas,
+                   double* FK, int64_t np, const double* dC, const double* dN, const double* dA,
+                   const double* ddofs, const double* domegas, double* dFK) {
+    MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0 && np >= 0, "froude_krylov: negative size");
+    MH_REQUIRE(g > 0.0 && rho > 0.0, "froude_krylov: g and rho must be positive");
+    if (mesh->nfaces == 0 || nomega == 0 || ndof == 0 || nbeta == 0) return MH_OK;
+    MH_REQUIRE(omegas && betas && dofs && FK, "froude_krylov: null array");
+    MH_REQUIRE(ndof <= 16, "froude_krylov: at most 16 dofs");
+    for (int64_t w = 0; w < nomega; ++w)
+        MH_REQUIRE(omegas[w] > 0.0, "froude_krylov: frequencies must be positive");
+    MH_REQUIRE(np == 0 || dFK, "froude_krylov: null gradient output");
+    Bump bump;
+    const size_t o_om = bump.take(sizeof(double) * (size_t)nomega);
+    const size_t o_be = bump.take(sizeof(double) * (size_t)nbeta);
+    const size_t o_dom = bump.take(sizeof(double) * (size_t)(nomega * (np > 0 ? np : 1)));
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    double* d_om = reinterpret_cast<double*>(ws + o_om);
+    double* d_be = reinterpret_cast<double*>(ws + o_be);
+    double* d_dom = reinterpret_cast<double*>(ws + o_dom);
+    // host tangents of omega (np x nomega) -> device (nomega x np); the
+    // pageable copies complete before the call returns
+    std::vector<double> hdom((size_t)(nomega * (np > 0 ? np : 1)), 0.0);
+    if (domegas)
+        for (int64_t m = 0; m < np; ++m)
+            for (int64_t w = 0; w < nomega; ++w) hdom[w * np + m] = domegas[m * nomega + w];
+    MH_HIP_CHECK(hipMemcpyAsync(d_om, omegas, sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
+    MH_HIP_CHECK(hipMemcpyAsync(d_be, betas, sizeof(double) * nbeta, hipMemcpyHostToDevice, c->stream));
+    if (np > 0)
+        MH_HIP_CHECK(hipMemcpyAsync(d_dom, hdom.data(), sizeof(double) * hdom.size(), hipMemcpyHostToDevice,
+                                    c->stream));
+    TRY(launch_froude_krylov(c, mesh, nomega, d_om, g, rho, ndof, dofs, nbeta, d_be,
+                             reinterpret_cast<double2*>(FK), np, dC, dN, dA, ddofs, d_dom,
+                             reinterpret_cast<double2*>(dFK)));
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+int mh_froude_krylov(mh_context_t ctx, const mh_mesh_t* mesh, int64_t nomega, const double* omegas,
+                     double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+                     const double* betas, double* FK) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    return fk_impl(CTX(ctx), mesh, nomega, omegas, g, rho, ndof, dof_fields, nbeta, betas, FK, 0, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int mh_froude_krylov_grad(mh_context_t ctx, const mh_mesh_t* mesh, int64_t nomega, const double* omegas,
+                          double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+                          const double* betas, int64_t nparam, const double* dcenters,
+                          const double* dnormals, const double* dareas, const double* ddof_fields,
+                          const double* domegas, double* FK, double* dFK) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    return fk_impl(CTX(ctx), mesh, nomega, omegas, g, rho, ndof, dof_fields, nbeta, betas, FK, nparam,
+                   dcenters, dnormals, dareas, ddof_fields, domegas, dFK);
 }
 
 // ---- host-array entry points -----------------------------------------------

@@ -205,6 +205,15 @@ int launch_forces(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, const dou
                   const double2* phi, int64_t ldb, int64_t bstride, double* A, double* B,
                   double2* F);
 
+// Froude-Krylov force FK (nomega x nbeta x ndof complex) and, when np > 0 and
+// dFK is given, its derivatives along np parameters (nomega x np x nbeta x
+// ndof); tangents as in mh_freq_sweep_grad, d_dom on the device (nomega x np).
+int launch_froude_krylov(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, const double* d_omega,
+                         double g, double rho, int64_t ndof, const double* dofs, int64_t nbeta,
+                         const double* d_beta, double2* FK, int64_t np, const double* dC,
+                         const double* dN, const double* dA, const double* ddofs, const double* d_dom,
+                         double2* dFK);
+
 // ---- launchers (grad.hip) ---------------------------------------------------
 int launch_pack_tangents(Context* ctx, const mh_mesh_t* m, int P, int64_t nparam, int64_t m0,
                          const double* dV, const double* dC, const double* dN, const double* dA,

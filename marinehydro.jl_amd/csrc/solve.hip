@@ -1646,10 +1646,14 @@ __global__ __launch_bounds__(256) void k_rel_residual(int64_t rows, const double
     const int64_t item = blockIdx.y, j = blockIdx.x;
     const double2* rc = r + item * stride + j * ld;
     const double2* bc = b + item * stride + j * ld;
+    // fmax drops NaN operands: a non-finite residual entry is mapped to +Inf
+    // first, so a NaN or Inf solution can never look converged
     double mr = 0.0, mb = 0.0;
     for (int64_t i = threadIdx.x; i < rows; i += 256) {
-        mr = fmax(mr, fabs(rc[i].x) + fabs(rc[i].y));
-        mb = fmax(mb, fabs(bc[i].x) + fabs(bc[i].y));
+        const double ar = fabs(rc[i].x) + fabs(rc[i].y);
+        const double ab = fabs(bc[i].x) + fabs(bc[i].y);
+        mr = fmax(mr, isfinite(ar) ? ar : INFINITY);
+        mb = fmax(mb, isfinite(ab) ? ab : INFINITY);
     }
     for (int off = 32; off > 0; off >>= 1) {
         mr = fmax(mr, __shfl_xor(mr, off));
@@ -1665,7 +1669,7 @@ __global__ __launch_bounds__(256) void k_rel_residual(int64_t rows, const double
     if (threadIdx.x == 0) {
         const double R = fmax(fmax(s[0][0], s[0][1]), fmax(s[0][2], s[0][3]));
         const double B = fmax(fmax(s[1][0], s[1][1]), fmax(s[1][2], s[1][3]));
-        out[item * gridDim.x + j] = (B > 0.0) ? R / B : R;
+        out[item * gridDim.x + j] = !isfinite(B) ? INFINITY : ((B > 0.0) ? R / B : R);
     }
 }
 

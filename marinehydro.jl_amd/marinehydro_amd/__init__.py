@@ -13,9 +13,15 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   integrate_pressure, airy_waves_potential, airy_waves_velocity,
                   airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep,
                   calculate_radiation_forces, DiffractionForce, Tangents, shape_tangents,
-                  freq_sweep_grad)
+                  freq_sweep_grad, froude_krylov, froude_krylov_grad)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
                      mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
                      translational_dofs, rotational_dofs, rigid_dofs)
+from .problems import (FloatingBody, combine_floatingbodies, LinearPotentialFlowProblem,
+                       DiffractionProblem, RadiationProblem, LinearPotentialFlowResult,
+                       DiffractionResult, RadiationResult, make_result, problems_from_data,
+                       solve_problem, solve_all_problems, assemble_hydrodynamic_coefficients,
+                       compute_hydrodynamic_coefficients, create_DimStack, DimArray,
+                       compute_and_label_hydrodynamic_coefficients)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -58,6 +58,9 @@ _SIGS = {
     "mh_freq_sweep_grad": (_int, [_vp, _vp, _int, _int, _int, _i64, _vp, _d, _d, _i64, _vp, _i64,
                                   _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                   _vp, _vp]),
+    "mh_froude_krylov": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _vp]),
+    "mh_froude_krylov_grad": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _i64, _vp,
+                                     _vp, _vp, _vp, _vp, _vp, _vp]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   _d, _d, _d, _vp]),
     "mh_diffraction_force_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,
@@ -103,6 +106,21 @@ def check(rc: int, what: str = "") -> None:
 _contexts: dict = {}
 
 
+def destroy_contexts() -> None:
+    """Releases every library context (streams, events, device workspace,
+    pinned host memory) while the HIP runtime is still alive.  Registered
+    with atexit on first use, so the handles never outlive the runtime or a
+    profiler attached to it; safe to call more than once."""
+    if _lib is None:
+        return
+    while _contexts:
+        _, h = _contexts.popitem()
+        _lib.mh_context_destroy(h)
+
+
+_atexit_registered = False
+
+
 def context(device: int = 0):
     """Library context bound to `device` and torch's current stream there."""
     import torch
@@ -111,6 +129,14 @@ def context(device: int = 0):
     key = (device, stream)
     ctx = _contexts.get(key)
     if ctx is None:
+        global _atexit_registered
+        if not _atexit_registered:
+            import atexit
+
+            # registered after torch is imported, so it runs before torch's
+            # own exit handlers (atexit runs last-in, first-out)
+            atexit.register(destroy_contexts)
+            _atexit_registered = True
         h = _vp()
         check(lib().mh_context_create(device, _vp(stream) if stream else None, C.byref(h)),
               "mh_context_create")

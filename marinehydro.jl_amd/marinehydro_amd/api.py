@@ -398,8 +398,10 @@ def diffraction_force(potential, mesh, omega, dof):
 
 
 def FroudeKrylovForce(mesh, omega, dof, beta=0.0):
-    """src/waves.jl:62-66."""
-    return integrate_pressure(mesh, airy_waves_pressure(mesh.centers, omega, beta), dof)
+    """src/waves.jl:62-66 (and :160-164 with a heading): the incident-wave
+    pressure integrated over the hull, computed by libmhip."""
+    FK = froude_krylov(mesh, [omega], [dof], betas=[beta])
+    return complex(FK[0, 0, 0])
 
 
 # ---------------------------------------------------------------------------
@@ -590,3 +592,76 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
     gB_ = np.ascontiguousarray(out.get("gB").transpose(0, 1, 3, 2)[:, :npar])
     gF_ = np.ascontiguousarray(out.get("gF")[:, :npar, :nbeta, :])
     return A_, B_, F_, gA_, gB_, gF_
+
+
+# ---------------------------------------------------------------------------
+# Froude-Krylov and excitation forces (src/waves.jl:62-66, :150-164;
+# src/problems_and_results.jl:134-136)
+# ---------------------------------------------------------------------------
+def _fk_inputs(torch, dm, omegas, dof_fields, betas):
+    n = dm.nfaces
+    om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
+    be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
+    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+    dofs = torch.from_numpy(np.ascontiguousarray(np.stack([f.T for f in fields]))).to(
+        f"cuda:{dm.device}")
+    return om, be, fields, dofs
+
+
+def froude_krylov(mesh, omegas, dof_fields, betas=(0.0,), device=0, g=None, rho=None):
+    """Froude-Krylov forces for every frequency, heading and influenced dof
+    on the GPU: FK[w, b, i] (numpy complex128)."""
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    om, be, fields, dofs = _fk_inputs(torch, dm, omegas, dof_fields, betas)
+    nw, nb, nd = om.shape[0], be.shape[0], len(fields)
+    out = _Outputs(torch, f"cuda:{dm.device}", [("FK", (nw, max(nb, 1), max(nd, 1)), True)])
+    L.check(L.lib().mh_froude_krylov(L.context(dm.device), C.byref(dm.struct), nw, om.ctypes.data,
+                                     float(g or SETTINGS.g), float(rho or SETTINGS.rho), nd,
+                                     dofs.data_ptr(), nb, be.ctypes.data, out.ptr("FK")),
+            "froude_krylov")
+    return np.ascontiguousarray(out.get("FK")[:, :nb, :nd])
+
+
+def froude_krylov_grad(mesh, omegas, dof_fields, betas=(0.0,), tangents: "Tangents" = None,
+                       device=0, g=None, rho=None):
+    """froude_krylov plus derivatives along the tangent directions:
+    returns (FK[w, b, i], dFK[w, m, b, i])."""
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    n = dm.nfaces
+    om, be, fields, dofs = _fk_inputs(torch, dm, omegas, dof_fields, betas)
+    nw, nb, nd = om.shape[0], be.shape[0], len(fields)
+    T = tangents
+    npar = T.nparam
+    dev = f"cuda:{dm.device}"
+    dC = _dev_tangent(torch, T.dcenters, (npar, n, 3), dev)
+    dN = _dev_tangent(torch, T.dnormals, (npar, n, 3), dev)
+    dA = _dev_tangent(torch, T.dareas, (npar, n), dev)
+    ddof = _dev_tangent(torch, T.ddof_fields, (npar, nd, n, 3), dev)
+    dom = None if T.domegas is None else np.ascontiguousarray(
+        np.asarray(T.domegas, dtype=np.float64).reshape(npar, nw))
+    out = _Outputs(torch, dev, [("FK", (nw, max(nb, 1), max(nd, 1)), True),
+                                ("dFK", (nw, max(npar, 1), max(nb, 1), max(nd, 1)), True)])
+    ptr = lambda t: t.data_ptr() if t is not None else None
+    L.check(L.lib().mh_froude_krylov_grad(
+        L.context(dm.device), C.byref(dm.struct), nw, om.ctypes.data, float(g or SETTINGS.g),
+        float(rho or SETTINGS.rho), nd, dofs.data_ptr(), nb, be.ctypes.data, npar, ptr(dC), ptr(dN),
+        ptr(dA), ptr(ddof), dom.ctypes.data if dom is not None else None, out.ptr("FK"),
+        out.ptr("dFK")), "froude_krylov_grad")
+    return (np.ascontiguousarray(out.get("FK")[:, :nb, :nd]),
+            np.ascontiguousarray(out.get("dFK")[:, :npar, :nb, :nd]))
+
+
+def _dev_tangent(torch, a, shape, dev):
+    """(nparam, n, c) host tangents -> device, column-major per parameter."""
+    if a is None:
+        return None
+    if torch.is_tensor(a):
+        return a
+    a = np.asarray(a, dtype=np.float64).reshape(shape)
+    if a.ndim == 3:
+        a = np.transpose(a, (0, 2, 1))
+    elif a.ndim == 4:
+        a = np.transpose(a, (0, 1, 3, 2))
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)

@@ -860,3 +860,35 @@ int orc_freq_solve(int m, int direct, int64_t nf, const double* V, int64_t nv,
     free(phi);
     return info;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Froude-Krylov force, src/waves.jl:62-66 / :150-164: the incident-wave     */
+/* pressure p = i omega rho phi_Airy (airy_waves_pressure, :150-154, with    */
+/* airy_waves_potential :112-119) at the panel centres, integrated by        */
+/* integrate_pressure (:21-25): F_FK = sum_i p_i (-n_i . dof_i) a_i.          */
+/* FK[2*(d + b*ndof)] = Re, [+1] = Im for influenced dof d and heading b.    */
+/* ------------------------------------------------------------------------ */
+void orc_froude_krylov(int64_t nf, const double* C, const double* N,
+                       const double* A, double omega, double g, double rho,
+                       int64_t ndof, const double* dofs, int64_t nbeta,
+                       const double* betas, double* FK) {
+    double k = omega * omega / g;
+    for (int64_t b = 0; b < nbeta; ++b) {
+        double cb = cos(betas[b]), sb = sin(betas[b]);
+        for (int64_t d = 0; d < ndof; ++d) {
+            const double* f = dofs + d * nf * 3;
+            zc force = 0;
+            for (int64_t i = 0; i < nf; ++i) {
+                double x = C[i], y = C[i + nf], z = C[i + 2 * nf];
+                double wbar = x * cb + y * sb;
+                zc phi0 = -I * g / omega * exp(k * z) * cexp(I * k * wbar);
+                zc p = I * omega * rho * phi0;
+                double amp = -(f[i] * N[i] + f[i + nf] * N[i + nf] +
+                               f[i + 2 * nf] * N[i + 2 * nf]);
+                force += p * amp * A[i];
+            }
+            FK[2 * (d + b * ndof)] = creal(force);
+            FK[2 * (d + b * ndof) + 1] = cimag(force);
+        }
+    }
+}

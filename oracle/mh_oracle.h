@@ -113,6 +113,14 @@ int orc_freq_solve(int gf_mask, int direct, int64_t nf, const double* vertices,
                    int64_t ndof, const double* dof_fields, int64_t nbeta,
                    const double* betas, double* AB, double* F, int nthreads);
 
+/* ---- Froude-Krylov force (src/waves.jl:62-66, :150-164) ---------------- */
+/* FK[2*(d + b*ndof)]: complex force on influenced dof d, heading b.  dofs:
+ * ndof fields of nf x 3 column-major each. */
+void orc_froude_krylov(int64_t nf, const double* C, const double* N,
+                       const double* A, double omega, double g, double rho,
+                       int64_t ndof, const double* dofs, int64_t nbeta,
+                       const double* betas, double* FK);
+
 #ifdef __cplusplus
 }
 #endif

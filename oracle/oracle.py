@@ -75,6 +75,8 @@ def lib():
             "orc_freq_solve": (C.c_int, [C.c_int, C.c_int, C.c_int64, _dp, C.c_int64, _ip, _dp, _dp,
                                          _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int64,
                                          _dp, C.c_int64, _dp, _dp, _dp, C.c_int]),
+            "orc_froude_krylov": (None, [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                         C.c_double, C.c_int64, _dp, C.c_int64, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -262,3 +264,24 @@ def freq_solve(mask, mesh, omega, dof_fields, betas=(), direct=True, g=9.81, rho
     B = AB[ndof * ndof:].reshape(ndof, ndof, order="F")
     Fc = (F[0::2] + 1j * F[1::2])[: ndof * len(bet)].reshape(ndof, len(bet), order="F")
     return A, B, Fc
+
+
+def froude_krylov(mesh, omega, dof_fields, betas=(0.0,), g=9.81, rho=1000.0):
+    """Froude-Krylov force (reference src/waves.jl:62-66, :150-164).
+
+    Returns FK[ndof, nbeta] (complex), influenced dof by heading."""
+    nf = mesh.nfaces
+    Cc = np.asfortranarray(mesh.centers, dtype=np.float64)
+    N = np.asfortranarray(mesh.normals, dtype=np.float64)
+    A = np.ascontiguousarray(mesh.areas, dtype=np.float64)
+    ndof = len(dof_fields)
+    D = np.zeros((ndof, nf, 3), dtype=np.float64)
+    for d, f in enumerate(dof_fields):
+        D[d] = np.broadcast_to(np.asarray(f, dtype=np.float64), (nf, 3))
+    Dcm = np.ascontiguousarray(np.transpose(D, (0, 2, 1)))
+    bet = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
+    out = np.zeros(2 * ndof * max(len(bet), 1))
+    lib().orc_froude_krylov(nf, Cc.ctypes.data_as(_dp), N.ctypes.data_as(_dp), A.ctypes.data_as(_dp),
+                            float(omega), float(g), float(rho), ndof, Dcm.ctypes.data_as(_dp),
+                            len(bet), bet.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return (out[0::2] + 1j * out[1::2])[: ndof * len(bet)].reshape(ndof, len(bet), order="F")
