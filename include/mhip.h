@@ -152,6 +152,21 @@ int mh_assemble(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
                 int formulation, int64_t nk, const double* wavenumbers,
                 double* S, double* D, double* K, int64_t ld, int64_t stride);
 
+/* mh_assemble without any allocation in the call.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The panel records, wavenumbers and error flag live in one
+ * fixed device array of the library (about 0.8 MB per device, held until
+ * the process ends) instead of the context workspace; the one-pass pair
+ * kernel needs no other scratch.  Same arguments and results as
+ * mh_assemble; at most 4096 panels and 256 wavenumbers per call (larger
+ * sizes return MH_ERR_ARGUMENT).  Calls on one device share the array:
+ * calls that overlap on a device (two threads or unordered streams) race.
+ * Synchronises (reads the mesh-index flag). */
+int mh_assemble_noalloc(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
+                        int formulation, int64_t nk, const double* wavenumbers,
+                        double* S, double* D, double* K, int64_t ld, int64_t stride);
+
 /* ---- dense solve -------------------------------------------------------
  * Batched LU with partial pivoting (zgetrf semantics; pivot = max |re|+|im|)
  * in place; ipiv: n x batch, 0-based (pivot row of step k of item b at

@@ -279,16 +279,25 @@ __global__ __launch_bounds__(256) void k_assemble_sym(
     }
 }
 
-int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out) {
+__global__ void k_clear_int(int* __restrict__ p, int count) {
+    const int t = (int)threadIdx.x;
+    if (t < count) p[t] = 0;
+}
+
+int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out, int* flag_buf) {
     const int64_t nf = m->nfaces;
     if (nf == 0) return MH_OK;
-    MH_HIP_CHECK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+    int* d_flag = flag_buf ? flag_buf : ctx->d_flag;
+    if (flag_buf)  // a fill kernel, not a memset node, for callers that capture
+        hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, ctx->stream, d_flag, 1);
+    else
+        MH_HIP_CHECK(hipMemsetAsync(d_flag, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_pack_panels, dim3((unsigned)((nf + 127) / 128)), dim3(128), 0, ctx->stream,
                        nf, m->nvertices, m->vertices, m->faces, m->centers, m->normals, m->areas,
-                       m->radii, out, ctx->d_flag);
+                       m->radii, out, d_flag);
     MH_HIP_CHECK(hipGetLastError());
     int flag = 0;
-    MH_HIP_CHECK(hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP_CHECK(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (flag) {
         set_error("mesh: a face references a vertex index outside [0, nvertices)");
@@ -396,6 +405,33 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
     else
         dispatch_form<0>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S,
                          D, K, ld, stride, DK32);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+// One-pass assembly (k_assemble: Rankine and wave terms of each pair in one
+// kernel, no frequency-independent scratch matrices).  No kernel timer.
+int launch_assemble_onepass(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, int gf, int form,
+                            int64_t nk, const double* d_k, double2* S, double2* D, double2* K, int64_t ld,
+                            int64_t stride) {
+    const int64_t n = m->nfaces;
+    if (n == 0 || nk == 0) return MH_OK;
+    const int64_t tiles = (n + 63) / 64;
+    const int64_t groups = (nk + kFG - 1) / kFG;
+    if (tiles > 0x7fffffff || groups > 65535) {
+        set_error("assemble: problem too large for one launch");
+        return MH_ERR_ARGUMENT;
+    }
+    dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)groups);
+    if (gf & GF_WU)
+        dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S, D, K, ld,
+                         stride, nullptr);
+    else if (gf & GF_DELHOMMEAU)
+        dispatch_form<2>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S, D, K, ld,
+                         stride, nullptr);
+    else
+        dispatch_form<0>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S, D, K, ld,
+                         stride, nullptr);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }

@@ -109,9 +109,15 @@ int ensure_events(Context* ctx, size_t count);
     } while (0)
 
 // ---- launchers (assemble.hip) ---------------------------------------------
-// Packs the mesh into PanelRec records; flags a face index out of range in
-// *d_flag.
-int launch_pack_panels(Context* ctx, const mh_mesh_t* mesh, PanelRec* out);
+// Packs the mesh into PanelRec records; a face index out of range is flagged
+// in flag_buf (one int, cleared by a fill kernel first) or, when flag_buf is
+// null, in ctx->d_flag, and returned as MH_ERR_ARGUMENT (synchronises).
+int launch_pack_panels(Context* ctx, const mh_mesh_t* mesh, PanelRec* out, int* flag_buf = nullptr);
+// Assembly in one kernel per launch, without the frequency-independent
+// scratch matrices launch_assemble keeps in the context.
+int launch_assemble_onepass(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels, int gf_mask,
+                            int form, int64_t nk, const double* d_k, double2* S, double2* D, double2* K,
+                            int64_t ld, int64_t stride);
 int launch_assemble(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels,
                     int gf_mask, int form, int64_t nk, const double* d_k, double2* S,
                     double2* D, double2* K, int64_t ld, int64_t stride, float2* DK32 = nullptr);

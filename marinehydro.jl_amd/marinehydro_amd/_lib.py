@@ -50,6 +50,7 @@ _SIGS = {
     "mh_gf_eval": (_int, [_vp, _int, _int, _int, _i64, _vp, _vp, _d, _vp]),
     "mh_wave_term_eval": (_int, [_vp, _int, _i64, _vp, _vp, _vp]),
     "mh_assemble": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
+    "mh_assemble_noalloc": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
     "mh_lu_factor": (_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "mh_lu_solve": (_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _int, _i64, _vp, _i64, _i64]),
     "mh_solve": (_int, [_vp, _i64, _i64, _int, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
