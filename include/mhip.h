@@ -188,6 +188,44 @@ int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct,
              const double* bc, int64_t nrhs, double* phi, int64_t ldb,
              int64_t bstride);
 
+/* ---- reverse mode (matrix-level cotangents) ----------------------------
+ * Cotangent convention of Zygote / ChainRules for complex arrays: zbar =
+ * dL/dRe z + i dL/dIm z, so a real input u receives sum Re(conj(zbar) dz/du).
+ *
+ * Pullback of assemble_matrices(gfs, mesh, k; direct) (src/matrix_assembly.jl:
+ * 4-42, what Zygote differentiates in test/matrix_assembly_differentiation.jl:
+ * 34-64) for nk wavenumbers: Sbar and DKbar (nk matrices each, ld, stride;
+ * either may be NULL = zero) are the cotangents of S and of D (formulation
+ * MH_DIRECT) or K (MH_INDIRECT).  Outputs (device, each may be NULL):
+ * dvertices nv x 3, dcenters nf x 3, dnormals nf x 3 (column-major), dareas
+ * nf, summed over the wavenumbers, and dwavenumbers (nk).  The mesh fields
+ * are independent inputs, as in the reference's Mesh: centres enter as the
+ * collocation points and as the source-panel centres, vertices only through
+ * the near-field Rankine formula.  Radii select branches and get no
+ * cotangent.  The pair terms are recomputed, never stored.
+ * wavenumbers: HOST array.  Synchronises. */
+int mh_assemble_vjp(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
+                    int formulation, int64_t nk, const double* wavenumbers,
+                    const double* Sbar, const double* DKbar, int64_t ld,
+                    int64_t stride, double* dvertices, double* dcenters,
+                    double* dnormals, double* dareas, double* dwavenumbers);
+/* Pullback of solve(D, S, bc; direct) (src/matrix_assembly.jl:105-133; the
+ * implicit_linear rrule of ImplicitAD at :120): LU, ipiv are the factors of
+ * D (direct) or K (indirect) from mh_lu_factor; phibar is the cotangent of
+ * phi (n x nrhs per item, ldb, bstride, like bc).  Outputs (device, each may
+ * be NULL): DKbar and Sbar (n x n per item, ld, stride), bcbar (like bc):
+ *   direct  (y = S bc, phi = D \ y): ybar = D^-H phibar, Dbar = -ybar phi^H,
+ *            Sbar = ybar bc^H, bcbar = S^H ybar;
+ *   indirect (sigma = K \ bc, phi = S sigma): Sbar = phibar sigma^H,
+ *            bcbar = K^-H S^H phibar, Kbar = -bcbar sigma^H.
+ * One transposed solve on the given factors (plus one forward solve that
+ * recomputes phi or sigma).  Synchronises. */
+int mh_solve_vjp(mh_context_t ctx, int64_t n, int64_t batch, int direct,
+                 const double* LU, const int64_t* ipiv, const double* S,
+                 int64_t ld, int64_t stride, const double* bc, int64_t nrhs,
+                 const double* phibar, int64_t ldb, int64_t bstride,
+                 double* DKbar, double* Sbar, double* bcbar);
+
 /* ---- one frequency sweep, end to end -----------------------------------
  * For every omega: k = omega^2/g; assemble; radiation BCs for each dof
  * field (radiation_bc, src/waves.jl:89-102) and Airy diffraction BCs for

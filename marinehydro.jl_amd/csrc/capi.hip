@@ -943,6 +943,117 @@ int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int
                       dof_fields, nbeta, betas, A, B, F, nparam > 0 ? &ga : nullptr);
 }
 
+// ---- reverse mode: assembly and solve ---------------------------------------
+int mh_assemble_vjp(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int formulation, int64_t nk,
+                    const double* wavenumbers, const double* Sbar, const double* DKbar, int64_t ld,
+                    int64_t stride, double* dvertices, double* dcenters, double* dnormals, double* dareas,
+                    double* dwavenumbers) {
+    TRY(check_ctx(ctx));
+    TRY(check_mesh(mesh));
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble_vjp: bad Green-function mask");
+    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)), "assemble_vjp: choose one wave term");
+    MH_REQUIRE(formulation == MH_DIRECT || formulation == MH_INDIRECT,
+               "assemble_vjp: formulation must be MH_DIRECT or MH_INDIRECT");
+    MH_REQUIRE(nk >= 0, "assemble_vjp: negative nk");
+    const int64_t n = mesh->nfaces;
+    if (n == 0 || nk == 0) return MH_OK;
+    MH_REQUIRE(wavenumbers, "assemble_vjp: null wavenumbers");
+    MH_REQUIRE(ld >= n && (nk == 1 || stride >= ld * n), "assemble_vjp: ld/stride too small");
+    Context* c = CTX(ctx);
+    Bump bump;
+    const size_t o_p = bump.take(sizeof(PanelRec) * (size_t)n);
+    const size_t o_k = bump.take(sizeof(double) * (size_t)nk);
+    const size_t o_w = bump.take(sizeof(double) * (size_t)assemble_vjp_ws_doubles(n, nk));
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    PanelRec* P = reinterpret_cast<PanelRec*>(ws + o_p);
+    double* dk = reinterpret_cast<double*>(ws + o_k);
+    MH_HIP_CHECK(hipMemcpyAsync(dk, wavenumbers, sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
+    TRY(launch_pack_panels(c, mesh, P));
+    TRY(launch_assemble_vjp(c, mesh, P, gf_mask, formulation, nk, dk, reinterpret_cast<const double2*>(Sbar),
+                            reinterpret_cast<const double2*>(DKbar), ld, stride,
+                            reinterpret_cast<double*>(ws + o_w), dvertices, dcenters, dnormals, dareas,
+                            dwavenumbers));
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MH_OK;
+}
+
+int mh_solve_vjp(mh_context_t ctx, int64_t n, int64_t batch, int direct, const double* LU, const int64_t* ipiv,
+                 const double* S, int64_t ld, int64_t stride, const double* bc, int64_t nrhs, const double* phibar,
+                 int64_t ldb, int64_t bstride, double* DKbar, double* Sbar, double* bcbar) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(n >= 0 && batch >= 0 && nrhs >= 0, "solve_vjp: negative size");
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    MH_REQUIRE(LU && ipiv && S && bc && phibar, "solve_vjp: null array");
+    MH_REQUIRE(ld >= n && ldb >= n, "solve_vjp: leading dimension too small");
+    MH_REQUIRE(batch == 1 || (stride >= ld * n && bstride >= ldb * nrhs), "solve_vjp: stride too small");
+    MH_REQUIRE(n <= kMaxSolveN, "solve_vjp: at most 4096 unknowns");
+    Context* c = CTX(ctx);
+    const int64_t bs = n * nrhs;  // internal n x nrhs blocks
+    Bump bump;
+    const size_t o_perm = bump.take(sizeof(int) * (size_t)(n * batch));
+    const size_t o_inv = bump.take(sizeof(double2) * (size_t)tri_inv_elems(n, batch));
+    const size_t o_a = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_b = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_c = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    int* perm = reinterpret_cast<int*>(ws + o_perm);
+    double2* inv = reinterpret_cast<double2*>(ws + o_inv);
+    double2* ta = reinterpret_cast<double2*>(ws + o_a);
+    double2* tb = reinterpret_cast<double2*>(ws + o_b);
+    double2* tc = reinterpret_cast<double2*>(ws + o_c);
+    const double2* LU2 = reinterpret_cast<const double2*>(LU);
+    const double2* S2 = reinterpret_cast<const double2*>(S);
+    const double2* b2 = reinterpret_cast<const double2*>(bc);
+    const double2* pb = reinterpret_cast<const double2*>(phibar);
+    TRY(tri_prepare(c, n, batch, LU2, ld, stride, ipiv, perm, inv));
+    // ta <- conj(phibar)
+    TRY(conj_batched(c, n, nrhs, batch, pb, ldb, bstride, ta, n, bs));
+    if (direct) {
+        // y = S bc, phi = D \ y:  ybar = D^-H phibar, Dbar = -ybar phi^H,
+        // Sbar = ybar bc^H, bcbar = S^H ybar
+        TRY(zgemm_nn_batched(c, n, nrhs, batch, S2, ld, stride, b2, ldb, bstride, tb, n, bs));
+        TRY(tri_solve(c, n, batch, LU2, ld, stride, perm, inv, 0, 3, nrhs, tb, n, bs));      // tb = phi
+        TRY(tri_solve(c, n, batch, LU2, ld, stride, perm, inv, 1, 3, nrhs, ta, n, bs));      // ta = conj(ybar)
+        TRY(conj_batched(c, n, nrhs, batch, ta, n, bs, tc, n, bs));                          // tc = ybar
+        if (DKbar)
+            TRY(outer_conj_batched(c, n, nrhs, batch, -1.0, tc, tb, n, bs, reinterpret_cast<double2*>(DKbar), ld,
+                                   stride));
+        if (Sbar) {
+            // bc may have its own layout: copy into tb (phi is no longer needed)
+            TRY(conj_batched(c, n, nrhs, batch, b2, ldb, bstride, tb, n, bs));
+            TRY(conj_batched(c, n, nrhs, batch, tb, n, bs, tb, n, bs));
+            TRY(outer_conj_batched(c, n, nrhs, batch, 1.0, tc, tb, n, bs, reinterpret_cast<double2*>(Sbar), ld,
+                                   stride));
+        }
+        if (bcbar) {  // S^H ybar = conj(S^T conj(ybar))
+            TRY(zgemm_tn_batched(c, n, nrhs, batch, S2, ld, stride, ta, n, bs, tb, n, bs));
+            TRY(conj_batched(c, n, nrhs, batch, tb, n, bs, reinterpret_cast<double2*>(bcbar), ldb, bstride));
+        }
+    } else {
+        // sigma = K \ bc, phi = S sigma:  Sbar = phibar sigma^H,
+        // sigmabar = S^H phibar, bcbar = K^-H sigmabar, Kbar = -bcbar sigma^H
+        TRY(conj_batched(c, n, nrhs, batch, b2, ldb, bstride, tb, n, bs));
+        TRY(conj_batched(c, n, nrhs, batch, tb, n, bs, tb, n, bs));                          // tb = bc
+        TRY(tri_solve(c, n, batch, LU2, ld, stride, perm, inv, 0, 3, nrhs, tb, n, bs));      // tb = sigma
+        if (Sbar) {
+            TRY(conj_batched(c, n, nrhs, batch, ta, n, bs, tc, n, bs));                      // tc = phibar
+            TRY(outer_conj_batched(c, n, nrhs, batch, 1.0, tc, tb, n, bs, reinterpret_cast<double2*>(Sbar), ld,
+                                   stride));
+        }
+        // conj(sigmabar) = S^T conj(phibar);  K^T t = conj(sigmabar) gives t = conj(bcbar)
+        TRY(zgemm_tn_batched(c, n, nrhs, batch, S2, ld, stride, ta, n, bs, tc, n, bs));
+        TRY(tri_solve(c, n, batch, LU2, ld, stride, perm, inv, 1, 3, nrhs, tc, n, bs));
+        TRY(conj_batched(c, n, nrhs, batch, tc, n, bs, ta, n, bs));                          // ta = bcbar
+        if (bcbar) TRY(conj_batched(c, n, nrhs, batch, tc, n, bs, reinterpret_cast<double2*>(bcbar), ldb, bstride));
+        if (DKbar)
+            TRY(outer_conj_batched(c, n, nrhs, batch, -1.0, ta, tb, n, bs, reinterpret_cast<double2*>(DKbar), ld,
+                                   stride));
+    }
+    return tri_check(c);
+}
+
 // ---- Froude-Krylov force ----------------------------------------------------
 static int fk_impl(Context* c, const mh_mesh_t* mesh, int64_t nomega, const double* omegas, double g,
                    double rho, int64_t ndof, const double* dofs, int64_t nbeta, const double* betas,

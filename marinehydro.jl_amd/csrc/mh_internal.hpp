@@ -220,6 +220,22 @@ int launch_froude_krylov(Context* ctx, const mh_mesh_t* mesh, int64_t nomega, co
                          const double* dN, const double* dA, const double* ddofs, const double* d_dom,
                          double2* dFK);
 
+// ---- launchers (vjp.hip) ----------------------------------------------------
+// Reverse mode of the assembly: cotangents Sb, Db (nk complex matrices, ld,
+// stride; either may be null) -> dV (nv x 3), dC, dN (nf x 3), dA (nf) summed
+// over the wavenumbers, and dk (nk); outputs may be null.  ws holds
+// assemble_vjp_ws_doubles(n, nk) doubles.
+int64_t assemble_vjp_ws_doubles(int64_t n, int64_t nk);
+int launch_assemble_vjp(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, int gf, int form, int64_t nk,
+                        const double* d_k, const double2* Sb, const double2* Db, int64_t ld, int64_t stride,
+                        double* ws, double* dV, double* dC, double* dN, double* dA, double* dk);
+// Y = conj(X), n x cols per item (X and Y may be the same buffer)
+int conj_batched(Context* ctx, int64_t rows, int64_t cols, int64_t batch, const double2* X, int64_t ldx, int64_t sx,
+                 double2* Y, int64_t ldy, int64_t sy);
+// M = alpha X Y^H (X, Y n x nr with ldb, bs; M n x n with ld, stride)
+int outer_conj_batched(Context* ctx, int64_t n, int64_t nr, int64_t batch, double alpha, const double2* X,
+                       const double2* Y, int64_t ldb, int64_t bs, double2* M, int64_t ld, int64_t stride);
+
 // ---- launchers (grad.hip) ---------------------------------------------------
 int launch_pack_tangents(Context* ctx, const mh_mesh_t* m, int P, int64_t nparam, int64_t m0,
                          const double* dV, const double* dC, const double* dN, const double* dA,

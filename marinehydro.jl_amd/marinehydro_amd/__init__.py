@@ -13,7 +13,8 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   integrate_pressure, airy_waves_potential, airy_waves_velocity,
                   airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep,
                   calculate_radiation_forces, DiffractionForce, Tangents, shape_tangents,
-                  freq_sweep_grad, froude_krylov, froude_krylov_grad)
+                  freq_sweep_grad, froude_krylov, froude_krylov_grad,
+                  assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
                      mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
                      translational_dofs, rotational_dofs, rigid_dofs)

@@ -59,6 +59,10 @@ _SIGS = {
     "mh_freq_sweep_grad": (_int, [_vp, _vp, _int, _int, _int, _i64, _vp, _d, _d, _i64, _vp, _i64,
                                   _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                   _vp, _vp]),
+    "mh_assemble_vjp": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp,
+                               _vp, _vp]),
+    "mh_solve_vjp": (_int, [_vp, _i64, _i64, _int, _vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
+                            _i64, _vp, _vp, _vp]),
     "mh_froude_krylov": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _vp]),
     "mh_froude_krylov_grad": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _i64, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -665,3 +665,106 @@ def _dev_tangent(torch, a, shape, dev):
     elif a.ndim == 4:
         a = np.transpose(a, (0, 1, 3, 2))
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+# ---------------------------------------------------------------------------
+# reverse mode (the rrules Zygote uses on the reference's assembly and solve)
+# ---------------------------------------------------------------------------
+def _cotangent_batch(torch, X, n, nk, device):
+    """None or an (n, n) / (nk, n, n) complex cotangent (logical [i, j]) ->
+    column-major device batch."""
+    if X is None:
+        return None
+    Xm = _as_colmajor_batch(torch, X, device)
+    if Xm.shape != (nk, n, n):
+        raise ValueError(f"cotangent of shape {tuple(Xm.shape)}, expected ({nk}, {n}, {n})")
+    return Xm
+
+
+def assemble_matrices_vjp(green_functions, mesh, wavenumbers, Sbar, DKbar=None, direct=True, device=0):
+    """Pullback of assemble_matrices (src/matrix_assembly.jl:4-42) in Zygote's
+    convention for complex arrays: returns a dict of real cotangents
+    {"vertices": (nv, 3), "centers": (nf, 3), "normals": (nf, 3),
+    "areas": (nf,), "wavenumbers": (nk,)} for the cotangents Sbar of S and
+    DKbar of D (direct) or K (indirect), each (n, n) or (nk, n, n)."""
+    torch = _torch()
+    dm = _dmesh(mesh, device)
+    n, nv = dm.nfaces, dm.mesh.nvertices
+    ks = np.ascontiguousarray(np.asarray(wavenumbers, dtype=np.float64).reshape(-1))
+    nk = ks.shape[0]
+    Sb = _cotangent_batch(torch, Sbar, n, nk, dm.device)
+    Db = _cotangent_batch(torch, DKbar, n, nk, dm.device)
+    dev = f"cuda:{dm.device}"
+    dV = torch.zeros((3, nv), dtype=torch.float64, device=dev)
+    dC = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    dN = torch.zeros((3, n), dtype=torch.float64, device=dev)
+    dA = torch.zeros(n, dtype=torch.float64, device=dev)
+    dk = torch.zeros(nk, dtype=torch.float64, device=dev)
+    form = L.DIRECT if direct else L.INDIRECT
+    L.check(L.lib().mh_assemble_vjp(L.context(dm.device), C.byref(dm.struct), gf_mask(green_functions), form,
+                                    nk, ks.ctypes.data, Sb.data_ptr() if Sb is not None else None,
+                                    Db.data_ptr() if Db is not None else None, n, n * n, dV.data_ptr(),
+                                    dC.data_ptr(), dN.data_ptr(), dA.data_ptr(), dk.data_ptr()),
+            "assemble_matrices_vjp")
+    return {"vertices": dV.cpu().numpy().T.copy(), "centers": dC.cpu().numpy().T.copy(),
+            "normals": dN.cpu().numpy().T.copy(), "areas": dA.cpu().numpy(),
+            "wavenumbers": dk.cpu().numpy()}
+
+
+def rrule_assemble_matrices(green_functions, mesh, wavenumber, direct=True, device=0):
+    """(S, DK), pullback: the ChainRulesCore.rrule a Julia binding attaches
+    to assemble_matrices.  pullback(Sbar, DKbar) -> dict of cotangents."""
+    S, DK = assemble_matrices(green_functions, mesh, wavenumber, direct, device)
+
+    def pullback(Sbar, DKbar=None):
+        out = assemble_matrices_vjp(green_functions, mesh, [wavenumber], Sbar, DKbar, direct, device)
+        out["wavenumber"] = float(out.pop("wavenumbers")[0])
+        return out
+
+    return (S, DK), pullback
+
+
+def solve_vjp(D, S, bc, phibar, direct=True, device=0):
+    """Pullback of solve(D, S, bc; direct) (src/matrix_assembly.jl:105-133,
+    the implicit_linear rrule): returns (Dbar, Sbar, bcbar) for the
+    cotangent phibar of phi, in Zygote's convention for complex arrays."""
+    torch = _torch()
+    Dm = _as_colmajor_batch(torch, D, device)
+    Sm = _as_colmajor_batch(torch, S, device)
+    n = Dm.shape[1]
+    b = bc if torch.is_tensor(bc) else torch.from_numpy(np.asarray(bc, dtype=np.complex128))
+    pb = phibar if torch.is_tensor(phibar) else torch.from_numpy(np.asarray(phibar, dtype=np.complex128))
+    b = b.to(device=f"cuda:{device}", dtype=torch.complex128)
+    pb = pb.to(device=f"cuda:{device}", dtype=torch.complex128)
+    vec = b.dim() == 1
+    if vec:
+        b, pb = b.reshape(-1, 1), pb.reshape(-1, 1)
+    if Dm.shape[0] != 1 or Sm.shape != Dm.shape or b.shape[0] != n or pb.shape != b.shape:
+        raise ValueError("solve_vjp: D, S must be n x n, bc and phibar of the same shape")
+    nrhs = b.shape[1]
+    bm = b.transpose(0, 1).contiguous()
+    pm = pb.transpose(0, 1).contiguous()
+    ipiv = torch.empty(n, dtype=torch.int64, device=f"cuda:{device}")
+    ctx = L.context(device)
+    L.check(L.lib().mh_lu_factor(ctx, n, 1, Dm.data_ptr(), n, n * n, ipiv.data_ptr()), "solve_vjp: lu")
+    Db = torch.empty_like(Dm)
+    Sb = torch.empty_like(Sm)
+    bb = torch.empty_like(bm)
+    L.check(L.lib().mh_solve_vjp(ctx, n, 1, int(bool(direct)), Dm.data_ptr(), ipiv.data_ptr(), Sm.data_ptr(), n,
+                                 n * n, bm.data_ptr(), nrhs, pm.data_ptr(), n, n * nrhs, Db.data_ptr(),
+                                 Sb.data_ptr(), bb.data_ptr()), "solve_vjp")
+    Dbar = Db[0].transpose(0, 1)
+    Sbar = Sb[0].transpose(0, 1)
+    bcbar = bb.transpose(0, 1)
+    return Dbar, Sbar, (bcbar[:, 0] if vec else bcbar)
+
+
+def rrule_solve(D, S, bc, direct=True, device=0):
+    """phi, pullback: the rrule of solve (ImplicitAD.implicit_linear's
+    adjoint).  pullback(phibar) -> (Dbar, Sbar, bcbar)."""
+    phi = solve(D, S, bc, direct, device)
+
+    def pullback(phibar):
+        return solve_vjp(D, S, bc, phibar, direct, device)
+
+    return phi, pullback
