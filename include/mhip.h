@@ -119,6 +119,19 @@ int mh_synchronize(mh_context_t ctx);
 int mh_profile_enable(mh_context_t ctx, int on);
 int mh_profile_report(mh_context_t ctx, char* buf, int64_t cap);
 
+/* Mixed-precision control (no reference counterpart; the reference solves
+ * in FP64 only).  max_sweeps: iterative-refinement sweeps allowed per
+ * frequency chunk (default 12, range 1-100); fp64_fallback (default 1): a
+ * chunk whose refinement does not reach the 1e-10 relative residual, whose
+ * FP32 factorisation meets an exact zero pivot, or whose matrix entries lie
+ * beyond the split range of the FP32 factorisation (|x| >= 16) is solved
+ * again in FP64 in the same call; with 0 the call returns
+ * MH_ERR_NOT_CONVERGED / MH_ERR_SINGULAR instead.  A NaN or Inf residual is
+ * always returned as MH_ERR_NOT_CONVERGED.  mh_fallback_count: chunks solved
+ * again in FP64 since the context was created. */
+int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback);
+int mh_fallback_count(mh_context_t ctx, int64_t* count);
+
 /* ---- Green functions ---------------------------------------------------
  * Batched per-pair evaluation of the sum of the selected terms.
  * Replaces greens / gradient_greens / integral / integral_gradient

@@ -500,7 +500,8 @@ constexpr double kIrTol = 1e-10;
 // systems solve from it with coalesced reads.
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
-                    int nsys, float2* LUt32, double2* Pa) {
+                    int nsys, float2* LUt32, double2* Pa, bool* nonfinite) {
+    *nonfinite = false;
     // a forward and a transposed system form their residuals together, in
     // one pass over A (Pa: workspace of zgemv_pair)
     const bool pair = nsys == 2 && Pa && sys[0].trans == 0 && sys[1].trans == 1 && sys[0].nrhs <= 4 &&
@@ -517,7 +518,8 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
         MH_HIP_CHECK(hipMemcpyAsync(q.R, q.Bm, sizeof(double2) * (size_t)(q.bstride * batch),
                                     hipMemcpyDeviceToDevice, c->stream));
     }
-    for (int it = 0; it < 12; ++it) {
+    const int max_sweeps = c->ir_max_sweeps;
+    for (int it = 0; it < max_sweeps; ++it) {
         const bool both = pair && !done[0] && !done[1];
         size_t off = 0;
         for (int k = 0; k < nsys; ++k) {
@@ -568,7 +570,20 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
         }
         // one FP32 solve leaves a residual far above the tolerance: the first
         // correction is always made, without a host round trip for the check
-        if (it == 0) continue;
+        // (unless it is the only sweep allowed)
+        if (it == 0 && max_sweeps > 1) continue;
+        if (it == 0) {  // residuals of the single correction, for the check below
+            if (nsys > 1) TRY(join_aux(c));
+            size_t o2 = 0;
+            for (int k = 0; k < nsys; ++k) {
+                const IrSystem& q = sys[k];
+                const size_t cnt = (size_t)(batch * q.nrhs);
+                TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
+                MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + o2, q.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                            c->stream));
+                o2 += cnt;
+            }
+        }
         MH_HIP_CHECK(hipStreamSynchronize(c->stream));
         if (nsys > 1 && !c->profiling) MH_HIP_CHECK(hipStreamSynchronize(c->aux));
         off = 0;
@@ -579,15 +594,20 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
                 // every column below the tolerance; a NaN compares false and
                 // keeps the system unconverged
                 bool ok = true;
-                for (size_t e = 0; e < cnt; ++e) ok = ok && (c->h_pin[off + e] < kIrTol);
+                for (size_t e = 0; e < cnt; ++e) {
+                    ok = ok && (c->h_pin[off + e] < kIrTol);
+                    if (!(c->h_pin[off + e] <= 1e300)) *nonfinite = true;
+                }
                 done[k] = ok;
             }
             all = all && done[k];
             off += cnt;
         }
         if (all) return MH_OK;
+        if (*nonfinite) break;  // NaN or Inf: more sweeps cannot help
     }
-    set_error("mixed precision: iterative refinement did not reach a 1e-10 relative residual");
+    set_error(*nonfinite ? "mixed precision: non-finite residual (NaN or Inf in the system or its solution)"
+                         : "mixed precision: iterative refinement did not reach a 1e-10 relative residual");
     return MH_ERR_NOT_CONVERGED;
 }
 
@@ -718,6 +738,8 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
             TRY(launch_pack_tangents(c, mesh, Pg, ga->nparam, 0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
             TRY(launch_rankine_tangents(c, mesh, P, tan, Pg, rank_mask, form, reinterpret_cast<double*>(ws + o_dR)));
         }
+        // FP64 when the mixed attempt of this chunk falls back
+        bool use_mixed = mixed;
         if (mixed) {
             // FP32 LU of D (or K) + FP64 iterative refinement; D/K stays FP64.
             // The forward right-hand side S bc (direct) is formed on the side
@@ -756,25 +778,42 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 }
                 nsys = 2;
             }
-            {
-                const int rc = ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
-                                        nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr,
-                                        o_pa ? reinterpret_cast<double2*>(ws + o_pa) : nullptr);
-                if (rc == MH_ERR_NOT_CONVERGED) TRY(check_info(c, info, nw));  // a more specific cause
-                TRY(rc);
+            bool nonfinite = false;
+            int rc = ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
+                              nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr,
+                              o_pa ? reinterpret_cast<double2*>(ws + o_pa) : nullptr, &nonfinite);
+            if (rc == MH_ERR_NOT_CONVERGED) {
+                const int rc_info = check_info(c, info, nw);  // a more specific cause
+                if (rc_info != MH_OK) rc = rc_info;
             }
-            if (nsys > 1) TRY(join_aux(c));
-            if (!direct) TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
-        } else if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
-            TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
-        } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
-            MH_HIP_CHECK(hipMemcpy2DAsync(aug, sizeof(double2) * stride, rhs, sizeof(double2) * bstride,
-                                          sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
-                                          c->stream));
+            if (rc == MH_ERR_NOT_CONVERGED || rc == MH_ERR_SINGULAR) {
+                // NaN or Inf is an input error; otherwise (refinement too
+                // slow, an FP32 zero pivot, entries beyond the split range)
+                // the chunk is solved again in FP64 from the FP64 D/K,
+                // which the mixed attempt left intact
+                if (nonfinite || !c->fp64_fallback) return rc;
+                if (nsys > 1) TRY(join_aux(c));
+                use_mixed = false;
+                c->fallbacks += 1;
+            } else {
+                TRY(rc);
+                if (nsys > 1) TRY(join_aux(c));
+                if (!direct)
+                    TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
+            }
         }
-        if (!mixed) {
+        if (!use_mixed) {
+            if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
+                TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
+            } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
+                MH_HIP_CHECK(hipMemcpy2DAsync(aug, sizeof(double2) * stride, rhs, sizeof(double2) * bstride,
+                                              sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
+                                              c->stream));
+            }
             MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
-            double2* dDt = (ga && ndof) ? reinterpret_cast<double2*>(ws + o_Dt) : nullptr;
+            // the transposed FP64 factors only when the workspace was laid out
+            // for the FP64 path (a fallback solves the adjoint from the factors)
+            double2* dDt = (ga && ndof && !mixed) ? reinterpret_cast<double2*>(ws + o_Dt) : nullptr;
             TRY(lu_factor(c, n, ncols, nw, dD, ld, stride, piv, info, dDt, (ga && ndof) ? perm : nullptr,
                           inv));  // + transposed factors and solve data
             if (ga && ndof) {  // adjoint solve on aux, alongside the back substitution
@@ -833,7 +872,7 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
                 TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, lam, n, wa, mu, n, wa));
             }
             aS = lam; aDw = lam; sS = rhs; sD = phi; wss = bstride; rhoW = mu;
-        } else if (mixed) {  // lambda = K^-T S^T c
+        } else if (use_mixed) {  // lambda = K^-T S^T c
             aS = cw; aDw = lam; sS = ybuf; sD = ybuf; wss = bstride; rhoW = lam;
         } else {
             aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
@@ -1052,6 +1091,21 @@ int mh_solve_vjp(mh_context_t ctx, int64_t n, int64_t batch, int direct, const d
                                    stride));
     }
     return tri_check(c);
+}
+
+int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(max_sweeps >= 1 && max_sweeps <= 100, "set_refinement: max_sweeps must be in [1, 100]");
+    CTX(ctx)->ir_max_sweeps = max_sweeps;
+    CTX(ctx)->fp64_fallback = fp64_fallback != 0;
+    return MH_OK;
+}
+
+int mh_fallback_count(mh_context_t ctx, int64_t* count) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(count != nullptr, "fallback_count: null output");
+    *count = CTX(ctx)->fallbacks;
+    return MH_OK;
 }
 
 // ---- Froude-Krylov force ----------------------------------------------------

@@ -58,6 +58,12 @@ struct Context {
     size_t rk_bytes = 0;
     bool profiling = false;
     std::vector<ProfRec> prof;
+    // mixed precision: refinement sweeps allowed, whether a frequency chunk
+    // that does not converge is solved again in FP64, and how often that
+    // happened since the context was created
+    int ir_max_sweeps = 12;
+    bool fp64_fallback = true;
+    int64_t fallbacks = 0;
 };
 
 // Brackets one kernel launch with events on the context's stream when the

@@ -17,7 +17,8 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
                      mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
-                     translational_dofs, rotational_dofs, rigid_dofs)
+                     translational_dofs, rotational_dofs, rigid_dofs, axisymmetric_mesh,
+                     wavebot_profile, wavebot_mesh, reference_face_geometry)
 from .problems import (FloatingBody, combine_floatingbodies, LinearPotentialFlowProblem,
                        DiffractionProblem, RadiationProblem, LinearPotentialFlowResult,
                        DiffractionResult, RadiationResult, make_result, problems_from_data,

@@ -47,6 +47,8 @@ _SIGS = {
     "mh_synchronize": (_int, [_vp]),
     "mh_profile_enable": (_int, [_vp, _int]),
     "mh_profile_report": (_int, [_vp, C.c_char_p, _i64]),
+    "mh_set_refinement": (_int, [_vp, _int, _int]),
+    "mh_fallback_count": (_int, [_vp, C.POINTER(_i64)]),
     "mh_gf_eval": (_int, [_vp, _int, _int, _int, _i64, _vp, _vp, _d, _vp]),
     "mh_wave_term_eval": (_int, [_vp, _int, _i64, _vp, _vp, _vp]),
     "mh_assemble": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
@@ -162,3 +164,17 @@ def profile_report(device=0) -> dict:
     buf = C.create_string_buffer(1 << 16)
     check(lib().mh_profile_report(context(device), buf, len(buf)), "profile_report")
     return json.loads(buf.value.decode())
+
+
+def set_refinement(max_sweeps=12, fp64_fallback=True, device=0):
+    """Mixed precision: refinement sweeps allowed and whether a chunk that
+    does not converge is solved again in FP64 (mh_set_refinement)."""
+    check(lib().mh_set_refinement(context(device), int(max_sweeps), int(bool(fp64_fallback))),
+          "set_refinement")
+
+
+def fallback_count(device=0) -> int:
+    """Frequency chunks solved again in FP64 since the context was created."""
+    out = _i64(0)
+    check(lib().mh_fallback_count(context(device), C.byref(out)), "fallback_count")
+    return int(out.value)

@@ -339,3 +339,121 @@ def rigid_dofs(mesh: Mesh, names, rotation_center=(0.0, 0.0, 0.0)):
         else:
             out.append(rotational_dofs(mesh, nm, rotation_center))
     return out
+
+
+# --------------------------------------------------------------------------
+# surfaces of revolution (reference src/meshes.jl:247-479)
+# --------------------------------------------------------------------------
+def axisymmetric_mesh(profile, n_theta=72, dprofile=None):
+    """Revolve a (r, z) profile about the z axis (src/meshes.jl:247-310).
+
+    profile: (np, 2) points from the waterline down; a point with r = 0 is a
+    single vertex and the faces next to it are triangles, stored as quads
+    with the last vertex repeated (src/meshes.jl:372-373).  Ring i of a
+    regular point holds n_theta vertices at theta = 2 pi q / n_theta.
+    dprofile: optional {name: (np, 2) d profile / d name}; the matching
+    d vertices / d name are returned alongside.
+    Returns (vertices (nv, 3), faces (nf, 4) 0-based, {name: dvertices}).
+    """
+    P = np.asarray(profile, dtype=np.float64)
+    if n_theta < 3:
+        raise ValueError("n_theta must be at least 3")
+    if P.shape[0] < 2:
+        raise ValueError("profile must contain at least two points")
+    if np.any(P[:, 0] < 0):
+        raise ValueError("profile radii must be nonnegative")
+    th = 2 * np.pi * np.arange(n_theta) / n_theta
+    ct, st = np.cos(th), np.sin(th)
+    V, starts, sizes = [], [], []
+    dnames = list(dprofile) if dprofile else []
+    dV = {k: [] for k in dnames}
+    for p, (r, z) in enumerate(P):
+        starts.append(len(V))
+        if r == 0.0:
+            sizes.append(1)
+            V.append([0.0, 0.0, z])
+            for k in dnames:
+                dr, dz = dprofile[k][p]
+                dV[k].append([0.0, 0.0, dz])
+        else:
+            sizes.append(n_theta)
+            for q in range(n_theta):
+                V.append([r * ct[q], r * st[q], z])
+                for k in dnames:
+                    dr, dz = dprofile[k][p]
+                    dV[k].append([dr * ct[q], dr * st[q], dz])
+    F = []
+    for layer in range(P.shape[0] - 1):
+        s1, s2 = starts[layer], starts[layer + 1]
+        n1, n2 = sizes[layer], sizes[layer + 1]
+        if n1 == 1 and n2 == 1:
+            raise ValueError("adjacent points cannot both have radius 0")
+        for q in range(n_theta):
+            q2 = (q + 1) % n_theta
+            if n1 == 1:
+                F.append([s1, s2 + q, s2 + q2, s2 + q2])
+            elif n2 == 1:
+                F.append([s1 + q, s2, s1 + q2, s1 + q2])
+            else:
+                F.append([s1 + q, s2 + q, s2 + q2, s1 + q2])
+    return (np.array(V), np.array(F, dtype=np.int64),
+            {k: np.array(v) for k, v in dV.items()})
+
+
+def wavebot_profile(cylinder_height=0.16, frustum_height=0.37, top_radius=0.88, bottom_radius=0.35,
+                    n_points=(5, 10, 5)):
+    """WaveBot hull profile (src/meshes.jl:312-349): a cylinder below the
+    waterline, a frustum, and the bottom disk to the axis.  Returns the (r, z)
+    points and their derivatives w.r.t. (cylinder_height, frustum_height,
+    top_radius, bottom_radius), keyed "d1", "d2", "r1", "r2"."""
+    n_bottom, n_frustum, n_cylinder = n_points
+    h1, h2, r1, r2 = map(float, (cylinder_height, frustum_height, top_radius, bottom_radius))
+    pts, d = [], {"d1": [], "d2": [], "r1": [], "r2": []}
+
+    def add(r, z, dr, dz):
+        pts.append([r, z])
+        for k in d:
+            d[k].append([dr.get(k, 0.0), dz.get(k, 0.0)])
+
+    for j in range(n_cylinder):
+        t = j / (n_cylinder - 1)
+        add(r1, -h1 * t, {"r1": 1.0}, {"d1": -t})
+    for j in range(1, n_frustum):
+        t = j / (n_frustum - 1)
+        add(r1 + (r2 - r1) * t, -h1 - h2 * t, {"r1": 1.0 - t, "r2": t}, {"d1": -1.0, "d2": -t})
+    zb = -h1 - h2
+    for j in range(1, n_bottom):
+        t = j / (n_bottom - 1)
+        add(r2 * (1 - t), zb, {"r2": 1.0 - t}, {"d1": -1.0, "d2": -1.0})
+    return np.array(pts), {k: np.array(v) for k, v in d.items()}
+
+
+def reference_face_geometry(vertices, faces):
+    """Face quantities the way the reference builds a Mesh from a mesh of
+    quadrangles and triangles (src/meshes.jl:352-430): area-weighted centre
+    of the triangles (a, b, c) and (a, c, d), area = their sum, normal =
+    normalised cross(b - a, c - a), radius = |a - centre| (first vertex)."""
+    V = np.asarray(vertices, dtype=np.float64)
+    Fc = np.asarray(faces)
+    a, b, c, d = (V[Fc[:, i]] for i in range(4))
+    a1, _ = _tri(a, b, c)
+    a2, _ = _tri(a, c, d)
+    area = a1 + a2
+    center = ((a + b + c) / 3.0 * a1[:, None] + (a + c + d) / 3.0 * a2[:, None]) / area[:, None]
+    nrm = np.cross(b - a, c - a)
+    normal = nrm / np.linalg.norm(nrm, axis=1)[:, None]
+    radius = np.linalg.norm(a - center, axis=1)
+    return center, normal, area, radius
+
+
+def wavebot_mesh(r1=0.88, r2=0.35, d1=0.16, d2=0.37, n_points=(5, 10, 5), n_theta=72) -> Body:
+    """src/meshes.jl:458-479: the WaveBot hull of top radius r1, bottom
+    radius r2, cylinder height d1 and frustum height d2, as a Mesh with the
+    reference's face conventions, plus d vertices / d (r1, r2, d1, d2).  Its
+    quads are planar trapezoids and stay planar along these parameters, so
+    face_geometry_jvp's normal (of the diagonals) has the same derivative as
+    the reference's cross(b - a, c - a)."""
+    prof, dprof = wavebot_profile(d1, d2, r1, r2, n_points)
+    V, F, dV = axisymmetric_mesh(prof, n_theta, dprof)
+    c, n, a, r = reference_face_geometry(V, F)
+    return Body(Mesh(V, F, c, n, a, r), {"r1": r1, "r2": r2, "d1": d1, "d2": d2}, dV)

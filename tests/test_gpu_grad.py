@@ -123,3 +123,28 @@ def test_rotational_dof_and_several_frequencies(mh, oracle, gpu):
         check(dA[w, 0], gA, 1e-6, ("dA", w), A)
         check(dB[w, 0], gB, 1e-6, ("dB", w), B)
         check(dF[w, 0].T, gF, 1e-6, ("dF", w), F)
+
+
+def test_wavebot_shape_gradients(mh, oracle, gpu):
+    # shape gradients of the paper's WEC hull (src/meshes.jl:458-479) w.r.t.
+    # its four design variables, heave and surge, against central
+    # differences of the oracle on regenerated meshes (h = 1e-6)
+    p0 = {"r1": 0.88, "r2": 0.35, "d1": 0.16, "d2": 0.37}
+    names = list(p0)
+    kw = {"n_points": (3, 6, 3), "n_theta": 24}
+    b = mh.wavebot_mesh(**p0, **kw)
+    dofs = [[0.0, 0.0, 1.0], [1.0, 0.0, 0.0]]
+    om = 2.5
+    T = mh.shape_tangents(b.mesh, [b.dvertices[k] for k in names])
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(b.mesh, [om], dofs, betas=[0.0], tangents=T, rho=1025.0)
+    h = 1e-6
+    for m, k in enumerate(names):
+        outs = []
+        for s in (+1, -1):
+            p = dict(p0)
+            p[k] += s * h
+            outs.append(oracle.freq_solve(GF_WU, mh.wavebot_mesh(**p, **kw).mesh, om, dofs, [0.0], rho=1025.0))
+        fA, fB, fF = [(outs[0][q] - outs[1][q]) / (2 * h) for q in range(3)]
+        check(dA[0, m], fA, 1e-6, ("dA", k), A[0])
+        check(dB[0, m], fB, 1e-6, ("dB", k), B[0])
+        check(dF[0, m, 0], fF[:, 0], 1e-6, ("dF", k), F[0, 0])

@@ -1,0 +1,53 @@
+"""Mixed precision: the FP64 fallback of a chunk whose refinement does not
+converge (forced from the host by allowing a single refinement sweep; every
+kernel runs normally), and non-finite input reported as an error rather
+than 'converged'."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_fp64_fallback_when_refinement_cannot_converge(mh, gpu, direct):
+    body = mh.hemisphere(1.0, ntheta=16, nphi=16)
+    m = body.mesh
+    om = [0.5, 1.2, 2.4]
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=3, omega_param=1)
+    ref = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="fp64")
+    try:
+        mh._lib.set_refinement(max_sweeps=1, fp64_fallback=True)
+        n0 = mh._lib.fallback_count()
+        got = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="mixed")
+        assert mh._lib.fallback_count() == n0 + 1
+        for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
+            assert relmax(g, r) < 1e-12, name  # solved in FP64 again
+        mh._lib.set_refinement(max_sweeps=1, fp64_fallback=False)
+        with pytest.raises(ArithmeticError, match="did not reach"):
+            mh.freq_sweep(m, om, dofs, [0.0], direct=direct, precision="mixed")
+    finally:
+        mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
+    # defaults restored: converges in mixed precision without a fallback
+    n0 = mh._lib.fallback_count()
+    got = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="mixed")
+    assert mh._lib.fallback_count() == n0
+    for g, r in zip(got, ref):
+        assert relmax(g, r) < 1e-8
+
+
+def test_nan_in_the_system_is_an_error_not_converged(mh, gpu):
+    m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
+    dof = np.tile([0.0, 0.0, 1.0], (m.nfaces, 1))
+    dof[3, 2] = np.nan  # a NaN right-hand-side entry
+    n0 = mh._lib.fallback_count()
+    with pytest.raises(ArithmeticError, match="non-finite"):
+        mh.freq_sweep(m, [1.0], [dof], precision="mixed")
+    assert mh._lib.fallback_count() == n0  # not retried in FP64
+    # the context keeps working
+    A, B, F = mh.freq_sweep(m, [1.0], [[0, 0, 1.0]], precision="mixed")
+    assert np.all(np.isfinite(A)) and np.all(np.isfinite(B))
