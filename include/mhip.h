@@ -314,6 +314,49 @@ int mh_froude_krylov_grad(mh_context_t ctx, const mh_mesh_t* mesh, int64_t nomeg
                           const double* dareas, const double* ddof_fields,
                           const double* domegas, double* FK, double* dFK);
 
+/* ---- uneven multi-body batches -----------------------------------------
+ * One body's sweep, the arguments of mh_freq_sweep_grad (nparam = 0: no
+ * gradients, tangent and gradient pointers unused).  omegas, betas and
+ * domegas are HOST arrays; the mesh and every other array are on the
+ * device. */
+typedef struct {
+    const mh_mesh_t* mesh;
+    int gf_mask;
+    int direct;
+    int64_t nomega;
+    const double* omegas;
+    int64_t ndof;
+    const double* dof_fields;
+    int64_t nbeta;
+    const double* betas;
+    int64_t nparam;
+    const double* dvertices;
+    const double* dcenters;
+    const double* dnormals;
+    const double* dareas;
+    const double* ddof_fields;
+    const double* domegas;
+    double *A, *B, *F, *dA, *dB, *dF;
+} mh_sweep_item_t;
+
+/* The reference's loop over problems (solve_all_problems, src/solve.jl:17-21;
+ * problems_from_data, src/problems_and_results.jl:70-100) for many bodies of
+ * uneven panel counts: a work list ordered longest first, taken by nworkers
+ * worker contexts (each with its own stream and workspace, created on first
+ * use and kept by ctx) so that bodies overlap on the GPU.  nworkers = 0 uses
+ * the autotuned configuration (or 2 before tuning).  Outputs per item as
+ * mh_freq_sweep_grad.  Synchronises; on failure the message names the item.
+ * Limits per item as mh_freq_sweep (gradients: ndof + nbeta <= 4). */
+int mh_multi_sweep(mh_context_t ctx, int64_t nitems, const mh_sweep_item_t* items,
+                   int precision, double g, double rho, int nworkers);
+/* Times the given items (a sample of the workload; their outputs are
+ * overwritten) under each candidate worker count (1, 2, 3, 4, 6) and LU leaf
+ * width (the default for the panel height, or 8 columns), keeps the fastest
+ * for the context and returns it (each output may be NULL). */
+int mh_multi_autotune(mh_context_t ctx, int64_t nitems, const mh_sweep_item_t* items,
+                      int precision, double g, double rho, int* best_workers,
+                      int* best_leaf, double* best_ms);
+
 /* ---- host-array entry points (one call, synchronous) -------------------
  * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
  * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.

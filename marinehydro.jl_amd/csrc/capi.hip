@@ -102,6 +102,22 @@ int mh_context_create(int device, void* stream, mh_context_t* out) {
         return MH_ERR_ARGUMENT;
     }
     *out = nullptr;
+    Context* c = nullptr;
+    TRY(context_new(device, reinterpret_cast<hipStream_t>(stream), false, &c));
+    *out = reinterpret_cast<mh_context_t>(c);
+    return MH_OK;
+}
+
+int mh_context_destroy(mh_context_t ctx) {
+    if (!ctx) return MH_OK;
+    context_free(CTX(ctx));
+    return MH_OK;
+}
+}  // extern "C"
+
+namespace mh {
+int context_new(int device, hipStream_t stream, bool own_stream, Context** out) {
+    *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         set_error("no HIP device");
@@ -120,19 +136,29 @@ int mh_context_create(int device, void* stream, mh_context_t* out) {
     }
     Context* c = new Context();
     c->device = device;
-    c->stream = reinterpret_cast<hipStream_t>(stream);
+    c->stream = stream;
+    if (own_stream) {
+        // default priority (no priority argument anywhere in the library)
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            set_error("context: hipStreamCreateWithFlags failed");
+            return MH_ERR_HIP;
+        }
+        c->owns_stream = true;
+    }
     if (hipMalloc(&c->d_flag, 256) != hipSuccess) {
+        if (c->owns_stream) (void)hipStreamDestroy(c->stream);
         delete c;
         set_error("mh_context_create: hipMalloc failed");
         return MH_ERR_HIP;
     }
-    *out = reinterpret_cast<mh_context_t>(c);
+    *out = c;
     return MH_OK;
 }
 
-int mh_context_destroy(mh_context_t ctx) {
-    if (!ctx) return MH_OK;
-    Context* c = CTX(ctx);
+void context_free(Context* c) {
+    for (Context* w : c->workers) context_free(w);
+    c->workers.clear();
     (void)hipSetDevice(c->device);
     // every stream of the context drains before any buffer it may still be
     // reading or writing is released
@@ -160,9 +186,12 @@ int mh_context_destroy(mh_context_t ctx) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
-    return MH_OK;
 }
+}  // namespace mh
+
+extern "C" {
 
 int mh_synchronize(mh_context_t ctx) {
     TRY(check_ctx(ctx));
@@ -611,15 +640,10 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
     return MH_ERR_NOT_CONVERGED;
 }
 
-// Optional gradient request of sweep_impl (see mh_freq_sweep_grad).
-struct GradArgs {
-    int64_t nparam;
-    const double *dV, *dC, *dN, *dA, *ddofs;  // device tangents (may be NULL)
-    const double* domega;                      // host, nparam x nomega (may be NULL)
-    double *dAout, *dBout, *dFout;             // device outputs
-};
+}  // extern "C"
 
-static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
+namespace mh {
+int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,
                       int64_t nomega,
                       const double* omegas, double g, double rho, int64_t ndof,
                       const double* dof_fields, int64_t nbeta, const double* betas, double* A,
@@ -927,11 +951,9 @@ static int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct
     return MH_OK;
 }
 
-static int sweep_checks(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int precision,
-                        int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
-                        const double* dof_fields, int64_t nbeta, const double* betas, double* A,
-                        double* B, double* F) {
-    TRY(check_ctx(ctx));
+int sweep_args_ok(const mh_mesh_t* mesh, int gf_mask, int precision, int64_t nomega, const double* omegas,
+                  double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+                  const double* betas, double* A, double* B, double* F) {
     TRY(check_mesh(mesh));
     MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "freq_sweep: bad Green-function mask");
     MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
@@ -950,6 +972,19 @@ static int sweep_checks(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, in
     for (int64_t w = 0; w < nomega; ++w)
         MH_REQUIRE(omegas[w] > 0.0, "freq_sweep: frequencies must be positive");
     return MH_OK;
+}
+
+}  // namespace mh
+
+extern "C" {
+
+static int sweep_checks(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int precision,
+                        int64_t nomega, const double* omegas, double g, double rho, int64_t ndof,
+                        const double* dof_fields, int64_t nbeta, const double* betas, double* A,
+                        double* B, double* F) {
+    TRY(check_ctx(ctx));
+    return sweep_args_ok(mesh, gf_mask, precision, nomega, omegas, g, rho, ndof, dof_fields, nbeta, betas, A, B,
+                         F);
 }
 
 int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct, int precision,

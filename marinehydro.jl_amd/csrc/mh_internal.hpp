@@ -64,7 +64,37 @@ struct Context {
     int ir_max_sweeps = 12;
     bool fp64_fallback = true;
     int64_t fallbacks = 0;
+    // LU leaf width cap (0: the default width for the panel height; 4 or 8)
+    int leaf_cap = 0;
+    // multi-body scheduler: worker contexts (each with a stream of its own)
+    // and the configuration the autotuner chose (0: not tuned)
+    bool owns_stream = false;
+    std::vector<Context*> workers;
+    int tuned_workers = 0, tuned_leaf = 0;
 };
+
+// Creates a context on `device` (current afterwards) using `stream`, or a
+// new non-blocking stream of default priority owned by the context when
+// own_stream is set; context_free releases everything, workers included.
+int context_new(int device, hipStream_t stream, bool own_stream, Context** out);
+void context_free(Context* c);
+
+// Optional gradient request of a sweep (mh_freq_sweep_grad).
+struct GradArgs {
+    int64_t nparam;
+    const double *dV, *dC, *dN, *dA, *ddofs;  // device tangents (may be NULL)
+    const double* domega;                      // host, nparam x nomega (may be NULL)
+    double *dAout, *dBout, *dFout;             // device outputs
+};
+// One body's frequency sweep (mh_freq_sweep / mh_freq_sweep_grad after
+// their argument checks), synchronous.
+int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int precision, int64_t nomega,
+               const double* omegas, double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+               const double* betas, double* A, double* B, double* F, const GradArgs* ga);
+// Argument checks of a sweep, without the context.
+int sweep_args_ok(const mh_mesh_t* mesh, int gf_mask, int precision, int64_t nomega, const double* omegas,
+                  double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
+                  const double* betas, double* A, double* B, double* F);
 
 // Brackets one kernel launch with events on the context's stream when the
 // kernel timer is on; no cost otherwise.

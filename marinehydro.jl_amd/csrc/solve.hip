@@ -1094,7 +1094,10 @@ static int lu_leaf(const LuPlan<T>& P, int64_t k0, int w, int64_t x_lo = 0, int6
 // (each leaf applies its interchanges to the rest of the block column)
 template <class T>
 static int lu_panel(const LuPlan<T>& P, int64_t k0, int w) {
-    const int L = leaf_width<T>(P.n - P.pc_lo);
+    int L = leaf_width<T>(P.n - P.pc_lo);
+    // a narrower leaf (the autotuner's choice): the same kernels, called
+    // with fewer columns each
+    if (P.ctx->leaf_cap > 0 && P.ctx->leaf_cap < L) L = P.ctx->leaf_cap;
     if (w <= L) return lu_leaf(P, k0, w);
     int h = ((w / 2) + L - 1) / L * L;
     if (h >= w) h = w - L;

@@ -16,7 +16,7 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   freq_sweep_grad, froude_krylov, froude_krylov_grad,
                   assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
-                     mesh_from_vertices, hemisphere, sphere, vertical_cylinder, box, translated,
+                     mesh_from_vertices, hemisphere, floating_sphere, sphere, vertical_cylinder, box, translated,
                      translational_dofs, rotational_dofs, rigid_dofs, axisymmetric_mesh,
                      wavebot_profile, wavebot_mesh, reference_face_geometry)
 from .problems import (FloatingBody, combine_floatingbodies, LinearPotentialFlowProblem,
@@ -25,5 +25,6 @@ from .problems import (FloatingBody, combine_floatingbodies, LinearPotentialFlow
                        solve_problem, solve_all_problems, assemble_hydrodynamic_coefficients,
                        compute_hydrodynamic_coefficients, create_DimStack, DimArray,
                        compute_and_label_hydrodynamic_coefficients)
+from .batches import SweepItem, multi_sweep, multi_autotune, random_body, random_workload
 
 __all__ = [n for n in dir() if not n.startswith("_")]

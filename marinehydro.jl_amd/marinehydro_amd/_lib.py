@@ -68,6 +68,9 @@ _SIGS = {
     "mh_froude_krylov": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _vp]),
     "mh_froude_krylov_grad": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _i64, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mh_multi_sweep": (_int, [_vp, _i64, _vp, _int, _d, _d, _int]),
+    "mh_multi_autotune": (_int, [_vp, _i64, _vp, _int, _d, _d, C.POINTER(_int), C.POINTER(_int),
+                                 C.POINTER(_d)]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   _d, _d, _d, _vp]),
     "mh_diffraction_force_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d,
@@ -79,6 +82,15 @@ class MeshT(C.Structure):
     """mh_mesh_t"""
     _fields_ = [("nfaces", _i64), ("nvertices", _i64), ("vertices", _vp), ("faces", _vp),
                 ("centers", _vp), ("normals", _vp), ("areas", _vp), ("radii", _vp)]
+
+
+class SweepItemT(C.Structure):
+    """mh_sweep_item_t"""
+    _fields_ = [("mesh", _vp), ("gf_mask", _int), ("direct", _int), ("nomega", _i64), ("omegas", _vp),
+                ("ndof", _i64), ("dof_fields", _vp), ("nbeta", _i64), ("betas", _vp), ("nparam", _i64),
+                ("dvertices", _vp), ("dcenters", _vp), ("dnormals", _vp), ("dareas", _vp),
+                ("ddof_fields", _vp), ("domegas", _vp), ("A", _vp), ("B", _vp), ("F", _vp), ("dA", _vp),
+                ("dB", _vp), ("dF", _vp)]
 
 
 _lib = None

@@ -206,6 +206,52 @@ def hemisphere(radius=1.0, center=(0.0, 0.0, 0.0), ntheta=10, nphi=10) -> Body:
     return _latlong_sphere(radius, center, ntheta, nphi, ntheta // 2)
 
 
+def floating_sphere(radius=1.0, draft=1.0, center=(0.0, 0.0), ntheta=10, nphi=10) -> Body:
+    """Immersed cap of a sphere of the given radius floating at the given
+    draft (depth of its lowest point), 0 < draft < 2 radius: centre at
+    z = radius - draft, vertex rings from the waterline (theta_0 with
+    cos theta_0 = draft / radius - 1) to the bottom pole, theta_a = theta_0 +
+    (pi - theta_0) a / ntheta, a = 0 .. ntheta, and phi_b = 2 pi b / nphi;
+    ntheta * nphi quads (the bottom ring degenerate).  d vertices / d radius
+    and / d draft are exact."""
+    R, d = float(radius), float(draft)
+    if not 0.0 < d < 2.0 * R:
+        raise ValueError("draft must lie in (0, 2 radius)")
+    cx, cy = float(center[0]), float(center[1])
+    c0 = d / R - 1.0
+    th0 = np.arccos(c0)
+    s0 = np.sqrt(1.0 - c0 * c0)
+    dth0_dR = -(-d / R ** 2) / s0  # d arccos(c0) = -dc0 / sqrt(1 - c0^2)
+    dth0_dd = -(1.0 / R) / s0
+    a = np.arange(ntheta + 1)
+    w = 1.0 - a / ntheta
+    th = th0 * w + np.pi * (a / ntheta)
+    phi = 2 * np.pi * np.arange(nphi) / nphi
+    T, P = np.meshgrid(th, phi, indexing="ij")
+    W, _ = np.meshgrid(w, phi, indexing="ij")
+    st, ct, cp, sp = np.sin(T), np.cos(T), np.cos(P), np.sin(P)
+    zc = R - d
+    V = np.stack([cx + R * st * cp, cy + R * st * sp, zc + R * ct], axis=-1)
+    V[0, :, 2] = 0.0  # the waterline ring exactly at z = 0
+    dT_dR, dT_dd = dth0_dR * W, dth0_dd * W
+    dVR = np.stack([st * cp + R * ct * cp * dT_dR, st * sp + R * ct * sp * dT_dR,
+                    1.0 + ct - R * st * dT_dR], axis=-1)
+    dVd = np.stack([R * ct * cp * dT_dd, R * ct * sp * dT_dd, -1.0 - R * st * dT_dd], axis=-1)
+    dVR[0, :, 2] = 0.0
+    dVd[0, :, 2] = 0.0
+    V, dVR, dVd = V.reshape(-1, 3), dVR.reshape(-1, 3), dVd.reshape(-1, 3)
+    F = []
+    for i in range(ntheta):
+        for b in range(nphi):
+            b2 = (b + 1) % nphi
+            F.append([i * nphi + b, (i + 1) * nphi + b, (i + 1) * nphi + b2, i * nphi + b2])
+    F = np.array(F, dtype=np.int64)
+    ctr = np.array([cx, cy, zc])
+    F = _orient_outward(V, F, lambda c: np.tile(ctr, (len(c), 1)))
+    mesh = mesh_from_vertices(V, F)
+    return Body(mesh, {"radius": R, "draft": d}, {"radius": dVR, "draft": dVd})
+
+
 def sphere(radius=1.0, center=(0.0, 0.0, 0.0), ntheta=10, nphi=10) -> Body:
     """Full latitude-longitude sphere (both poles degenerate rings); used for
     closed-surface identities, not a floating body."""
