@@ -50,7 +50,14 @@ enum mh_gf {
     MH_GF_RANKINE_REFLECTED = 2,     /* RankineReflected()                  */
     MH_GF_RANKINE_REFLECTED_NEG = 4, /* RankineReflectedNegative()          */
     MH_GF_WU = 8,                    /* GFWu(), wu.jl                       */
-    MH_GF_DELHOMMEAU = 16            /* ExactGuevelDelhommeau()             */
+    MH_GF_DELHOMMEAU = 16,           /* ExactGuevelDelhommeau()             */
+    /* The same term from a table of the exact one (bicubic Hermite over
+     * nested (r, z) grids, log singularity subtracted, exact beyond
+     * r, |z| > 32): no reference counterpart; within 1e-9 of the exact term
+     * and about as fast as GFWu.  Derivative entry points differentiate the
+     * exact term in its place.  The mixed-precision sweep substitutes it for
+     * MH_GF_DELHOMMEAU in the assembly. */
+    MH_GF_DELHOMMEAU_TABLE = 32
 };
 /* The tuple used by calculate_radiation_forces / DiffractionForce
  * (src/matrix_assembly.jl:100-102, src/waves.jl:41-45). */

@@ -16,6 +16,8 @@
 // wavenumber of the group (blockIdx.z).  With a wave term the launch is
 // split: the Rankine parts once (k_rankine_parts), then the wave term once
 // per unordered pair (k_assemble_sym), since it is symmetric in (i, j).
+#include <mutex>
+
 #include "mh_internal.hpp"
 
 namespace mh {
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(256) void k_assemble(
     int64_t n, const double* __restrict__ C, const double* __restrict__ N,
     const PanelRec* __restrict__ panels, int gf, int64_t nk, const double* __restrict__ ks,
     double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K, int64_t ld,
-    int64_t stride, float2* __restrict__ DK32) {
+    int64_t stride, float2* __restrict__ DK32, DelhTab tab) {
     // DK32 (optional): complex64 copy of D (FORM 1) or K (FORM 2), n x n per
     // wavenumber, for the FP32 factorisation of the mixed-precision path
     const int lane = threadIdx.x & 63;
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_assemble(
         for (int64_t f = 0; f < nfg; ++f) {
             const int64_t m = f0 + f;
             cplx Sv = {0.0, 0.0}, gxi[3] = {{0, 0}, {0, 0}, {0, 0}}, gx[3] = {{0, 0}, {0, 0}, {0, 0}};
-            if (WAVE) wave_term<WAVE>(ks[m], x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], p.area, Sv, gxi, gx);
+            if (WAVE) wave_term<WAVE>(ks[m], x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], p.area, Sv, gxi, gx, tab);
             if (!row_ok) continue;
             const int64_t off = m * stride + i + j * ld;
             S[off] = make_double2(-kInv4Pi * (IR + Sv.re), -kInv4Pi * Sv.im);
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void k_assemble_sym(
     int64_t n, const double* __restrict__ C, const double* __restrict__ N, const PanelRec* __restrict__ panels,
     int64_t nk, const double* __restrict__ ks, const double* __restrict__ RS, const double* __restrict__ RD,
     const double* __restrict__ RK, double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K,
-    int64_t ld, int64_t stride, float2* __restrict__ DK32) {
+    int64_t ld, int64_t stride, float2* __restrict__ DK32, DelhTab tab) {
     // tile pair (I, J), I <= J: blockIdx.x = J (J + 1) / 2 + I
     const int pidx = (int)blockIdx.x;
     int J = (int)((::sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void k_assemble_sym(
                     const double k = ks[m];
                     cplx G, Gr, Gz;
                     double d1, d2;
-                    wave_raw<WAVE>(k, dx, dy, zs, hd, G, Gr, Gz, d1, d2);
+                    wave_raw<WAVE>(k, dx, dy, zs, hd, G, Gr, Gz, d1, d2, tab);
                     // gradient w.r.t. the source point of (i, j), without the area:
                     // (d1 Gr, d2 Gr, k Gz); of (j, i): (-d1 Gr, -d2 Gr, k Gz)
                     const double kaj = k * aj, kai = k * ai;
@@ -310,19 +312,19 @@ template <int WAVE>
 static void dispatch_form(int form, dim3 grid, hipStream_t st, int64_t n, const double* C,
                           const double* N, const PanelRec* P, int gf, int64_t nk,
                           const double* ks, double2* S, double2* D, double2* K, int64_t ld,
-                          int64_t stride, float2* DK32) {
+                          int64_t stride, float2* DK32, DelhTab tab = DelhTab{}) {
     switch (form) {
         case 1:
             hipLaunchKernelGGL((k_assemble<WAVE, 1>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride, DK32);
+                               S, D, K, ld, stride, DK32, tab);
             break;
         case 2:
             hipLaunchKernelGGL((k_assemble<WAVE, 2>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride, DK32);
+                               S, D, K, ld, stride, DK32, tab);
             break;
         default:
             hipLaunchKernelGGL((k_assemble<WAVE, 3>), grid, dim3(256), 0, st, n, C, N, P, gf, nk, ks,
-                               S, D, K, ld, stride, DK32);
+                               S, D, K, ld, stride, DK32, tab);
             break;
     }
 }
@@ -332,6 +334,8 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
                     int64_t stride, float2* DK32) {
     const int64_t n = m->nfaces;
     if (n == 0 || nk == 0) return MH_OK;
+    DelhTab tab{};
+    if (gf & GF_DELH_TABLE) TRY_RC(get_delh_table(ctx, &tab));
     const int64_t tiles = (n + 63) / 64;
     const int64_t groups = (nk + kFG - 1) / kFG;
     if (tiles > 0x7fffffff || groups > 65535) {
@@ -340,7 +344,7 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
     }
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)groups);
     const double nmat = 1.0 + ((form & 1) ? 1.0 : 0.0) + ((form & 2) ? 1.0 : 0.0);
-    if (gf & (GF_WU | GF_DELHOMMEAU)) {
+    if (gf & kGfWaveMask) {
         // Rankine parts once, then the symmetric wave-term pass
         const bool rank = (gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) != 0;
         const int nparts = 1 + ((form & 1) ? 1 : 0) + ((form & 2) ? 1 : 0);
@@ -381,11 +385,15 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
         dim3 gs((unsigned)pairs, (unsigned)sgroups);
         ProfScope ps(ctx, "assemble", 0.0, 16.0 * nmat * (double)n * (double)n * (double)nk);
 #define MH_AS(W, F) hipLaunchKernelGGL((k_assemble_sym<W, F>), gs, dim3(256), 0, ctx->stream, n, m->centers, \
-                                       m->normals, panels, nk, d_k, RS, RD, RK, S, D, K, ld, stride, DK32)
+                                       m->normals, panels, nk, d_k, RS, RD, RK, S, D, K, ld, stride, DK32, tab)
         if (gf & GF_WU) {
             if (form == 1) MH_AS(1, 1);
             else if (form == 2) MH_AS(1, 2);
             else MH_AS(1, 3);
+        } else if (gf & GF_DELH_TABLE) {
+            if (form == 1) MH_AS(3, 1);
+            else if (form == 2) MH_AS(3, 2);
+            else MH_AS(3, 3);
         } else {
             if (form == 1) MH_AS(2, 1);
             else if (form == 2) MH_AS(2, 2);
@@ -423,6 +431,7 @@ int launch_assemble_onepass(Context* ctx, const mh_mesh_t* m, const PanelRec* pa
         return MH_ERR_ARGUMENT;
     }
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)groups);
+    MH_REQUIRE(!(gf & GF_DELH_TABLE), "assemble: the one-pass kernel takes the exact Delhommeau term only");
     if (gf & GF_WU)
         dispatch_form<1>(form, grid, ctx->stream, n, m->centers, m->normals, panels, gf, nk, d_k, S, D, K, ld,
                          stride, nullptr);
@@ -441,7 +450,7 @@ int launch_assemble_onepass(Context* ctx, const mh_mesh_t* m, const PanelRec* pa
 // ---------------------------------------------------------------------------
 __global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
                           const double* __restrict__ pts, const mh_panel_t* __restrict__ pan,
-                          double k, double2* __restrict__ out) {
+                          double k, double2* __restrict__ out, DelhTab tab) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np) return;
     const double x0 = pts[t], x1 = pts[t + np], x2 = pts[t + 2 * np];
@@ -497,13 +506,15 @@ __global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
             }
         }
     }
-    if (gf & (GF_WU | GF_DELHOMMEAU)) {
+    if (gf & kGfWaveMask) {
         cplx Sv, gxi[3], gx[3];
         const double area = panel_int ? p.area : 1.0;
         if (gf & GF_WU)
             wave_term<1>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
-        else
+        else if (gf & GF_DELHOMMEAU)
             wave_term<2>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx);
+        else
+            wave_term<3>(k, x0 - p.c[0], x1 - p.c[1], x2 + p.c[2], area, Sv, gxi, gx, tab);
         val_re += Sv.re;
         val_im += Sv.im;
         for (int c = 0; c < 3; ++c) {
@@ -521,21 +532,30 @@ __global__ void k_gf_eval(int gf, int op, int wrt_first, int64_t np,
 int launch_gf_eval(Context* ctx, int gf, int op, int wrt_first, int64_t np, const double* pts,
                    const mh_panel_t* pan, double k, double2* out) {
     if (np == 0) return MH_OK;
+    DelhTab tab{};
+    if (gf & GF_DELH_TABLE) TRY_RC(get_delh_table(ctx, &tab));
     hipLaunchKernelGGL(k_gf_eval, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, ctx->stream, gf,
-                       op, wrt_first, np, pts, pan, k, out);
+                       op, wrt_first, np, pts, pan, k, out, tab);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
 }
 
 __global__ void k_wave_term(int term, int64_t n, const double* __restrict__ r,
-                            const double* __restrict__ z, double2* __restrict__ out) {
+                            const double* __restrict__ z, double2* __restrict__ out, DelhTab tab) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     cplx G, dr, dz;
-    if (term == GF_WU)
+    if (term == GF_WU) {
         wu_dimless<true>(r[t], z[t], G, dr, dz);
-    else
+    } else if (term == GF_DELHOMMEAU) {
         delhommeau_dimless(r[t], z[t], G, dr, dz);
+    } else {
+        double g[2], gr[2], gz[2];
+        delh_table_eval(tab, r[t], z[t], g, gr, gz);
+        G = {g[0], g[1]};
+        dr = {gr[0], gr[1]};
+        dz = {gz[0], gz[1]};
+    }
     out[t] = make_double2(G.re, G.im);
     out[t + n] = make_double2(dr.re, dr.im);
     out[t + 2 * n] = make_double2(dz.re, dz.im);
@@ -543,14 +563,64 @@ __global__ void k_wave_term(int term, int64_t n, const double* __restrict__ r,
 
 int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const double* z,
                      double2* out) {
-    if (term != GF_WU && term != GF_DELHOMMEAU) {
-        set_error("wave_term: term must be MH_GF_WU or MH_GF_DELHOMMEAU");
+    if (term != GF_WU && term != GF_DELHOMMEAU && term != GF_DELH_TABLE) {
+        set_error("wave_term: term must be MH_GF_WU, MH_GF_DELHOMMEAU or MH_GF_DELHOMMEAU_TABLE");
         return MH_ERR_ARGUMENT;
     }
     if (n == 0) return MH_OK;
+    DelhTab tab{};
+    if (term == GF_DELH_TABLE) TRY_RC(get_delh_table(ctx, &tab));
     hipLaunchKernelGGL(k_wave_term, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream,
-                       term, n, r, z, out);
+                       term, n, r, z, out, tab);
     MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Table of the Delhommeau term (delh_table.hpp): built once per device from
+// the exact FP64 term on first use and kept until the process ends.
+// ---------------------------------------------------------------------------
+__global__ void k_build_tab(double* __restrict__ lv0, double* __restrict__ lvs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n0 = (int64_t)kTabN0 * kTabN0;
+    if (t >= tab_records()) return;
+    double r, z, *rec;
+    if (t < n0) {
+        const int64_t i = t % kTabN0, j = t / kTabN0;
+        r = (double)i * kTabH0;
+        z = -(double)j * kTabH0;
+        rec = lv0 + t * kTabRec;
+    } else {
+        const int64_t u = t - n0, per = (int64_t)kTabNn * kTabNn;
+        const int l = (int)(u / per) + 1;
+        const int64_t v = u % per, i = v % kTabNn, j = v / kTabNn;
+        const double h = ldexp(kTabR1, 1 - l) / (double)(kTabNn - 1);
+        r = (double)i * h;
+        z = -(double)j * h;
+        rec = lvs + u * kTabRec;
+    }
+    tab_node(r, z, rec);
+}
+
+static std::mutex g_tab_mutex;
+static double* g_tab_mem[64] = {};
+
+int get_delh_table(Context* ctx, DelhTab* out) {
+    MH_REQUIRE(ctx->device >= 0 && ctx->device < 64, "Delhommeau table: device index beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_tab_mutex);
+    double*& mem = g_tab_mem[ctx->device];
+    if (!mem) {
+        const int64_t nrec = tab_records();
+        double* p = nullptr;
+        MH_HIP_CHECK(hipMalloc(&p, sizeof(double) * kTabRec * (size_t)nrec));
+        hipLaunchKernelGGL(k_build_tab, dim3((unsigned)((nrec + 127) / 128)), dim3(128), 0, ctx->stream, p,
+                           p + (size_t)kTabN0 * kTabN0 * kTabRec);
+        MH_HIP_CHECK(hipGetLastError());
+        MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // other streams may read it at once
+        mem = p;
+    }
+    out->lv0 = mem;
+    out->lvs = mem + (size_t)kTabN0 * kTabN0 * kTabRec;
     return MH_OK;
 }
 

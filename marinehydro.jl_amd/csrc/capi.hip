@@ -62,6 +62,20 @@ static int check_mesh(const mh_mesh_t* m) {
     return MH_OK;
 }
 
+// A sum of Green-function terms with at most one wave term.
+static int check_gf(int gf, const char* who) {
+    if (gf <= 0 || gf >= 64) {
+        set_error(std::string(who) + ": bad Green-function mask");
+        return MH_ERR_ARGUMENT;
+    }
+    const int w = gf & (MH_GF_WU | MH_GF_DELHOMMEAU | MH_GF_DELHOMMEAU_TABLE);
+    if (w & (w - 1)) {
+        set_error(std::string(who) + ": choose one wave term");
+        return MH_ERR_ARGUMENT;
+    }
+    return MH_OK;
+}
+
 int ensure_aux(Context* c) {
     if (c->aux) return MH_OK;
     MH_HIP_CHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
@@ -257,9 +271,7 @@ int mh_gf_eval(mh_context_t ctx, int gf_mask, int op, int wrt_first, int64_t npa
     TRY(check_ctx(ctx));
     MH_REQUIRE(npairs >= 0, "gf_eval: negative npairs");
     MH_REQUIRE(op >= 0 && op <= 3, "gf_eval: unknown op");
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "gf_eval: bad Green-function mask");
-    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
-               "gf_eval: choose one wave term");
+    TRY(check_gf(gf_mask, "gf_eval"));
     if (npairs == 0) return MH_OK;
     MH_REQUIRE(points && panels && out, "gf_eval: null array");
     return launch_gf_eval(CTX(ctx), gf_mask, op, wrt_first, npairs, points, panels, k,
@@ -296,9 +308,7 @@ int mh_assemble(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int formul
                 int64_t stride) {
     TRY(check_ctx(ctx));
     TRY(check_mesh(mesh));
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble: bad Green-function mask");
-    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
-               "assemble: choose one wave term");
+    TRY(check_gf(gf_mask, "assemble"));
     MH_REQUIRE(formulation >= 1 && formulation <= 3, "assemble: formulation must be 1, 2 or 3");
     MH_REQUIRE(nk >= 0, "assemble: negative nk");
     const int64_t n = mesh->nfaces;
@@ -699,7 +709,11 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     size_t o_cw = 0, o_lam = 0, o_mu = 0, o_tan = 0, o_dk = 0, o_dom = 0, o_part = 0, o_T1 = 0, o_T2 = 0;
     size_t o_dR = 0, o_Y = 0;
     const int rank_mask = gf_mask & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG);
-    const int wave_mask = gf_mask & (GF_WU | GF_DELHOMMEAU);
+    const int wave_mask = gf_mask & kGfWaveMask;
+    // the mixed-precision assembly takes the Delhommeau term from its table
+    // (within 1e-9 of the exact term); gradients differentiate the exact term
+    const int asm_mask = (mixed && (gf_mask & GF_DELHOMMEAU)) ? ((gf_mask & ~GF_DELHOMMEAU) | GF_DELH_TABLE)
+                                                              : gf_mask;
     const int64_t tiles = (n + 63) / 64;
     const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
     if (ga) {
@@ -743,7 +757,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     double2* aug = dD + n * ld;  // first right-hand-side column of item 0
     for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
         const int64_t nw = (nomega - w0) < chunk ? (nomega - w0) : chunk;
-        TRY(launch_assemble(c, mesh, P, gf_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
+        TRY(launch_assemble(c, mesh, P, asm_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
                             direct ? nullptr : dD, ld, stride,
                             mixed ? reinterpret_cast<float2*>(ws + o_D32) : nullptr));
         TRY(launch_bcs(c, mesh, nw, d_om + w0, g, ndof, dof_fields, nbeta, d_be, rhs, ldb, bstride));
@@ -955,9 +969,7 @@ int sweep_args_ok(const mh_mesh_t* mesh, int gf_mask, int precision, int64_t nom
                   double g, double rho, int64_t ndof, const double* dof_fields, int64_t nbeta,
                   const double* betas, double* A, double* B, double* F) {
     TRY(check_mesh(mesh));
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "freq_sweep: bad Green-function mask");
-    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)),
-               "freq_sweep: choose one wave term");
+    TRY(check_gf(gf_mask, "freq_sweep"));
     MH_REQUIRE(nomega >= 0 && ndof >= 0 && nbeta >= 0, "freq_sweep: negative size");
     MH_REQUIRE(g > 0.0 && rho > 0.0, "freq_sweep: g and rho must be positive");
     MH_REQUIRE(precision == MH_FP64 || precision == MH_MIXED, "freq_sweep: unknown precision");
@@ -1024,8 +1036,7 @@ int mh_assemble_vjp(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int fo
                     double* dwavenumbers) {
     TRY(check_ctx(ctx));
     TRY(check_mesh(mesh));
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble_vjp: bad Green-function mask");
-    MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)), "assemble_vjp: choose one wave term");
+    TRY(check_gf(gf_mask, "assemble_vjp"));
     MH_REQUIRE(formulation == MH_DIRECT || formulation == MH_INDIRECT,
                "assemble_vjp: formulation must be MH_DIRECT or MH_INDIRECT");
     MH_REQUIRE(nk >= 0, "assemble_vjp: negative nk");

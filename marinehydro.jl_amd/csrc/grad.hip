@@ -789,7 +789,8 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)nw);
     MH_REQUIRE(nr <= 4, "gradients: at most 4 dofs + headings per pass");
     ProfScope ps(ctx, "tangent_pairs", 0.0, 0.0);
-    const int wave = (gf & GF_WU) ? 1 : ((gf & GF_DELHOMMEAU) ? 2 : 0);
+    // the tabulated Delhommeau term is differentiated as the exact one
+    const int wave = (gf & GF_WU) ? 1 : ((gf & (GF_DELHOMMEAU | GF_DELH_TABLE)) ? 2 : 0);
     const bool nr2 = nr <= 2;
     if (wave && !(gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG))) {
         // wave term only: one evaluation per unordered pair

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "delh_table.hpp"
 #include "delhommeau.hpp"
 #include "mh_dual.hpp"
 
@@ -27,7 +28,9 @@ enum : int {
     GF_RANKINE_REFLECTED_NEG = 4,
     GF_WU = 8,
     GF_DELHOMMEAU = 16,
+    GF_DELH_TABLE = 32,  // the Delhommeau term from the table (delh_table.hpp)
 };
+constexpr int kGfWaveMask = GF_WU | GF_DELHOMMEAU | GF_DELH_TABLE;
 
 // Packed source-panel record (one per face, 192 B), read wave-uniformly by
 // the pair kernels so it lands in scalar registers.
@@ -381,13 +384,21 @@ __device__ __forceinline__ void rankine_terms(int gf, T x0, T x1, T x2, const PT
 // Wave term of one pair at wavenumber k without the source area: G, dG/dr,
 // dG/dz in reduced coordinates and d1, d2 = dr/dxi_1, dr/dxi_2 (zero at
 // r <= 1e-6), for horizontal offset (dx, dy) of horizontal length hd.
+// WAVE: 1 Wu, 2 exact Delhommeau, 3 tabulated Delhommeau (needs tab).
 template <int WAVE>
 __device__ __forceinline__ void wave_raw(double k, double dx, double dy, double zsum, double hd, cplx& G,
-                                         cplx& Gr, cplx& Gz, double& d1, double& d2) {
+                                         cplx& Gr, cplx& Gz, double& d1, double& d2, const DelhTab& tab = DelhTab{}) {
     const double r = k * hd;
     const double z = k * zsum;
     if (WAVE == 1) wu_dimless<true>(r, z, G, Gr, Gz);
     if (WAVE == 2) delhommeau_dimless(r, z, G, Gr, Gz);
+    if constexpr (WAVE == 3) {
+        double g[2], gr[2], gz[2];
+        delh_table_eval(tab, r, z, g, gr, gz);
+        G = {g[0], g[1]};
+        Gr = {gr[0], gr[1]};
+        Gz = {gz[0], gz[1]};
+    }
     d1 = 0.0;
     d2 = 0.0;
     if (::fabs(r) > 1e-6) {
@@ -403,12 +414,19 @@ __device__ __forceinline__ void wave_raw(double k, double dx, double dy, double 
 // abstract_greens_function.jl:88-155.
 template <int WAVE, class T>
 __device__ __forceinline__ void wave_term(T k, T dx, T dy, T zsum, T area, cplxT<T>& Sv,
-                                          cplxT<T> gxi[3], cplxT<T> gx[3]) {
+                                          cplxT<T> gxi[3], cplxT<T> gx[3], const DelhTab& tab = DelhTab{}) {
     const T r = k * horiz_dist(dx, dy);
     const T z = k * zsum;
     cplxT<T> G, dGdr, dGdz;
     if (WAVE == 1) wu_dimless<true>(r, z, G, dGdr, dGdz);
     if (WAVE == 2) delhommeau_dimless(r, z, G, dGdr, dGdz);
+    if constexpr (WAVE == 3) {  // values only (T = double)
+        double g[2], gr[2], gz[2];
+        delh_table_eval(tab, r, z, g, gr, gz);
+        G = {g[0], g[1]};
+        dGdr = {gr[0], gr[1]};
+        dGdz = {gz[0], gz[1]};
+    }
     T d1 = Cst<T>::make(0.0), d2 = Cst<T>::make(0.0);  // dr/dxi_1, dr/dxi_2
     if (::fabs(val(r)) > 1e-6) {
         const T kk = k * k / r;

@@ -157,6 +157,9 @@ int launch_assemble_onepass(Context* ctx, const mh_mesh_t* mesh, const PanelRec*
 int launch_assemble(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels,
                     int gf_mask, int form, int64_t nk, const double* d_k, double2* S,
                     double2* D, double2* K, int64_t ld, int64_t stride, float2* DK32 = nullptr);
+// Table of the Delhommeau term on ctx's device (built on first use, then
+// shared by every context on the device until the process ends).
+int get_delh_table(Context* ctx, DelhTab* out);
 int launch_gf_eval(Context* ctx, int gf_mask, int op, int wrt_first, int64_t npairs,
                    const double* points, const mh_panel_t* panels, double k, double2* out);
 int launch_wave_term(Context* ctx, int term, int64_t n, const double* r, const double* z,

@@ -70,7 +70,8 @@ extern "C" int mh_assemble_noalloc(mh_context_t ctx, const mh_mesh_t* mesh, int 
         MH_REQUIRE(mesh->vertices && mesh->faces && mesh->centers && mesh->normals && mesh->areas &&
                        mesh->radii,
                    "mesh: null array");
-    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble_noalloc: bad Green-function mask");
+    MH_REQUIRE(gf_mask > 0 && gf_mask < 32, "assemble_noalloc: bad Green-function mask (the tabulated "
+                                            "Delhommeau term allocates its table and is not available here)");
     MH_REQUIRE(!((gf_mask & MH_GF_WU) && (gf_mask & MH_GF_DELHOMMEAU)), "assemble_noalloc: choose one wave term");
     MH_REQUIRE(formulation >= 1 && formulation <= 3, "assemble_noalloc: formulation must be 1, 2 or 3");
     MH_REQUIRE(nk >= 0, "assemble_noalloc: negative nk");

@@ -355,7 +355,7 @@ int launch_assemble_vjp(Context* ctx, const mh_mesh_t* m, const PanelRec* panels
     MH_REQUIRE(form == 1 || form == 2, "assemble_vjp: formulation must be direct (1) or indirect (2)");
     MH_REQUIRE(tiles <= 65535, "assemble_vjp: too many panels for one launch");
     const int rank = gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG);
-    const int wavem = gf & (GF_WU | GF_DELHOMMEAU);
+    const int wavem = gf & kGfWaveMask;  // the tabulated term is differentiated as the exact one
     double* Wr = ws;
     double* Wd = Wr + n * n;
     double* colR = Wd + n * n;

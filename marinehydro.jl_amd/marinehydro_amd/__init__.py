@@ -6,7 +6,7 @@ reference's Julia API for that path.
 """
 from . import _lib
 from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineReflected,
-                  RankineReflectedNegative, GFWu, ExactGuevelDelhommeau, gf_mask, Element, element,
+                  RankineReflectedNegative, GFWu, ExactGuevelDelhommeau, TabulatedDelhommeau, gf_mask, Element, element,
                   gf_eval, greens, gradient_greens, integral, integral_gradient,
                   both_integral_and_integral_gradient, DeviceMesh, assemble_matrices,
                   assemble_matrices_batched, assemble_matrix_wu, solve, radiation_bc,

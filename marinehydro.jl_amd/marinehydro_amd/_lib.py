@@ -29,6 +29,7 @@ GF_RANKINE_REFLECTED = 2
 GF_RANKINE_REFLECTED_NEG = 4
 GF_WU = 8
 GF_DELHOMMEAU = 16
+GF_DELHOMMEAU_TABLE = 32
 
 DIRECT, INDIRECT, BOTH = 1, 2, 3
 OP_GREENS, OP_GRADIENT_GREENS, OP_INTEGRAL, OP_INTEGRAL_GRADIENT = 0, 1, 2, 3

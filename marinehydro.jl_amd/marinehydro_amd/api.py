@@ -95,6 +95,14 @@ class ExactGuevelDelhommeau(GreensFunction):
     mask = L.GF_DELHOMMEAU
 
 
+class TabulatedDelhommeau(GreensFunction):
+    """The Guevel-Delhommeau term from a table of the exact one (no
+    reference counterpart): bicubic Hermite interpolation of its real part
+    with the log singularity subtracted, closed-form imaginary part; within
+    about 1e-9 of ExactGuevelDelhommeau.  Derivatives use the exact term."""
+    mask = L.GF_DELHOMMEAU_TABLE
+
+
 def gf_mask(gfs) -> int:
     """Bit mask of a GreensFunction or an iterable of them (the reference
     sums over tuples/vectors, abstract_greens_function.jl:18-83)."""
@@ -107,8 +115,9 @@ def gf_mask(gfs) -> int:
         if m & g.mask:
             raise ValueError(f"{g!r} appears twice in the Green-function tuple")
         m |= g.mask
-    if (m & L.GF_WU) and (m & L.GF_DELHOMMEAU):
-        raise ValueError("choose one wave term (GFWu or ExactGuevelDelhommeau)")
+    w = m & (L.GF_WU | L.GF_DELHOMMEAU | L.GF_DELHOMMEAU_TABLE)
+    if w & (w - 1):
+        raise ValueError("choose one wave term (GFWu, ExactGuevelDelhommeau or TabulatedDelhommeau)")
     if m == 0:
         raise ValueError("empty Green-function tuple")
     return m
