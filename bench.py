@@ -137,6 +137,30 @@ def cpu_baseline(args, mh, mesh, omegas, dofs, betas, g, rho, res, grad):
         ({k: float(v) for k, v in gerrs.items()} if gerrs else None)
 
 
+def delhommeau_assembly(mh, torch, np):
+    """S and D of the 4,000-panel cylinder at 32 wavenumbers (omega 0.2-2.5
+    rad/s), exact and tabulated Delhommeau term, direct formulation: wall
+    time of each assembly (after one warm-up call that also builds the
+    table) and the largest difference relative to the largest entry."""
+    c = mh.vertical_cylinder(5.0, 10.0, ntheta=80, nz=30, nr=20).mesh
+    dm = mh.DeviceMesh(c)
+    ks = np.linspace(0.2, 2.5, 32) ** 2 / 9.81
+    out = {}
+    for name, wave in (("exact", mh.ExactGuevelDelhommeau()), ("table", mh.TabulatedDelhommeau())):
+        gfs = (mh.Rankine(), mh.RankineReflected(), wave)
+        mh.assemble_matrices_batched(gfs, dm, ks[:2])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S, D = mh.assemble_matrices_batched(gfs, dm, ks)
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0, S, D)
+        log(f"bench: Delhommeau ({name}) assembly, 4000 panels x 32 wavenumbers: {out[name][0] * 1e3:.1f} ms")
+    (te, Se, De), (tt, St, Dt) = out["exact"], out["table"]
+    eS = (St - Se).abs().max().item() / Se.abs().max().item()
+    eD = (Dt - De).abs().max().item() / De.abs().max().item()
+    log(f"bench: table vs exact: {te / tt:.1f}x faster; max difference S {eS:.2e}, D {eD:.2e} of the largest entry")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +179,9 @@ def main():
                          "for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--delhommeau-assembly", action="store_true",
+                    help="only the assembly of the 4,000-panel cylinder (R 5 m, draft 10 m) at 32 wavenumbers "
+                         "with the exact and the tabulated Delhommeau term (timings and their difference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep is "
@@ -178,6 +205,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"bench: copied {nbytes} bytes 3 times")
+        return
+    if args.delhommeau_assembly:
+        delhommeau_assembly(mh, torch, np)
         return
     rho, g = 1025.0, 9.81
     body = mh.hemisphere(1.0, ntheta=args.ntheta, nphi=args.nphi)
@@ -303,6 +333,14 @@ def main():
         ent = tr.get(f"{dom_name}|{args.precision}|{n}x{args.nomega}")
         if ent:
             roof["traffic"] = ent["bytes_per_launch"]
+    # matrix writes of the assembly, live: S and D in FP64 (+ D in FP32 on
+    # the mixed path) written once per frequency, over the kernel's time
+    asm_write = None
+    if "assemble" in stats:
+        wbytes = (32.0 + (8.0 if args.precision == "mixed" else 0.0)) * n * n * args.nomega
+        asm_write = {"bytes": wbytes, "ms": round(stats["assemble"]["ms"], 3),
+                     "tb_per_s": round(wbytes / (stats["assemble"]["ms"] * 1e-3) / 1e12, 3),
+                     "frac_of_hbm": round(wbytes / (stats["assemble"]["ms"] * 1e-3) / (HBM_PEAK * 1e9), 3)}
     kernel_ms = {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"bench: kernel ms per sweep {kernel_ms}")
 
@@ -339,6 +377,7 @@ def main():
         "rel_err_grad_vs_oracle_fd": grad_err,
         "fp64_forward_fd_sweep_ms": round(fd_ms, 3) if fd_ms else None,
         "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,
+        "assembly_matrix_writes": asm_write,
         "kernel_ms_per_step": kernel_ms,
     }
     print(json.dumps(out), flush=True)
