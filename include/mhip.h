@@ -129,10 +129,11 @@ int mh_profile_report(mh_context_t ctx, char* buf, int64_t cap);
 /* Mixed-precision control (no reference counterpart; the reference solves
  * in FP64 only).  max_sweeps: iterative-refinement sweeps allowed per
  * frequency chunk (default 12, range 1-100); fp64_fallback (default 1): a
- * chunk whose refinement does not reach the 1e-10 relative residual, whose
- * FP32 factorisation meets an exact zero pivot, or whose matrix entries lie
- * beyond the split range of the FP32 factorisation (|x| >= 16) is solved
- * again in FP64 in the same call; with 0 the call returns
+ * chunk whose refinement does not reach the 1e-10 relative residual or
+ * whose FP32 factorisation meets an exact zero pivot is solved again in
+ * FP64 in the same call (the FP32 factorisation scales every row and column
+ * of its matrix-core operands by a power of two, so no finite magnitude is
+ * out of its range); with 0 the call returns
  * MH_ERR_NOT_CONVERGED / MH_ERR_SINGULAR instead.  A NaN or Inf residual is
  * always returned as MH_ERR_NOT_CONVERGED.  mh_fallback_count: chunks solved
  * again in FP64 since the context was created. */

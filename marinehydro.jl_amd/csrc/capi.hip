@@ -192,6 +192,7 @@ void context_free(Context* c) {
     }
     if (c->tri_err) (void)hipFree(c->tri_err);
     if (c->rk) (void)hipFree(c->rk);
+    if (c->himg) (void)hipFree(c->himg);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->aux) (void)hipStreamDestroy(c->aux);
@@ -402,9 +403,8 @@ static int check_info(Context* c, const int* info, int64_t batch) {
             return MH_ERR_SINGULAR;
         }
         if (h[b] & 2) {
-            set_error("mixed precision: matrix entries beyond the split range of the FP32 factorisation "
-                      "(|x| >= 65504 / 4096 = 16 after the 2^12 operand scaling) in batch item " +
-                      std::to_string(b) + "; use the FP64 path");
+            set_error("mixed precision: non-finite entries in the FP32 factorisation of batch item " +
+                      std::to_string(b));
             return MH_ERR_NOT_CONVERGED;
         }
     }

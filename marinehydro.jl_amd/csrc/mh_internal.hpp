@@ -56,6 +56,9 @@ struct Context {
     // frequency-independent Rankine parts of the assembly (grow-only)
     void* rk = nullptr;
     size_t rk_bytes = 0;
+    // split operand images of the FP32 LU's trailing updates (grow-only)
+    void* himg = nullptr;
+    size_t himg_bytes = 0;
     bool profiling = false;
     std::vector<ProfRec> prof;
     // mixed precision: refinement sweeps allowed, whether a frequency chunk

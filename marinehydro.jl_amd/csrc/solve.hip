@@ -775,37 +775,128 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
 }
 
 // FP32 trailing updates of the mixed-precision LU on half-precision matrix
-// cores.  Every FP32 operand is scaled by 2^12 and split as x = hi + lo (hi =
-// fp16(x), lo = fp16(x - hi): 22 significant bits together; the scaling keeps
-// lo out of the fp16 subnormal range for |x| down to 3e-5, below which the
-// absolute error is under 2^-36) and each real product is formed as hi*hi +
-// hi*lo + lo*hi with FP32 accumulation, the sum scaled back by 2^-24; the
-// dropped lo*lo term is below 2^-22 of the product, the size of the FP32
-// accumulation error over a 128-deep update.  The complex product folds into real
-// products of doubled depth: with the 32 complex depths of a chunk laid out
-// as [re (32) | im (32)],
+// cores, C -= L21 U12 with L21 = A[r, k_lo:k_lo+kw] and U12 = A[k_lo:k_lo+kw, c].
+//
+// Operand images.  Once per block column, L21 and U12 are scaled (row i of
+// L21 by 2^er_i, column j of U12 by 2^ec_j, powers of two chosen so that the
+// row's or column's largest |re| or |im| lands in [2^13, 2^14)) and split as
+// x = hi + lo (hi = fp16(x), lo = fp16(x - hi): 22 significant bits
+// together).  Each real product is then formed as hi*hi + hi*lo + lo*hi with
+// FP32 accumulation and the sum scaled back by 2^-(er_i + ec_j), which is
+// exact.  The dropped lo*lo term is below 2^-22 of the product, the size of
+// the FP32 accumulation error over a 128-deep update; an entry more than
+// 2^17 below its row's or column's maximum has a subnormal lo, an absolute
+// error under 2^-39 of that maximum.  No finite entry is out of range; a
+// non-finite one sets bit 1 of info[item].
+//
+// The images hold 16-byte units of 8 halves.  Per item, for depth chunk c
+// (16 complex depths), hi/lo plane h and row (or column) x:
+//   unit ((c * 2 + h) * pad + x) * 4 + q,
+//   q = 0: re of depths 0-7, 1: re 8-15, 2: im 0-7, 3: im 8-15,
+// so one chunk of one 128-row tile is two contiguous 8 KB blocks.  The L21
+// image of block column k sits in one of two slots (k mod 2: the next
+// panel's image is written while the wide update of this one still reads
+// its own); the U12 image is indexed by absolute column, since the narrow
+// and the wide update cover disjoint columns.
+//
+// The complex product folds into one real product of depth 32 per chunk:
+// with the depths of a chunk laid out as [re (16) | im (16)],
 //   Re C = [Ar | Ai] . [Br ; -Bi],   Im C = [Ar | Ai] . [Bi ; Br]
-// (B is staged as [re | im | -im], so every operand is a plain LDS load),
-// and one v_mfma_f32_16x16x32_f16 covers a 16 x 16 block, 32 real depths
-// and one split term: twelve per block and chunk, against 32 FP32 MFMAs of
-// twice the cycles.  Lane layout of that instruction: A[m = l%16][k = 8 (l/16)
-// + e], B[k = 8 (l/16) + e][n = l%16], result register r is row 4 (l/16) + r.
-// As in k_lu_gemm_big the tile's columns are the instruction's rows, so the
-// results run along columns of C.  An operand outside the half-precision
-// range after scaling (|x| >= 65504 / 4096 = 15.99) sets bit 1 of
-// info[item].
+// (the -Bi half negated in registers), one v_mfma_f32_16x16x32_f16 per
+// 16 x 16 block and split term: six per block and chunk.  Lane layout of
+// that instruction: A[m = l%16][k = 8 (l/16) + e], B[k = 8 (l/16) + e]
+// [n = l%16], result register r is row 4 (l/16) + r.  As in k_lu_gemm_big
+// the tile's columns are the instruction's rows, so the results run along
+// columns of C.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-constexpr int kHK = 32;   // complex depths per chunk
-constexpr int kHPA = 72;  // LDS pitch of A in halves: 64 depths + 8 padding
-constexpr int kHPB = 104; // LDS pitch of B in halves: 96 depths + 8 padding
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+constexpr int kHK = 16;                      // complex depths per chunk
+constexpr int kHChunks = kNBOuter / kHK;     // chunks of a full block column
+constexpr int kHUnits = kHChunks * 2 * 4;    // image units per row or column
+constexpr int kHEMax = 14;                   // largest |x| of a row or column scaled into [2^13, 2^14)
 
-constexpr float kHScale = 4096.0f;      // 2^12 per operand
-constexpr float kHUnscale = 0x1p-24f;   // 2^-24 per product
-__device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo, bool& bad) {
-    const float xs = x * kHScale;  // exact
-    bad = bad || !(::fabsf(xs) < 65504.0f);
+__device__ __forceinline__ void split_h(float xs, _Float16& hi, _Float16& lo) {
     hi = (_Float16)xs;
-    lo = (_Float16)(xs - (float)hi);
+    lo = (_Float16)(xs - (float)hi);  // exact in FP32
+}
+
+// exponent scaling a row's or column's largest magnitude mx into [2^13, 2^14)
+__device__ __forceinline__ int split_exp(float mx) {
+    if (!(mx > 0.0f)) return 0;
+    int E;
+    (void)frexpf(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+    return kHEMax - E;
+}
+
+__device__ __forceinline__ bool finite2(float2 v) {
+    return ::fabsf(v.x) <= 3.402823466e38f && ::fabsf(v.y) <= 3.402823466e38f;  // false for NaN
+}
+
+// Operand images: 16 rows of L21 (ROWS) or columns of U12 per 512-thread
+// workgroup; wave c takes depth chunk c, lane (row or column x, unit q) the
+// 8 depths of image unit q (re or im), so each store of a wave writes 1 KB
+// of one plane contiguously.  The maximum of a row (column) is reduced over
+// its four units in registers and over the chunks through LDS.  L21: rows
+// [x0, x1) with x1 = npad, rows from n on written as zeros; U12: columns
+// [x0, x1).
+template <bool ROWS>
+__global__ __launch_bounds__(512) void k_split(const float2* __restrict__ A, int64_t ld, int64_t stride, int64_t n,
+                                               int64_t x0, int64_t x1, int64_t pad, int64_t k_lo, int kw,
+                                               u4v* __restrict__ img, int64_t img_stride, int* __restrict__ xexp,
+                                               int64_t xexp_stride, int* __restrict__ info) {
+    const int64_t item = blockIdx.y;
+    const int tid = threadIdx.x, c = tid >> 6, xl = (tid >> 2) & 15, q = tid & 3;
+    const int64_t x = x0 + (int64_t)blockIdx.x * 16 + xl;
+    const bool out_ok = x < x1, live = out_ok && (!ROWS || x < n);
+    const int d0 = kHK * c + 8 * (q & 1);
+    const float2* M = A + item * stride;
+    float v[8];
+    float mx = 0.0f, bad = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int64_t d = k_lo + d0 + e;
+        float2 z = make_float2(0.f, 0.f);
+        if (live && d0 + e < kw) z = ROWS ? M[x + d * ld] : M[d + x * ld];
+        v[e] = (q & 2) ? z.y : z.x;
+        if (!finite2(z)) bad = 1.0f;
+        mx = fmaxf(mx, fmaxf(::fabsf(z.x), ::fabsf(z.y)));
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 1));  // units q and q ^ 1 cover the chunk's 16 depths
+    bad = fmaxf(bad, __shfl_xor(bad, 1));
+    __shared__ float smx[8][16], sbad[8][16];
+    if (q == 0) {
+        smx[c][xl] = mx;
+        sbad[c][xl] = bad;
+    }
+    __syncthreads();
+    mx = 0.0f;
+    bad = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        mx = fmaxf(mx, smx[t][xl]);
+        bad = fmaxf(bad, sbad[t][xl]);
+    }
+    if (!out_ok) return;
+    int e = split_exp(mx);
+    if (bad != 0.0f) {
+        if (c == 0 && q == 0) atomicOr(info + item, 2);
+        e = 0;
+    }
+    if (kHK * c < kw) {
+        h8 hi, lo;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            _Float16 a0, a1;
+            split_h(__builtin_amdgcn_ldexpf(v[t], e), a0, a1);
+            hi[t] = a0;
+            lo[t] = a1;
+        }
+        u4v* out = img + item * img_stride;
+        out[((int64_t)(2 * c) * pad + x) * 4 + q] = __builtin_bit_cast(u4v, hi);
+        out[((int64_t)(2 * c + 1) * pad + x) * 4 + q] = __builtin_bit_cast(u4v, lo);
+    }
+    if (c == 0 && q == 0) xexp[item * xexp_stride + x] = e;
 }
 
 // XCD-aware tile order: workgroups are dealt to the 8 XCDs in turn, so
@@ -826,21 +917,134 @@ __device__ __forceinline__ bool xcd_tile(int64_t ntr, int64_t ntc, int64_t ntile
     return true;
 }
 
-__global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int64_t ld, int64_t stride,
-                                                    int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
-                                                    int64_t k_lo, int kw, int* __restrict__ info, int64_t ntr,
-                                                    int64_t ntc, int64_t ntiles) {
+// LDS slot of unit q of tile row (or column) x: the four units of a row are
+// permuted by the XOR key (x >> 2) & 3.  Rows x, x + 4, x + 8, x + 12 share
+// banks (64 banks of 4 bytes = four 64-byte rows); the key gives them four
+// different slots, and rows x and x + 4 read at units q and q ^ 2 (the two
+// halves of a wave) different slots as well.
+__device__ __forceinline__ int hkey(int x) { return (x >> 2) & 3; }
+__device__ __forceinline__ int hslot(int x, int q) { return x * 4 + (q ^ hkey(x)); }
+
+typedef __attribute__((address_space(3))) void* lptr_t;
+
+// Copies chunk c of a tile's L21 and U12 images into LDS (four 16-byte
+// buffer loads per lane straight to LDS, no registers): LDS unit p = (h 128
+// + x) 4 + s holds image unit q = s ^ key(x) of row (column) x, plane h.
+// vo = (4 x + q) 16 is the lane's byte offset within a plane of the tile;
+// planes lie npad (ncpad) rows apart, so the chunk and plane select a
+// wave-uniform offset.
+__device__ __forceinline__ void h_stage(__amdgpu_buffer_rsrc_t rl, __amdgpu_buffer_rsrc_t ru, int64_t npad,
+                                        int64_t ncpad, int c, u4v* sl, u4v* su, uint32_t vo, int wave) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t pl = (uint32_t)((2 * c + h) * npad * 64), pu = (uint32_t)((2 * c + h) * ncpad * 64);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rl, (lptr_t)(sl + 512 * h + 64 * wave), 16, vo, pl, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(ru, (lptr_t)(su + 512 * h + 64 * wave), 16, vo, pu, 0, 0);
+    }
+}
+
+// One chunk: 48 matrix-core products per wave out of one LDS stage.
+__device__ __forceinline__ void h_chunk(const u4v* sl, const u4v* su, int l, int wr, int wc, f4 (&accr)[4][2],
+                                        f4 (&acci)[4][2]) {
+    const int g = l >> 4;
+    const unsigned neg = g >= 2 ? 0x80008000u : 0u;  // -Bi in the upper half of [Br ; -Bi]
+    const u4v m = (u4v){neg, neg, neg, neg};
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int col = 32 * wc + 16 * b + (l & 15);
+        const h8 b1h = __builtin_bit_cast(h8, su[hslot(col, g)] ^ m);
+        const h8 b1l = __builtin_bit_cast(h8, su[512 + hslot(col, g)] ^ m);
+        const h8 b2h = __builtin_bit_cast(h8, su[hslot(col, g ^ 2)]);
+        const h8 b2l = __builtin_bit_cast(h8, su[512 + hslot(col, g ^ 2)]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int row = 64 * wr + 16 * a + (l & 15);
+            const h8 ah = __builtin_bit_cast(h8, sl[hslot(row, g)]);
+            const h8 al = __builtin_bit_cast(h8, sl[512 + hslot(row, g)]);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, ah, accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, ah, acci[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1l, ah, accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2l, ah, acci[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, al, accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, al, acci[a][b], 0, 0, 0);
+        }
+        // the second column block's fragments are not read ahead into this
+        // block's products (which would need 16 more VGPRs than 128 allow)
+        if (b == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Workgroup barriers of the depth loop, without the release fence of
+// __syncthreads (which would wait for every load in flight to LDS, those
+// into the other stage included).  The first waits for this wave's loads of
+// the stage about to be read (all but the four of the next chunk, or all),
+// the second for its LDS reads, which the next loads into the same stage
+// must not overtake; the memory clobber keeps the compiler from moving LDS
+// accesses across either.
+__device__ __forceinline__ void h_barrier_ready(bool next_in_flight) {
+    if (next_in_flight)
+        asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" ::: "memory");
+    else
+        asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+}
+__device__ __forceinline__ void h_barrier_done() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// The depth loop (kNBOuter = 8 chunks of 16) over two LDS stages, chunk c
+// in stage c mod 2.  The stages are distinct restrict parameters, so the
+// compiler's own wait before reading one stage does not include the loads
+// in flight into the other.
+__device__ __forceinline__ void h_loop(__amdgpu_buffer_rsrc_t rl, __amdgpu_buffer_rsrc_t ru, int64_t npad,
+                                       int64_t ncpad, u4v* __restrict__ sL0, u4v* __restrict__ sU0,
+                                       u4v* __restrict__ sL1, u4v* __restrict__ sU1, uint32_t vo, int wave, int l,
+                                       int wr, int wc, f4 (&accr)[4][2], f4 (&acci)[4][2]) {
+    h_stage(rl, ru, npad, ncpad, 0, sL0, sU0, vo, wave);
+    h_stage(rl, ru, npad, ncpad, 1, sL1, sU1, vo, wave);
+#pragma unroll
+    for (int c = 0; c < kHChunks; c += 2) {
+        h_barrier_ready(true);
+        h_chunk(sL0, sU0, l, wr, wc, accr, acci);
+        if (c + 2 < kHChunks) {
+            h_barrier_done();  // every wave is done with this stage
+            h_stage(rl, ru, npad, ncpad, c + 2, sL0, sU0, vo, wave);
+        }
+        h_barrier_ready(c + 2 < kHChunks);
+        h_chunk(sL1, sU1, l, wr, wc, accr, acci);
+        if (c + 3 < kHChunks) {
+            h_barrier_done();
+            h_stage(rl, ru, npad, ncpad, c + 3, sL1, sU1, vo, wave);
+        }
+    }
+}
+
+// 128 x 128 tile per 512-thread workgroup (8 waves, each 64 x 32 = 4 x 2
+// blocks), operand chunks double-buffered in 64 KB of LDS and at most 128
+// VGPRs (two workgroups per CU), with the loads of chunk c + 1 in flight
+// while chunk c is multiplied.  Depth kNBOuter (the width of every block
+// column that has a trailing update).
+__global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(float2* __restrict__ A, int64_t ld, int64_t stride,
+                                                       int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
+                                                       const u4v* __restrict__ Limg, int64_t l_stride,
+                                                       int64_t npad, const int* __restrict__ rexp,
+                                                       int64_t r_stride, const u4v* __restrict__ Uimg,
+                                                       int64_t u_stride, int64_t ncpad,
+                                                       const int* __restrict__ cexp, int64_t c_stride,
+                                                       int64_t ntr, int64_t ntc, int64_t ntiles) {
     int64_t item, ti, tj;
     if (!xcd_tile(ntr, ntc, ntiles, item, ti, tj)) return;
     float2* M = A + item * stride;
     const int64_t tr = r_lo + ti * kBT;
     const int64_t tc = c_lo + tj * kBT;
-    // [hi, lo][tile row][depth: re 0..31, im 32..63] and
-    // [hi, lo][tile column][depth: re 0..31, im 32..63, -im 64..95]
-    __shared__ __attribute__((aligned(16))) _Float16 sA[2][kBT][kHPA];
-    __shared__ __attribute__((aligned(16))) _Float16 sB[2][kBT][kHPB];
-    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    __shared__ u4v sL0[1024], sU0[1024], sL1[1024], sU1[1024];
+    const int tid = threadIdx.x, l = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
+    // this lane's staging unit: row (column) x of the tile, image unit q;
+    // buffer ranges end with the item's image (reads beyond return zeros)
+    const int x = tid >> 2, q = (tid & 3) ^ hkey(x);
+    const uint32_t vo = (uint32_t)(4 * x + q) * 16u;
+    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<u4v*>(Limg + item * l_stride + tr * 4), (short)0, (int)((l_stride - tr * 4) * 16), 0x00020000);
+    const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<u4v*>(Uimg + item * u_stride + tc * 4), (short)0, (int)((u_stride - tc * 4) * 16), 0x00020000);
     f4 accr[4][2], acci[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -849,154 +1053,69 @@ __global__ __launch_bounds__(512) void k_lu_gemm_h3(float2* __restrict__ A, int6
             accr[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
             acci[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
         }
-    // global loads of a chunk: A by (row, 8 depths), B by (column, 8 depths);
-    // raw buffer loads with a per-lane byte offset and a wave-uniform one,
-    // offsets of entries outside the ranges replaced by 0 and the values by
-    // zero (the host checks the extent fits 31 bits)
-    const int ra = tid & (kBT - 1), ga = __builtin_amdgcn_readfirstlane(tid >> 7);
-    const int cb = tid >> 2, gb = tid & 3;
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        M, (short)0, (int)(((r_hi - 1) + (c_hi - 1) * ld + 1) * 8), 0x00020000);
-    const bool rok = tr + ra < r_hi, cok = tc + cb < c_hi;
-    const uint32_t voa = rok ? (uint32_t)(tr + ra) * 8u : 0u;
-    const uint32_t vob = cok ? (uint32_t)((tc + cb) * ld + k_lo + 8 * gb) * 8u : 0u;
-    // operand chunks go through a ring of three register slots: the first
-    // three are loaded up front and chunk c + 3 is loaded as soon as chunk c
-    // has been staged, so two chunks of loads stay in flight
-    constexpr int kSlots = 3;
-    constexpr int kMaxChunks = kNBOuter / kHK;
-    float2 pa[kSlots][8], pb[kSlots][8];
-    bool bad = false;
-    auto gload = [&](int slot, int c) {
-        const int kc = kHK * c;
+    h_loop(rl, ru, npad, ncpad, sL0, sU0, sL1, sU1, vo, wave, l, wr, wc, accr, acci);
+    // C -= 2^-(er + ec) acc, one block of 16 rows (a) at a time, through
+    // buffer loads and stores with 32-bit offsets in the item's matrix; an
+    // entry outside [r_lo, r_hi) x [c_lo, c_hi) gets an offset beyond the
+    // buffer's range (its load reads zero) and is not stored.  Block
+    // a + 1's loads are issued before block a's stores, so waiting for them
+    // does not wait for stores.
+    const __amdgpu_buffer_rsrc_t rc =
+        __builtin_amdgcn_make_buffer_rsrc(M, (short)0, (int)(((c_hi - 1) * ld + r_hi) * 8), 0x00020000);
+    const int64_t gr0 = tr + 64 * wr + (l & 15), gc0 = tc + 32 * wc + 4 * (l >> 4);
+    int ec[2][4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int ka = kc + 8 * ga + e, kb = kc + 8 * gb + e;
-            const bool ina = rok && ka < kw, inb = cok && kb < kw;
-            const uint32_t soa = ka < kw ? (uint32_t)((k_lo + ka) * ld) * 8u : 0u;
-            const uint32_t sob = (uint32_t)(kc + e) * 8u;
-            const float2 va =
-                __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voa, (int)soa, 0));
-            const float2 vb = __builtin_bit_cast(
-                float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(inb ? vob : 0u), (int)(inb ? sob : 0u), 0));
-            pa[slot][e] = ina ? va : make_float2(0.f, 0.f);
-            pb[slot][e] = inb ? vb : make_float2(0.f, 0.f);
-        }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ec[b][r] = cexp[item * c_stride + gc0 + 16 * b + r];
+    auto off = [&](int a, int b, int r) -> uint32_t {
+        const int64_t gr = gr0 + 16 * a, gc = gc0 + 16 * b + r;
+        return (gr < r_hi && gc < c_hi) ? (uint32_t)((gr + gc * ld) * 8) : 0x80000000u;
     };
-    auto lstore = [&](int slot) {
-        h8 rh, rl, ih, il;
+    float2 cv[2][2][4];
+    auto load = [&](int a, float2 (&v)[2][4]) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            _Float16 a0, a1, a2, a3;
-            split_h(pa[slot][e].x, a0, a1, bad);
-            split_h(pa[slot][e].y, a2, a3, bad);
-            rh[e] = a0;
-            rl[e] = a1;
-            ih[e] = a2;
-            il[e] = a3;
-        }
-        *reinterpret_cast<h8*>(&sA[0][ra][8 * ga]) = rh;
-        *reinterpret_cast<h8*>(&sA[1][ra][8 * ga]) = rl;
-        *reinterpret_cast<h8*>(&sA[0][ra][32 + 8 * ga]) = ih;
-        *reinterpret_cast<h8*>(&sA[1][ra][32 + 8 * ga]) = il;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            _Float16 a0, a1, a2, a3;
-            split_h(pb[slot][e].x, a0, a1, bad);
-            split_h(pb[slot][e].y, a2, a3, bad);
-            rh[e] = a0;
-            rl[e] = a1;
-            ih[e] = a2;
-            il[e] = a3;
-        }
-        *reinterpret_cast<h8*>(&sB[0][cb][8 * gb]) = rh;
-        *reinterpret_cast<h8*>(&sB[1][cb][8 * gb]) = rl;
-        *reinterpret_cast<h8*>(&sB[0][cb][32 + 8 * gb]) = ih;
-        *reinterpret_cast<h8*>(&sB[1][cb][32 + 8 * gb]) = il;
-        *reinterpret_cast<h8*>(&sB[0][cb][64 + 8 * gb]) = -ih;
-        *reinterpret_cast<h8*>(&sB[1][cb][64 + 8 * gb]) = -il;
+            for (int r = 0; r < 4; ++r)
+                v[b][r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rc, (int)off(a, b, r), 0, 0));
     };
-    const int dq = 8 * (l >> 4);  // this lane's depths within a 32-deep step
-    float2 cv[4][2][4];           // this lane's C entries (half read during the last chunk)
+    load(0, cv[0]);
 #pragma unroll
-    for (int c = 0; c < kSlots; ++c)
-        if (kHK * c < kw) gload(c, c);
+    for (int a = 0; a < 4; ++a) {
+        if (a + 1 < 4) load(a + 1, cv[(a + 1) & 1]);
+        const int er = rexp[item * r_stride + gr0 + 16 * a];
 #pragma unroll
-    for (int c = 0; c < kMaxChunks; ++c) {
-        if (kHK * c >= kw) break;
-        if (c > 0) __syncthreads();  // previous chunk's operands consumed
-        lstore(c % kSlots);
-        if (kHK * (c + kSlots) < kw) gload(c % kSlots, c + kSlots);
-        if (kHK * (c + 1) >= kw) {  // last chunk: read half of this lane's C entries meanwhile
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
-                        const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
-                        cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : make_float2(0.f, 0.f);
-                    }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {  // real depths: re (st 0), im (st 1)
-            // B operands: [Br ; -Bi] for Re C and [Bi ; Br] for Im C
-            const int o1 = st == 0 ? dq : 64 + dq;
-            const int o2 = st == 0 ? 32 + dq : dq;
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int col = 32 * wc + 16 * b + (l & 15);
-                const h8 b1h = *reinterpret_cast<const h8*>(&sB[0][col][o1]);
-                const h8 b1l = *reinterpret_cast<const h8*>(&sB[1][col][o1]);
-                const h8 b2h = *reinterpret_cast<const h8*>(&sB[0][col][o2]);
-                const h8 b2l = *reinterpret_cast<const h8*>(&sB[1][col][o2]);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int row = 64 * wr + 16 * a + (l & 15);
-                    const h8 ah = *reinterpret_cast<const h8*>(&sA[0][row][32 * st + dq]);
-                    const h8 al = *reinterpret_cast<const h8*>(&sA[1][row][32 * st + dq]);
-                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, ah, accr[a][b], 0, 0, 0);
-                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, ah, acci[a][b], 0, 0, 0);
-                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1l, ah, accr[a][b], 0, 0, 0);
-                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2l, ah, acci[a][b], 0, 0, 0);
-                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1h, al, accr[a][b], 0, 0, 0);
-                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b2h, al, acci[a][b], 0, 0, 0);
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int sh = -(er + ec[b][r]);
+                const float2 v = make_float2(cv[a & 1][b][r].x - __builtin_amdgcn_ldexpf(accr[a][b][r], sh),
+                                             cv[a & 1][b][r].y - __builtin_amdgcn_ldexpf(acci[a][b][r], sh));
+                const uint32_t o = off(a, b, r);
+                if (o != 0x80000000u) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), rc, (int)o, 0, 0);
             }
-        }
     }
-    if (bad) atomicOr(info + item, 2);
-    // C -= acc: every entry of this lane read before any is written (a store
-    // between two loads would make each load wait for the store as well)
-#pragma unroll
-    for (int a = 2; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
-                const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
-                cv[a][b][r] = (gr < r_hi && gc < c_hi) ? M[gr + gc * ld] : make_float2(0.f, 0.f);
-            }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t gr = tr + 64 * wr + 16 * a + (l & 15);
-                const int64_t gc = tc + 32 * wc + 16 * b + 4 * (l >> 4) + r;
-                if (gr < r_hi && gc < c_hi)
-                    M[gr + gc * ld] = make_float2(cv[a][b][r].x - kHUnscale * accr[a][b][r],
-                                                  cv[a][b][r].y - kHUnscale * acci[a][b][r]);
-            }
 }
 
 // ---------------------------------------------------------------------------
 // host-side recursion
 // ---------------------------------------------------------------------------
+// Split operand images of the FP32 LU's trailing updates (in the context's
+// grow-only image buffer): two L21 slots (block column parity) and the U12
+// image, with their exponents; what each L21 slot holds (rows from r0,
+// depths k_lo .. k_lo + kw) is checked by the updates that read it.
+struct HImg {
+    u4v* L[2] = {nullptr, nullptr};
+    int* rexp[2] = {nullptr, nullptr};
+    u4v* U = nullptr;
+    int* cexp = nullptr;
+    int64_t npad = 0, ncpad = 0;
+    int64_t r0[2] = {-1, -1}, k_lo[2] = {-1, -1};
+    int kw[2] = {0, 0};
+    int slot = 0;  // slot of the block column the next updates use
+};
+
 template <class T>
 struct LuPlan {
     typedef typename Cx<T>::t C2;
@@ -1006,7 +1125,59 @@ struct LuPlan {
     int64_t* ipiv;
     int* info;
     int64_t pc_lo, pc_hi;  // block column being factored
+    HImg* img = nullptr;   // FP32 only
 };
+
+// Bytes of the image buffer for `batch` matrices of n rows and ncols columns.
+static size_t himg_bytes(int64_t n, int64_t ncols, int64_t batch) {
+    const int64_t npad = (n + kBT - 1) / kBT * kBT, ncpad = (ncols + kBT - 1) / kBT * kBT;
+    return (size_t)batch * ((size_t)(2 * npad + ncpad) * kHUnits * 16 + (size_t)(2 * npad + ncpad) * 4);
+}
+
+static int himg_setup(Context* ctx, int64_t n, int64_t ncols, int64_t batch, HImg& H) {
+    const size_t need = himg_bytes(n, ncols, batch);
+    if (need > ctx->himg_bytes) {
+        if (ctx->himg) {  // the previous factorisations have completed on the context's stream
+            MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->himg);
+            ctx->himg = nullptr;
+            ctx->himg_bytes = 0;
+        }
+        if (hipMalloc(&ctx->himg, need) != hipSuccess) {
+            ctx->himg = nullptr;
+            set_error("lu: hipMalloc of " + std::to_string(need) + " bytes for the split operand images failed");
+            return MH_ERR_HIP;
+        }
+        ctx->himg_bytes = need;
+    }
+    H.npad = (n + kBT - 1) / kBT * kBT;
+    H.ncpad = (ncols + kBT - 1) / kBT * kBT;
+    char* p = static_cast<char*>(ctx->himg);
+    const size_t lb = (size_t)batch * H.npad * kHUnits * 16, ub = (size_t)batch * H.ncpad * kHUnits * 16;
+    H.L[0] = reinterpret_cast<u4v*>(p);
+    H.L[1] = reinterpret_cast<u4v*>(p + lb);
+    H.U = reinterpret_cast<u4v*>(p + 2 * lb);
+    int* e = reinterpret_cast<int*>(p + 2 * lb + ub);
+    H.rexp[0] = e;
+    H.rexp[1] = e + batch * H.npad;
+    H.cexp = e + 2 * batch * H.npad;
+    return MH_OK;
+}
+
+// L21 image of the block column whose trailing rows start at r0
+static int lu_split_rows(const LuPlan<float>& P, int slot, int64_t r0, int64_t k_lo, int kw) {
+    HImg& H = *P.img;
+    const int64_t rows = H.npad - r0;
+    if (rows <= 0) return MH_OK;
+    ProfScope ps(P.ctx, "lu_split", 0.0, (8.0 * 2 + 8.0) * (double)(P.n - r0) * kw * P.batch);
+    hipLaunchKernelGGL(k_split<true>, dim3((unsigned)((rows + 15) / 16), (unsigned)P.batch), dim3(512), 0,
+                       P.ctx->stream, P.A, P.ld, P.stride, P.n, r0, H.npad, H.npad, k_lo, kw, H.L[slot],
+                       (int64_t)kHUnits * H.npad, H.rexp[slot], H.npad, P.info);
+    H.r0[slot] = r0;
+    H.k_lo[slot] = k_lo;
+    H.kw[slot] = kw;
+    return MH_OK;
+}
 
 template <class T>
 static int lu_swap(const LuPlan<T>& P, int64_t k0, int npiv, int64_t c_lo, int64_t c_hi) {
@@ -1023,25 +1194,47 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                    int64_t k_lo, int kw) {
     if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
-    const bool big = mr >= kBT && mc >= kBT;
+    // the split-image kernel takes tiles that start on multiples of kBT
+    // (the images and exponents end at the padded size) and full-depth blocks
+    const bool big = mr >= kBT && mc >= kBT &&
+                     (sizeof(T) == 8 || (r_lo % kBT == 0 && c_lo % kBT == 0 && kw == kNBOuter));
     const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_f16x3") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
-    ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
-                 2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
     if (big) {
         const dim3 grid((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch);
-        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores (kw <= kNBOuter)
+        if constexpr (sizeof(T) == 4) {  // FP32 LU of the mixed path: split-half matrix cores (kw = kNBOuter)
+            HImg* H = P.img;
+            MH_REQUIRE(H != nullptr, "lu: split operand images missing");
+            const int sl = H->slot;
+            MH_REQUIRE(H->r0[sl] >= 0 && r_lo >= H->r0[sl] && k_lo == H->k_lo[sl] && kw == H->kw[sl] &&
+                           kw == kNBOuter && r_hi <= H->npad && c_hi <= H->ncpad,
+                       "lu: trailing update does not match the split L21 image");
+            {  // U12 image of these columns
+                ProfScope pss(P.ctx, "lu_split", 0.0, (8.0 * 2 + 8.0) * (double)mc * kw * P.batch);
+                hipLaunchKernelGGL(k_split<false>, dim3((unsigned)((mc + 15) / 16), (unsigned)P.batch),
+                                   dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, P.n, c_lo, c_hi, H->ncpad,
+                                   k_lo, kw, H->U, (int64_t)kHUnits * H->ncpad, H->cexp, H->ncpad, P.info);
+            }
+            ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
+                         2.0 * sizeof(T) * (2.0 * mr * mc) * P.batch + 8.0 * (double)kw * (mr + mc) * P.batch);
             const int64_t ntr = grid.x, ntc = grid.y, ntiles = ntr * ntc * P.batch;
             const int64_t nwg = (ntiles + kXcd - 1) / kXcd * kXcd;
             MH_REQUIRE(nwg <= 0x7fffffff, "lu: too many tiles for one launch");
-            hipLaunchKernelGGL(k_lu_gemm_h3, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride,
-                               r_lo, r_hi, c_lo, c_hi, k_lo, kw, P.info, ntr, ntc, ntiles);
+            hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, P.A, P.ld,
+                               P.stride, r_lo, r_hi, c_lo, c_hi, H->L[sl],
+                               (int64_t)kHUnits * H->npad, H->npad, H->rexp[sl], H->npad, H->U,
+                               (int64_t)kHUnits * H->ncpad, H->ncpad, H->cexp, H->ncpad, ntr, ntc, ntiles);
         } else {
+            ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
+                         2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
             hipLaunchKernelGGL(k_lu_gemm_big<T>, grid, dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo,
                                r_hi, c_lo, c_hi, k_lo, kw);
         }
-    } else
+    } else {
+        ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
+                     2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
         hipLaunchKernelGGL(k_lu_gemm<T>, dim3((unsigned)((mr + 63) / 64), (unsigned)((mc + 63) / 64), (unsigned)P.batch),
                            dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, r_lo, r_hi, c_lo, c_hi, k_lo, kw);
+    }
     return MH_OK;
 }
 
@@ -1150,6 +1343,11 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
     MH_REQUIRE(sizeof(T) == 8 || (ncols * ld) * 8 <= 0x7fffffff, "lu: FP32 matrix extent above 2 GiB");
     MH_REQUIRE(n * ld * (int64_t)sizeof(typename Cx<T>::t) <= 0x7fffffff, "lu: matrix extent above 2 GiB");
     LuPlan<T> P{ctx, A, n, ncols, ld, stride, batch, ipiv, d_info, 0, 0};
+    HImg H;
+    if constexpr (sizeof(T) == 4) {
+        TRY_RC(himg_setup(ctx, n, ncols, batch, H));
+        P.img = &H;
+    }
     const int64_t npan = (n + kNBOuter - 1) / kNBOuter;
     TRY_RC(ensure_aux(ctx));
     TRY_RC(ensure_events(ctx, (size_t)(3 * npan + 2)));
@@ -1173,8 +1371,11 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         P.pc_lo = k0;
         P.pc_hi = k0 + w;
         rc = lu_panel(P, k0, w);
+        if constexpr (sizeof(T) == 4)  // L21 image of this block column, read by both updates
+            if (rc == MH_OK && kn0 < n) rc = lu_split_rows(P, (int)(k & 1), kn0, k0, w);
         if (rc != MH_OK) break;
         MH_HIP_CHECK(hipEventRecord(evF[k], sA));
+        H.slot = (int)(k & 1);
         if (inv) {  // this block column's pivots and diagonal blocks are final
             MH_HIP_CHECK(hipStreamWaitEvent(sT, evF[k], 0));
             ctx->stream = sT;

@@ -51,3 +51,27 @@ def test_nan_in_the_system_is_an_error_not_converged(mh, gpu):
     # the context keeps working
     A, B, F = mh.freq_sweep(m, [1.0], [[0, 0, 1.0]], precision="mixed")
     assert np.all(np.isfinite(A)) and np.all(np.isfinite(B))
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_fp32_factors_converge_in_two_sweeps(mh, gpu, direct):
+    """The FP32 factorisation (its trailing updates on the half-precision
+    matrix cores through split, per-row/column scaled operands) is accurate
+    to FP32 level: one FP32 solve and one correction reach the 1e-10
+    residual tolerance.  A factorisation with fp16-level errors (a dropped
+    split term, a wrong scale) contracts the error by ~1e-3 per sweep and
+    fails here, since the FP64 fallback is off.  1,058 panels: edge tiles
+    in both directions (1058 = 8 x 128 + 34), narrow and wide updates."""
+    m = mh.hemisphere(1.0, ntheta=46, nphi=46).mesh
+    assert m.nfaces == 1058
+    om = [0.4, 1.5, 3.0]
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    ref = mh.freq_sweep(m, om, dofs, [0.0], direct=direct, precision="fp64")
+    try:
+        mh._lib.set_refinement(max_sweeps=2, fp64_fallback=False)
+        got = mh.freq_sweep(m, om, dofs, [0.0], direct=direct, precision="mixed")
+    finally:
+        mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
+    for g, r, name in zip(got, ref, ("A", "B", "F")):
+        assert relmax(g, r) < 1e-8, name
+
