@@ -899,23 +899,7 @@ __global__ __launch_bounds__(512) void k_split(const float2* __restrict__ A, int
     if (c == 0 && q == 0) xexp[item * xexp_stride + x] = e;
 }
 
-// XCD-aware tile order: workgroups are dealt to the 8 XCDs in turn, so
-// workgroup w runs on XCD w % 8; XCD x takes the contiguous range of tiles
-// [x per, (x + 1) per) of the (item, column tile, row tile) order, which
-// keeps each item's operand tiles in one XCD's L2.
-constexpr int kXcd = 8;
-__device__ __forceinline__ bool xcd_tile(int64_t ntr, int64_t ntc, int64_t ntiles, int64_t& item, int64_t& ti,
-                                         int64_t& tj) {
-    const int64_t w = blockIdx.x;
-    const int64_t per = (ntiles + kXcd - 1) / kXcd;
-    const int64_t t = (w % kXcd) * per + w / kXcd;
-    if (t >= ntiles) return false;
-    item = t / (ntr * ntc);
-    const int64_t rem = t % (ntr * ntc);
-    tj = rem / ntr;
-    ti = rem % ntr;
-    return true;
-}
+constexpr int kXcd = 8;  // XCDs: workgroup w of a launch runs on XCD w % 8
 
 // LDS slot of unit q of tile row (or column) x: the four units of a row are
 // permuted by the XOR key (x >> 2) & 3.  Rows x, x + 4, x + 8, x + 12 share
@@ -989,62 +973,122 @@ __device__ __forceinline__ void h_barrier_ready(bool next_in_flight) {
 }
 __device__ __forceinline__ void h_barrier_done() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// The depth loop (kNBOuter = 8 chunks of 16) over two LDS stages, chunk c
-// in stage c mod 2.  The stages are distinct restrict parameters, so the
-// compiler's own wait before reading one stage does not include the loads
-// in flight into the other.
-__device__ __forceinline__ void h_loop(__amdgpu_buffer_rsrc_t rl, __amdgpu_buffer_rsrc_t ru, int64_t npad,
-                                       int64_t ncpad, u4v* __restrict__ sL0, u4v* __restrict__ sU0,
-                                       u4v* __restrict__ sL1, u4v* __restrict__ sU1, uint32_t vo, int wave, int l,
-                                       int wr, int wc, f4 (&accr)[4][2], f4 (&acci)[4][2]) {
-    h_stage(rl, ru, npad, ncpad, 0, sL0, sU0, vo, wave);
-    h_stage(rl, ru, npad, ncpad, 1, sL1, sU1, vo, wave);
+// Tiles of a launch: (item, column tile, row tile) order, the range
+// [x per, (x + 1) per) to XCD x (workgroup w runs on XCD w % 8), which keeps
+// each item's operand tiles in one XCD's L2.
+__device__ __forceinline__ void h_coords(int t, int ntr, int ntc, int& item, int& ti, int& tj) {
+    item = t / (ntr * ntc);
+    const int rem = t - item * (ntr * ntc);
+    tj = rem / ntr;
+    ti = rem - tj * ntr;
+}
+
+// Arguments of the split-image update; the host checks that every index
+// and byte offset within an item fits 31 bits.
+// Per item: kHUnits npad (ncpad) image units and npad (ncpad) exponents.
+struct HArgs {
+    float2* A;
+    int64_t stride;
+    int ld, r_lo, r_hi, c_lo, c_hi;
+    const u4v* Limg;
+    const int* rexp;
+    int npad;
+    const u4v* Uimg;
+    const int* cexp;
+    int ncpad;
+    int ntr, ntc, ntiles;
+};
+
+// C -= 2^-(er + ec) acc for one tile, one block of 16 rows (a) at a time,
+// through buffer loads and stores with 32-bit offsets in the item's
+// matrix; an entry outside [r_lo, r_hi) x [c_lo, c_hi) gets the offset
+// 2^31, beyond the buffer's range (below 2^31 bytes): the hardware's range
+// check returns zero for its load and drops its store, with no branch.  Block
+// a + 1's loads are issued before block a's stores, so waiting for them
+// does not wait for stores.
+// C is read and written once per update: non-temporal accesses (cache
+// policy bit NT), so its stream does not evict the operand images, which
+// every tile of a row (column) reads again, from the XCD's L2
+constexpr int kHNt = 2;
+__device__ __forceinline__ void h_epilogue(const HArgs& h, int item, int tr, int tc, int wr, int wc,
+                                           const f4 (&accr)[4][2], const f4 (&acci)[4][2]) {
+    // the lane index computed here, opaque to the compiler (which would
+    // otherwise keep the thread index live across the depth loop for it and
+    // spill it)
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    float2* M = h.A + item * h.stride;
+    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc(
+        M, (short)0, (int)(((uint32_t)(h.c_hi - 1) * (uint32_t)h.ld + (uint32_t)h.r_hi) * 8u), 0x00020000);
+    const int gr0 = tr + 64 * wr + (l & 15), gc0 = tc + 32 * wc + 4 * (l >> 4);
+    const int* ce = h.cexp + (int64_t)item * h.ncpad;
+    int ec[2][4];
 #pragma unroll
-    for (int c = 0; c < kHChunks; c += 2) {
-        h_barrier_ready(true);
-        h_chunk(sL0, sU0, l, wr, wc, accr, acci);
-        if (c + 2 < kHChunks) {
-            h_barrier_done();  // every wave is done with this stage
-            h_stage(rl, ru, npad, ncpad, c + 2, sL0, sU0, vo, wave);
-        }
-        h_barrier_ready(c + 2 < kHChunks);
-        h_chunk(sL1, sU1, l, wr, wc, accr, acci);
-        if (c + 3 < kHChunks) {
-            h_barrier_done();
-            h_stage(rl, ru, npad, ncpad, c + 3, sL1, sU1, vo, wave);
-        }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ec[b][r] = ce[gc0 + 16 * b + r];
+    auto off = [&](int a, int b, int r) -> uint32_t {
+        const int gr = gr0 + 16 * a, gc = gc0 + 16 * b + r;
+        return (gr < h.r_hi && gc < h.c_hi) ? ((uint32_t)gr + (uint32_t)gc * (uint32_t)h.ld) * 8u : 0x80000000u;
+    };
+    float2 cv[2][2][4];
+    auto load = [&](int a, float2 (&v)[2][4]) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[b][r] = __builtin_bit_cast(float2,
+                                             __builtin_amdgcn_raw_buffer_load_b64(rc, (int)off(a, b, r), 0, kHNt));
+    };
+    load(0, cv[0]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        if (a + 1 < 4) load(a + 1, cv[(a + 1) & 1]);
+        const int er = h.rexp[(int64_t)item * h.npad + (gr0 + 16 * a)];
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int sh = -(er + ec[b][r]);
+                const float2 v = make_float2(cv[a & 1][b][r].x - __builtin_amdgcn_ldexpf(accr[a][b][r], sh),
+                                             cv[a & 1][b][r].y - __builtin_amdgcn_ldexpf(acci[a][b][r], sh));
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), rc, (int)off(a, b, r), 0, kHNt);
+            }
     }
 }
 
-// 128 x 128 tile per 512-thread workgroup (8 waves, each 64 x 32 = 4 x 2
-// blocks), operand chunks double-buffered in 64 KB of LDS and at most 128
-// VGPRs (two workgroups per CU), with the loads of chunk c + 1 in flight
-// while chunk c is multiplied.  Depth kNBOuter (the width of every block
-// column that has a trailing update).
-__global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(float2* __restrict__ A, int64_t ld, int64_t stride,
-                                                       int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
-                                                       const u4v* __restrict__ Limg, int64_t l_stride,
-                                                       int64_t npad, const int* __restrict__ rexp,
-                                                       int64_t r_stride, const u4v* __restrict__ Uimg,
-                                                       int64_t u_stride, int64_t ncpad,
-                                                       const int* __restrict__ cexp, int64_t c_stride,
-                                                       int64_t ntr, int64_t ntc, int64_t ntiles) {
-    int64_t item, ti, tj;
-    if (!xcd_tile(ntr, ntc, ntiles, item, ti, tj)) return;
-    float2* M = A + item * stride;
-    const int64_t tr = r_lo + ti * kBT;
-    const int64_t tc = c_lo + tj * kBT;
-    __shared__ u4v sL0[1024], sU0[1024], sL1[1024], sU1[1024];
+__device__ __forceinline__ void h_rsrc(const HArgs& h, int item, int tr, int tc, __amdgpu_buffer_rsrc_t& rl,
+                                       __amdgpu_buffer_rsrc_t& ru) {
+    // buffer ranges end with the item's image (reads beyond return zeros)
+    const int64_t ls = (int64_t)kHUnits * h.npad, us = (int64_t)kHUnits * h.ncpad;
+    rl = __builtin_amdgcn_make_buffer_rsrc(const_cast<u4v*>(h.Limg + item * ls + 4 * tr), (short)0,
+                                           (int)((ls - 4 * tr) * 16), 0x00020000);
+    ru = __builtin_amdgcn_make_buffer_rsrc(const_cast<u4v*>(h.Uimg + item * us + 4 * tc), (short)0,
+                                           (int)((us - 4 * tc) * 16), 0x00020000);
+}
+
+// One tile per workgroup: tile x per + w / 8 of XCD x = w % 8.  The depth
+// loop (kNBOuter = 8 chunks of 16) runs over two LDS stages, chunk c in
+// stage c mod 2.  The stages are distinct restrict parameters, so the
+// compiler's own wait before reading one stage does not include the loads
+// in flight into the other.
+__device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sL0, u4v* __restrict__ sU0,
+                                      u4v* __restrict__ sL1, u4v* __restrict__ sU1) {
+    const int per = (h.ntiles + kXcd - 1) / kXcd, x8 = blockIdx.x % kXcd;
+    const int t = x8 * per + blockIdx.x / kXcd;
+    if (t >= h.ntiles || t >= (x8 + 1) * per) return;
     const int tid = threadIdx.x, l = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave & 1, wc = wave >> 1;  // 2 x 4 waves: 64 rows x 32 columns each
-    // this lane's staging unit: row (column) x of the tile, image unit q;
-    // buffer ranges end with the item's image (reads beyond return zeros)
+    // this lane's staging unit: row (column) x of the tile, image unit q
     const int x = tid >> 2, q = (tid & 3) ^ hkey(x);
     const uint32_t vo = (uint32_t)(4 * x + q) * 16u;
-    const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<u4v*>(Limg + item * l_stride + tr * 4), (short)0, (int)((l_stride - tr * 4) * 16), 0x00020000);
-    const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<u4v*>(Uimg + item * u_stride + tc * 4), (short)0, (int)((u_stride - tc * 4) * 16), 0x00020000);
+    int item, ti, tj;
+    h_coords(t, h.ntr, h.ntc, item, ti, tj);
+    const int tr = h.r_lo + ti * kBT, tc = h.c_lo + tj * kBT;
+    __amdgpu_buffer_rsrc_t rl, ru;
+    h_rsrc(h, item, tr, tc, rl, ru);
+    h_stage(rl, ru, h.npad, h.ncpad, 0, sL0, sU0, vo, wave);
+    h_stage(rl, ru, h.npad, h.ncpad, 1, sL1, sU1, vo, wave);
     f4 accr[4][2], acci[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -1053,49 +1097,32 @@ __global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(float2* __restrict__ A, i
             accr[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
             acci[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
         }
-    h_loop(rl, ru, npad, ncpad, sL0, sU0, sL1, sU1, vo, wave, l, wr, wc, accr, acci);
-    // C -= 2^-(er + ec) acc, one block of 16 rows (a) at a time, through
-    // buffer loads and stores with 32-bit offsets in the item's matrix; an
-    // entry outside [r_lo, r_hi) x [c_lo, c_hi) gets an offset beyond the
-    // buffer's range (its load reads zero) and is not stored.  Block
-    // a + 1's loads are issued before block a's stores, so waiting for them
-    // does not wait for stores.
-    const __amdgpu_buffer_rsrc_t rc =
-        __builtin_amdgcn_make_buffer_rsrc(M, (short)0, (int)(((c_hi - 1) * ld + r_hi) * 8), 0x00020000);
-    const int64_t gr0 = tr + 64 * wr + (l & 15), gc0 = tc + 32 * wc + 4 * (l >> 4);
-    int ec[2][4];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) ec[b][r] = cexp[item * c_stride + gc0 + 16 * b + r];
-    auto off = [&](int a, int b, int r) -> uint32_t {
-        const int64_t gr = gr0 + 16 * a, gc = gc0 + 16 * b + r;
-        return (gr < r_hi && gc < c_hi) ? (uint32_t)((gr + gc * ld) * 8) : 0x80000000u;
-    };
-    float2 cv[2][2][4];
-    auto load = [&](int a, float2 (&v)[2][4]) {
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                v[b][r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rc, (int)off(a, b, r), 0, 0));
-    };
-    load(0, cv[0]);
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        if (a + 1 < 4) load(a + 1, cv[(a + 1) & 1]);
-        const int er = rexp[item * r_stride + gr0 + 16 * a];
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int sh = -(er + ec[b][r]);
-                const float2 v = make_float2(cv[a & 1][b][r].x - __builtin_amdgcn_ldexpf(accr[a][b][r], sh),
-                                             cv[a & 1][b][r].y - __builtin_amdgcn_ldexpf(acci[a][b][r], sh));
-                const uint32_t o = off(a, b, r);
-                if (o != 0x80000000u) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), rc, (int)o, 0, 0);
-            }
+    for (int c = 0; c < kHChunks; c += 2) {
+        h_barrier_ready(true);
+        h_chunk(sL0, sU0, l, wr, wc, accr, acci);
+        if (c + 2 < kHChunks) {
+            h_barrier_done();  // every wave is done with this stage
+            h_stage(rl, ru, h.npad, h.ncpad, c + 2, sL0, sU0, vo, wave);
+        }
+        h_barrier_ready(c + 2 < kHChunks);
+        h_chunk(sL1, sU1, l, wr, wc, accr, acci);
+        if (c + 3 < kHChunks) {
+            h_barrier_done();
+            h_stage(rl, ru, h.npad, h.ncpad, c + 3, sL1, sU1, vo, wave);
+        }
     }
+    h_epilogue(h, item, tr, tc, wr, wc, accr, acci);
+}
+
+// 128 x 128 tile per 512-thread workgroup (8 waves, each 64 x 32 = 4 x 2
+// blocks), operand chunks double-buffered in 64 KB of LDS and at most 128
+// VGPRs (two workgroups per CU), with the loads of chunk c + 1 in flight
+// while chunk c is multiplied.  Depth kNBOuter (the width of every block
+// column that has a trailing update).
+__global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(HArgs h) {
+    __shared__ u4v sL0[1024], sU0[1024], sL1[1024], sU1[1024];
+    h_run(h, sL0, sU0, sL1, sU1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1218,11 +1245,14 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                          2.0 * sizeof(T) * (2.0 * mr * mc) * P.batch + 8.0 * (double)kw * (mr + mc) * P.batch);
             const int64_t ntr = grid.x, ntc = grid.y, ntiles = ntr * ntc * P.batch;
             const int64_t nwg = (ntiles + kXcd - 1) / kXcd * kXcd;
-            MH_REQUIRE(nwg <= 0x7fffffff, "lu: too many tiles for one launch");
-            hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, P.A, P.ld,
-                               P.stride, r_lo, r_hi, c_lo, c_hi, H->L[sl],
-                               (int64_t)kHUnits * H->npad, H->npad, H->rexp[sl], H->npad, H->U,
-                               (int64_t)kHUnits * H->ncpad, H->ncpad, H->cexp, H->ncpad, ntr, ntc, ntiles);
+            MH_REQUIRE(ntiles <= 0x7fffffff && (int64_t)kHUnits * H->npad * 16 <= 0x7fffffff &&
+                           (int64_t)kHUnits * H->ncpad * 16 <= 0x7fffffff,
+                       "lu: too many tiles for one launch");
+            const HArgs ha{P.A,       P.stride,    (int)P.ld,   (int)r_lo,       (int)r_hi,
+                           (int)c_lo, (int)c_hi,   H->L[sl],    H->rexp[sl],     (int)H->npad,
+                           H->U,      H->cexp,     (int)H->ncpad, (int)ntr,      (int)ntc,
+                           (int)ntiles};
+            hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, ha);
         } else {
             ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
                          2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
