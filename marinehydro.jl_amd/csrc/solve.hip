@@ -963,15 +963,20 @@ __device__ __forceinline__ void h_chunk(const u4v* sl, const u4v* su, int l, int
 // into the other stage included).  The first waits for this wave's loads of
 // the stage about to be read (all but the four of the next chunk, or all),
 // the second for its LDS reads, which the next loads into the same stage
-// must not overtake; the memory clobber keeps the compiler from moving LDS
-// accesses across either.
+// must not overtake.  The waits are compiler builtins, so the compiler's
+// own wait insertion knows about them; the memory clobber keeps it from
+// moving LDS accesses across the barrier.
 __device__ __forceinline__ void h_barrier_ready(bool next_in_flight) {
     if (next_in_flight)
-        asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
     else
-        asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    asm volatile("s_barrier" ::: "memory");
 }
-__device__ __forceinline__ void h_barrier_done() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+__device__ __forceinline__ void h_barrier_done() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    asm volatile("s_barrier" ::: "memory");
+}
 
 // Tiles of a launch: (item, column tile, row tile) order, the range
 // [x per, (x + 1) per) to XCD x (workgroup w runs on XCD w % 8), which keeps
@@ -1010,23 +1015,79 @@ struct HArgs {
 // policy bit NT), so its stream does not evict the operand images, which
 // every tile of a row (column) reads again, from the XCD's L2
 constexpr int kHNt = 2;
+// Copies rows [0, 16) x columns [0, 32) of this wave's C block (rows tr +
+// 64 wr, columns tc + 32 wc) into 4 KB of LDS (column x at 128 x bytes, 16
+// rows of 8 bytes), four 16-byte buffer loads per lane straight to LDS;
+// columns beyond c_hi read zeros.  Every row of the tile is inside (so no
+// 16-byte load straddles the end of the matrix).
+__device__ __forceinline__ void h_cstage(const HArgs& h, __amdgpu_buffer_rsrc_t rc, int tr, int tc, int wr, int wc,
+                                         int lane, u4v* region) {
+    const int gr = tr + 64 * wr + 2 * (lane & 7);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gc = tc + 32 * wc + 8 * i + (lane >> 3);
+        const uint32_t vo = gc < h.c_hi ? ((uint32_t)gr + (uint32_t)gc * (uint32_t)h.ld) * 8u : 0x80000000u;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rc, (lptr_t)(region + 64 * i), 16, vo, 0, 0, kHNt);
+    }
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t h_crsrc(const HArgs& h, int item) {
+    return __builtin_amdgcn_make_buffer_rsrc(h.A + item * h.stride, (short)0,
+                                             (int)(((uint32_t)(h.c_hi - 1) * (uint32_t)h.ld + (uint32_t)h.r_hi) * 8u),
+                                             0x00020000);
+}
+
+// PRE: this wave's C block 0 is in LDS at c0 (h_cstage); else loaded here.
+// acc *= 2^-(er + ec) for this lane's entries (rows 16 a + l % 16, columns
+// 16 b + 4 (l / 16) + r of the wave's block), once the operand fragments are
+// dead, so the epilogue holds no exponents.
+// The tile's exponents (rows, then columns) were copied to sE by h_estage.
+__device__ __forceinline__ void h_unscale(const int* sE, int wr, int wc, int l, f4 (&accr)[4][2],
+                                          f4 (&acci)[4][2]) {
+    int er[4], ec[2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) er[a] = sE[64 * wr + 16 * a + (l & 15)];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ec[b][r] = sE[kBT + 32 * wc + 16 * b + 4 * (l >> 4) + r];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int sh = -(er[a] + ec[b][r]);
+                accr[a][b][r] = __builtin_amdgcn_ldexpf(accr[a][b][r], sh);
+                acci[a][b][r] = __builtin_amdgcn_ldexpf(acci[a][b][r], sh);
+            }
+}
+
+// Copies the exponents of the tile's 128 rows and 128 columns into sE (a
+// 4-byte buffer load per lane straight to LDS: two by wave 0, two by wave 1),
+// issued before the first operand chunk, so they have landed long before
+// the depth loop ends; the loop's barriers make them visible to every wave.
+__device__ __forceinline__ void h_estage(const HArgs& h, int item, int tr, int tc, int wave, int lane, int* sE) {
+    if (wave > 1) return;
+    const int* e = wave == 0 ? h.rexp + (int64_t)item * h.npad + tr : h.cexp + (int64_t)item * h.ncpad + tc;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int*>(e), (short)0, 4 * kBT, 0x00020000);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lptr_t)(sE + kBT * wave + 64 * i), 4, (uint32_t)(64 * i + lane) * 4u,
+                                                 0, 0, 0);
+}
+
+// The accumulators are already scaled back (h_unscale).
+template <bool PRE>
 __device__ __forceinline__ void h_epilogue(const HArgs& h, int item, int tr, int tc, int wr, int wc,
-                                           const f4 (&accr)[4][2], const f4 (&acci)[4][2]) {
+                                           const f4 (&accr)[4][2], const f4 (&acci)[4][2], const float2* c0) {
     // the lane index computed here, opaque to the compiler (which would
     // otherwise keep the thread index live across the depth loop for it and
     // spill it)
     int l;
     asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
-    float2* M = h.A + item * h.stride;
-    const __amdgpu_buffer_rsrc_t rc = __builtin_amdgcn_make_buffer_rsrc(
-        M, (short)0, (int)(((uint32_t)(h.c_hi - 1) * (uint32_t)h.ld + (uint32_t)h.r_hi) * 8u), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rc = h_crsrc(h, item);
     const int gr0 = tr + 64 * wr + (l & 15), gc0 = tc + 32 * wc + 4 * (l >> 4);
-    const int* ce = h.cexp + (int64_t)item * h.ncpad;
-    int ec[2][4];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) ec[b][r] = ce[gc0 + 16 * b + r];
     auto off = [&](int a, int b, int r) -> uint32_t {
         const int gr = gr0 + 16 * a, gc = gc0 + 16 * b + r;
         return (gr < h.r_hi && gc < h.c_hi) ? ((uint32_t)gr + (uint32_t)gc * (uint32_t)h.ld) * 8u : 0x80000000u;
@@ -1040,18 +1101,22 @@ __device__ __forceinline__ void h_epilogue(const HArgs& h, int item, int tr, int
                 v[b][r] = __builtin_bit_cast(float2,
                                              __builtin_amdgcn_raw_buffer_load_b64(rc, (int)off(a, b, r), 0, kHNt));
     };
-    load(0, cv[0]);
+    if constexpr (PRE) {  // read before any further load is issued, so the wait covers the copy only
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cv[0][b][r] = c0[(16 * b + 4 * (l >> 4) + r) * 16 + (l & 15)];
+    } else {
+        load(0, cv[0]);
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         if (a + 1 < 4) load(a + 1, cv[(a + 1) & 1]);
-        const int er = h.rexp[(int64_t)item * h.npad + (gr0 + 16 * a)];
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int sh = -(er + ec[b][r]);
-                const float2 v = make_float2(cv[a & 1][b][r].x - __builtin_amdgcn_ldexpf(accr[a][b][r], sh),
-                                             cv[a & 1][b][r].y - __builtin_amdgcn_ldexpf(acci[a][b][r], sh));
+                const float2 v = make_float2(cv[a & 1][b][r].x - accr[a][b][r], cv[a & 1][b][r].y - acci[a][b][r]);
                 __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), rc, (int)off(a, b, r), 0, kHNt);
             }
     }
@@ -1072,8 +1137,13 @@ __device__ __forceinline__ void h_rsrc(const HArgs& h, int item, int tr, int tc,
 // stage c mod 2.  The stages are distinct restrict parameters, so the
 // compiler's own wait before reading one stage does not include the loads
 // in flight into the other.
-__device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sL0, u4v* __restrict__ sU0,
-                                      u4v* __restrict__ sL1, u4v* __restrict__ sU1) {
+__device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sS0, u4v* __restrict__ sS1,
+                                      int* __restrict__ sE) {
+    // stage s: L21 units [0, 1024), U12 units [1024, 2048)
+    u4v* const sL0 = sS0;
+    u4v* const sU0 = sS0 + 1024;
+    u4v* const sL1 = sS1;
+    u4v* const sU1 = sS1 + 1024;
     const int per = (h.ntiles + kXcd - 1) / kXcd, x8 = blockIdx.x % kXcd;
     const int t = x8 * per + blockIdx.x / kXcd;
     if (t >= h.ntiles || t >= (x8 + 1) * per) return;
@@ -1087,8 +1157,13 @@ __device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sL0, u4v
     const int tr = h.r_lo + ti * kBT, tc = h.c_lo + tj * kBT;
     __amdgpu_buffer_rsrc_t rl, ru;
     h_rsrc(h, item, tr, tc, rl, ru);
+    h_estage(h, item, tr, tc, wave, l, sE);
     h_stage(rl, ru, h.npad, h.ncpad, 0, sL0, sU0, vo, wave);
     h_stage(rl, ru, h.npad, h.ncpad, 1, sL1, sU1, vo, wave);
+    // every row of the tile inside: C block 0 is copied to LDS during the
+    // last chunk (4 KB per wave in stage 0)
+    const bool interior = tr + kBT <= h.r_hi;
+    u4v* creg = sS0 + 256 * wave;
     f4 accr[4][2], acci[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -1106,13 +1181,20 @@ __device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sL0, u4v
             h_stage(rl, ru, h.npad, h.ncpad, c + 2, sL0, sU0, vo, wave);
         }
         h_barrier_ready(c + 2 < kHChunks);
+        // last chunk: stage 0 is free (every wave has passed the barrier
+        // after its last reads of it); each wave copies its C block 0 there
+        if (c + 2 >= kHChunks && interior) h_cstage(h, h_crsrc(h, item), tr, tc, wr, wc, l, creg);
         h_chunk(sL1, sU1, l, wr, wc, accr, acci);
         if (c + 3 < kHChunks) {
             h_barrier_done();
             h_stage(rl, ru, h.npad, h.ncpad, c + 3, sL1, sU1, vo, wave);
         }
     }
-    h_epilogue(h, item, tr, tc, wr, wc, accr, acci);
+    h_unscale(sE, wr, wc, l, accr, acci);
+    if (interior)
+        h_epilogue<true>(h, item, tr, tc, wr, wc, accr, acci, reinterpret_cast<const float2*>(creg));
+    else
+        h_epilogue<false>(h, item, tr, tc, wr, wc, accr, acci, nullptr);
 }
 
 // 128 x 128 tile per 512-thread workgroup (8 waves, each 64 x 32 = 4 x 2
@@ -1121,8 +1203,9 @@ __device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sL0, u4v
 // while chunk c is multiplied.  Depth kNBOuter (the width of every block
 // column that has a trailing update).
 __global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(HArgs h) {
-    __shared__ u4v sL0[1024], sU0[1024], sL1[1024], sU1[1024];
-    h_run(h, sL0, sU0, sL1, sU1);
+    __shared__ u4v sS0[2048], sS1[2048];
+    __shared__ int sE[2 * kBT];
+    h_run(h, sS0, sS1, sE);
 }
 
 // ---------------------------------------------------------------------------
