@@ -365,6 +365,28 @@ int mh_multi_autotune(mh_context_t ctx, int64_t nitems, const mh_sweep_item_t* i
                       int precision, double g, double rho, int* best_workers,
                       int* best_leaf, double* best_ms);
 
+/* ---- captured sweeps ----------------------------------------------------
+ * No reference counterpart (the reference re-runs its loop, src/waves.jl:
+ * 29-52, on every call): the FP64 forward sweep of mh_freq_sweep (precision
+ * MH_FP64) for one body, recorded once as a HIP graph on a private context
+ * of ctx's device and replayed with new frequencies and headings, for
+ * design loops and serving that run one body (or one mesh shape) many
+ * times.  The mesh arrays, dof_fields and the outputs A, B, F are the device
+ * arrays given to mh_sweep_graph_create: their contents may change between
+ * launches (a body reshaped in place), their addresses and sizes may not,
+ * and they must outlive the graph.  Creation runs the sweep once with
+ * placeholder frequencies (1 rad/s; the outputs are overwritten by every
+ * launch).  Frequencies in one chunk only (matrices under 32 GiB).
+ * mh_sweep_graph_launch is synchronous, starts after the work already
+ * issued on ctx's stream, and reports mesh-index, pivot and solve errors as
+ * mh_freq_sweep does; launches of one graph must not overlap. */
+typedef struct mh_sweep_graph* mh_sweep_graph_t;
+int mh_sweep_graph_create(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int direct, int64_t nomega,
+                          int64_t ndof, const double* dof_fields, int64_t nbeta, double g, double rho, double* A,
+                          double* B, double* F, mh_sweep_graph_t* out);
+int mh_sweep_graph_launch(mh_context_t ctx, mh_sweep_graph_t graph, const double* omegas, const double* betas);
+int mh_sweep_graph_destroy(mh_sweep_graph_t graph);
+
 /* ---- host-array entry points (one call, synchronous) -------------------
  * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
  * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.

@@ -282,8 +282,17 @@ __global__ __launch_bounds__(256) void k_assemble_sym(
 }
 
 __global__ void k_clear_int(int* __restrict__ p, int count) {
-    const int t = (int)threadIdx.x;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (t < count) p[t] = 0;
+}
+
+int clear_ints(Context* ctx, int* p, int64_t count) {
+    if (count <= 0) return MH_OK;
+    MH_REQUIRE(count <= 0x7fffffff, "clear_ints: count too large");
+    hipLaunchKernelGGL(k_clear_int, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                       (int)count);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
 }
 
 int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out, int* flag_buf) {
@@ -298,6 +307,7 @@ int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out, int* fla
                        nf, m->nvertices, m->vertices, m->faces, m->centers, m->normals, m->areas,
                        m->radii, out, d_flag);
     MH_HIP_CHECK(hipGetLastError());
+    if (ctx->capturing) return MH_OK;  // the flag is read after the graph's launch
     int flag = 0;
     MH_HIP_CHECK(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP_CHECK(hipStreamSynchronize(ctx->stream));

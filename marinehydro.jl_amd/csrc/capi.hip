@@ -204,6 +204,24 @@ void context_free(Context* c) {
     if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
+int check_info(Context* c, const int* info, int64_t batch) {
+    std::vector<int> h(batch);
+    MH_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
+    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t b = 0; b < batch; ++b) {
+        if (h[b] & 1) {
+            set_error("solve: exact zero pivot in batch item " + std::to_string(b));
+            return MH_ERR_SINGULAR;
+        }
+        if (h[b] & 2) {
+            set_error("mixed precision: non-finite entries in the FP32 factorisation of batch item " +
+                      std::to_string(b));
+            return MH_ERR_NOT_CONVERGED;
+        }
+    }
+    return MH_OK;
+}
+
 }  // namespace mh
 
 extern "C" {
@@ -393,23 +411,6 @@ static int solve_impl(Context* c, int64_t n, int64_t batch, int direct, double2*
     return MH_OK;
 }
 
-static int check_info(Context* c, const int* info, int64_t batch) {
-    std::vector<int> h(batch);
-    MH_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost, c->stream));
-    MH_HIP_CHECK(hipStreamSynchronize(c->stream));
-    for (int64_t b = 0; b < batch; ++b) {
-        if (h[b] & 1) {
-            set_error("solve: exact zero pivot in batch item " + std::to_string(b));
-            return MH_ERR_SINGULAR;
-        }
-        if (h[b] & 2) {
-            set_error("mixed precision: non-finite entries in the FP32 factorisation of batch item " +
-                      std::to_string(b));
-            return MH_ERR_NOT_CONVERGED;
-        }
-    }
-    return MH_OK;
-}
 
 int mh_solve(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK, const double* S,
              int64_t ld, int64_t stride, const double* bc, int64_t nrhs, double* phi, int64_t ldb,
@@ -748,11 +749,37 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     PanelRec* P = reinterpret_cast<PanelRec*>(ws + o_P);
     std::vector<double> hk(nomega);
     for (int64_t w = 0; w < nomega; ++w) hk[w] = omegas[w] * omegas[w] / g;  // waves.jl:30
-    MH_HIP_CHECK(hipMemcpyAsync(d_om, omegas, sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
-    MH_HIP_CHECK(hipMemcpyAsync(d_k, hk.data(), sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
-    if (nbeta)
-        MH_HIP_CHECK(hipMemcpyAsync(d_be, betas, sizeof(double) * nbeta, hipMemcpyHostToDevice, c->stream));
-    TRY(launch_pack_panels(c, mesh, P));
+    // graph capture (FP64 forward sweep in one chunk): the frequencies,
+    // wavenumbers and headings are copied in before each launch of the
+    // graph; everything from the panel packing on is recorded
+    struct CaptureEnd {
+        Context* c;
+        bool on = false;
+        ~CaptureEnd() {
+            if (on) {  // an error inside the captured region: end it, drop the graph
+                hipGraph_t gr = nullptr;
+                (void)hipStreamEndCapture(c->stream, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+            }
+        }
+    } cap_end{c};
+    if (c->capturing) {
+        MH_REQUIRE(!ga && !mixed && chunk >= nomega,
+                   "sweep graph: FP64 forward sweeps whose matrices fit one chunk only");
+        c->cap_om = d_om;
+        c->cap_k = d_k;
+        c->cap_be = d_be;
+        c->cap_info = info;
+        MH_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        cap_end.on = true;
+        TRY(launch_pack_panels(c, mesh, P, c->d_flag));
+    } else {
+        MH_HIP_CHECK(hipMemcpyAsync(d_om, omegas, sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
+        MH_HIP_CHECK(hipMemcpyAsync(d_k, hk.data(), sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
+        if (nbeta)
+            MH_HIP_CHECK(hipMemcpyAsync(d_be, betas, sizeof(double) * nbeta, hipMemcpyHostToDevice, c->stream));
+        TRY(launch_pack_panels(c, mesh, P));
+    }
     const int form = direct ? MH_DIRECT : MH_INDIRECT;
     double2* aug = dD + n * ld;  // first right-hand-side column of item 0
     for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
@@ -848,7 +875,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                                               sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
                                               c->stream));
             }
-            MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
+            TRY(clear_ints(c, info, nw));  // a fill kernel: capturable
             // the transposed FP64 factors only when the workspace was laid out
             // for the FP64 path (a fallback solves the adjoint from the factors)
             double2* dDt = (ga && ndof && !mixed) ? reinterpret_cast<double2*>(ws + o_Dt) : nullptr;
@@ -885,6 +912,11 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
         // at the end of the chunk, after the gradient kernels: one host
         // synchronisation per chunk
         if (!ga || ndof == 0) {
+            if (c->capturing) {  // checked after each launch of the graph
+                cap_end.on = false;
+                MH_HIP_CHECK(hipStreamEndCapture(c->stream, &c->cap_graph));
+                continue;
+            }
             TRY(tri_check(c));
             TRY(check_info(c, info, nw));
             continue;

@@ -69,6 +69,14 @@ struct Context {
     int64_t fallbacks = 0;
     // LU leaf width cap (0: the default width for the panel height; 4 or 8)
     int leaf_cap = 0;
+    // graph capture of an FP64 forward sweep (graph.hip): while set,
+    // sweep_impl records its device work on `stream` into cap_graph (the
+    // host-array copies, the memset and the host checks are left out and
+    // the pointers they need are kept here for the launches)
+    bool capturing = false;
+    hipGraph_t cap_graph = nullptr;
+    double *cap_om = nullptr, *cap_k = nullptr, *cap_be = nullptr;
+    int* cap_info = nullptr;
     // multi-body scheduler: worker contexts (each with a stream of its own)
     // and the configuration the autotuner chose (0: not tuned)
     bool owns_stream = false;
@@ -152,6 +160,8 @@ int ensure_events(Context* ctx, size_t count);
 // in flag_buf (one int, cleared by a fill kernel first) or, when flag_buf is
 // null, in ctx->d_flag, and returned as MH_ERR_ARGUMENT (synchronises).
 int launch_pack_panels(Context* ctx, const mh_mesh_t* mesh, PanelRec* out, int* flag_buf = nullptr);
+// p[0 .. count) = 0 by a fill kernel on ctx->stream (capturable, unlike a memset)
+int clear_ints(Context* ctx, int* p, int64_t count);
 // Assembly in one kernel per launch, without the frequency-independent
 // scratch matrices launch_assemble keeps in the context.
 int launch_assemble_onepass(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels, int gf_mask,
@@ -214,6 +224,9 @@ int tri_solve_f32(Context* ctx, int64_t n, int64_t batch, const float2* LU, int6
 // Reports (and clears) a failed inter-workgroup wait or a bad pivot index of
 // the solves issued so far; synchronises the context stream.
 int tri_check(Context* ctx);
+// Reads the LU info words of a batch (synchronises): zero pivots and
+// non-finite entries become MH_ERR_SINGULAR / MH_ERR_NOT_CONVERGED.
+int check_info(Context* c, const int* info, int64_t batch);
 
 // Y[b] = S[b]^T * X[b]
 int zgemm_tn_batched(Context* ctx, int64_t n, int64_t nrhs, int64_t batch, const double2* S,

@@ -11,7 +11,7 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   both_integral_and_integral_gradient, DeviceMesh, assemble_matrices,
                   assemble_matrices_batched, assemble_matrix_wu, solve, radiation_bc,
                   integrate_pressure, airy_waves_potential, airy_waves_velocity,
-                  airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep,
+                  airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep, SweepGraph,
                   calculate_radiation_forces, DiffractionForce, Tangents, shape_tangents,
                   freq_sweep_grad, froude_krylov, froude_krylov_grad,
                   assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve)

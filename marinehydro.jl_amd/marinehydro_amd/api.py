@@ -480,6 +480,62 @@ def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=
     return A_, B_, F_
 
 
+class SweepGraph:
+    """freq_sweep(precision="fp64") of one body recorded once as a HIP graph
+    and replayed by launch(omegas, betas) (mh_sweep_graph_*).  The mesh must
+    be a DeviceMesh: its arrays may be updated in place between launches
+    (same sizes), and the graph reads them at every launch.  The number of
+    frequencies and headings is fixed at creation."""
+
+    def __init__(self, mesh, nomega, dof_fields, nbeta=0, green_functions=None, direct=True, device=0,
+                 g=None, rho=None):
+        torch = _torch()
+        if not isinstance(mesh, DeviceMesh):
+            raise TypeError("SweepGraph: the mesh must be a DeviceMesh (its arrays are recorded)")
+        self.dm, self.nomega, self.nbeta = mesh, int(nomega), int(nbeta)
+        n = mesh.nfaces
+        gfs = green_functions if green_functions is not None else (Rankine(), RankineReflected(), GFWu())
+        fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+        self.ndof = len(fields)
+        dev = f"cuda:{mesh.device}"
+        self.dofs = torch.from_numpy(np.ascontiguousarray(
+            np.stack([f.T for f in fields]) if self.ndof else np.zeros((1, 3, n)))).to(dev)
+        nd = max(self.ndof, 1)
+        self.out = _Outputs(torch, dev, [("A", (self.nomega, nd, nd), False), ("B", (self.nomega, nd, nd), False),
+                                         ("F", (self.nomega, max(self.nbeta, 1), nd), True)])
+        self.handle = C.c_void_p()
+        L.check(L.lib().mh_sweep_graph_create(
+            L.context(mesh.device), C.byref(mesh.struct), gf_mask(gfs), int(bool(direct)), self.nomega, self.ndof,
+            self.dofs.data_ptr(), self.nbeta, float(g or SETTINGS.g), float(rho or SETTINGS.rho),
+            self.out.ptr("A"), self.out.ptr("B"), self.out.ptr("F"), C.byref(self.handle)), "sweep_graph_create")
+
+    def launch(self, omegas, betas=()):
+        """Returns numpy (A, B, F) laid out as freq_sweep's."""
+        om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
+        be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
+        if om.shape[0] != self.nomega or be.shape[0] != self.nbeta:
+            raise ValueError(f"SweepGraph.launch: {self.nomega} frequencies and {self.nbeta} headings expected")
+        L.check(L.lib().mh_sweep_graph_launch(L.context(self.dm.device), self.handle, om.ctypes.data,
+                                              be.ctypes.data if self.nbeta else None), "sweep_graph_launch")
+        self.out.host = None  # read this launch's outputs
+        nd = self.ndof
+        A_ = np.ascontiguousarray(self.out.get("A").transpose(0, 2, 1)[:, :nd, :nd])
+        B_ = np.ascontiguousarray(self.out.get("B").transpose(0, 2, 1)[:, :nd, :nd])
+        F_ = np.ascontiguousarray(self.out.get("F")[:, :self.nbeta, :nd])
+        return A_, B_, F_
+
+    def close(self):
+        if self.handle:
+            L.lib().mh_sweep_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def calculate_radiation_forces(mesh, dof, omega):
     """src/waves.jl:29-37: [A, B] for one dof 3-vector (Wu, direct)."""
     A, B, _ = freq_sweep(mesh, [omega], [dof])
