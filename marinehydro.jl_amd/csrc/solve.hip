@@ -167,6 +167,10 @@ __global__ __launch_bounds__(NT) void k_lu_leaf(int64_t n, typename Cx<T>::t* __
                                                 int64_t stride, int64_t k0, int w, int64_t* __restrict__ ipiv,
                                                 int* __restrict__ info, int64_t pc_lo, int64_t pc_hi, int64_t x_lo,
                                                 int64_t x_hi) {
+    // panel-chain kernel: its waves win issue arbitration against those of
+    // a trailing-update GEMM sharing the CU (the chain is the LU's critical
+    // path)
+    __builtin_amdgcn_s_setprio(3);
     typedef typename Cx<T>::t C2;
     static_assert(NT == 256 || NT == 512 || NT == 1024, "4, 8 or 16 waves");
     constexpr int NW = NT / 64;
@@ -517,6 +521,10 @@ __device__ __forceinline__ void tb_stage(T (&xr)[kTbSlots], T (&xi)[kTbSlots], t
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_trsm_blk(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
                                                      int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+    // panel-chain kernel: its waves win issue arbitration against those of
+    // a trailing-update GEMM sharing the CU (the chain is the LU's critical
+    // path)
+    __builtin_amdgcn_s_setprio(3);
     typedef typename Cx<T>::t C2;
     static_assert(kNBOuter == 128 && kTbStage == 32, "four staged blocks of 32 unknowns");
     C2* M = A + (int64_t)blockIdx.y * stride;
@@ -562,6 +570,10 @@ template <class T>
 __global__ __launch_bounds__(256) void k_lu_gemm(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
                                                  int64_t r_lo, int64_t r_hi, int64_t c_lo,
                                                  int64_t c_hi, int64_t k_lo, int kw) {
+    // panel-chain kernel: its waves win issue arbitration against those of
+    // a trailing-update GEMM sharing the CU (the chain is the LU's critical
+    // path)
+    __builtin_amdgcn_s_setprio(3);
     typedef typename Cx<T>::t C2;
     typedef typename Cx<T>::acc acc_t;
     C2* M = A + (int64_t)blockIdx.z * stride;
@@ -1002,6 +1014,7 @@ struct HArgs {
     const int* cexp;
     int ncpad;
     int ntr, ntc, ntiles;
+    int chain;  // 1: the narrow update on the panel chain (raised wave priority)
 };
 
 // C -= 2^-(er + ec) acc for one tile, one block of 16 rows (a) at a time,
@@ -1203,6 +1216,7 @@ __device__ __forceinline__ void h_run(const HArgs& h, u4v* __restrict__ sS0, u4v
 // while chunk c is multiplied.  Depth kNBOuter (the width of every block
 // column that has a trailing update).
 __global__ __launch_bounds__(512, 4) void k_lu_gemm_hs(HArgs h) {
+    if (h.chain) __builtin_amdgcn_s_setprio(3);  // as the other panel-chain kernels
     __shared__ u4v sS0[2048], sS1[2048];
     __shared__ int sE[2 * kBT];
     h_run(h, sS0, sS1, sE);
@@ -1236,6 +1250,7 @@ struct LuPlan {
     int* info;
     int64_t pc_lo, pc_hi;  // block column being factored
     HImg* img = nullptr;   // FP32 only
+    bool chain = false;    // updates issued on the panel chain's stream
 };
 
 // Bytes of the image buffer for `batch` matrices of n rows and ncols columns.
@@ -1334,7 +1349,7 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
             const HArgs ha{P.A,       P.stride,    (int)P.ld,   (int)r_lo,       (int)r_hi,
                            (int)c_lo, (int)c_hi,   H->L[sl],    H->rexp[sl],     (int)H->npad,
                            H->U,      H->cexp,     (int)H->ncpad, (int)ntr,      (int)ntc,
-                           (int)ntiles};
+                           (int)ntiles, P.chain ? 1 : 0};
             hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, ha);
         } else {
             ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
@@ -1511,7 +1526,9 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
             if (k >= 1) MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[k - 1], 0));
             rc = lu_swap(P, k0, w, kn0, cw0);
             if (rc == MH_OK) rc = lu_trsm(P, k0, w, kn0, cw0);
+            P.chain = true;
             if (rc == MH_OK) rc = lu_gemm(P, kn0, n, kn0, cw0, k0, w);
+            P.chain = false;
         }
         if (LUt && rc == MH_OK) {  // block row k is final: transpose it on the side stream
             MH_HIP_CHECK(hipEventRecord(evN[k], sA));
