@@ -126,6 +126,17 @@ int mh_synchronize(mh_context_t ctx);
 int mh_profile_enable(mh_context_t ctx, int on);
 int mh_profile_report(mh_context_t ctx, char* buf, int64_t cap);
 
+/* solve(D, S, bc; direct) (src/matrix_assembly.jl:105-133) in mixed
+ * precision: an FP32 copy of D (K) is factored (trailing updates on split
+ * fp16 operands with a power-of-two scale per row and column, so no finite
+ * magnitude is out of range) and phi refined in FP64 against D (K) to a
+ * 1e-10 relative residual.  Arguments as mh_solve; phi must not alias bc.
+ * D (K) is left intact unless the refinement fails and the context allows
+ * the FP64 fallback (mh_set_refinement), which solves in place as mh_solve. */
+int mh_solve_mixed(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK, const double* S,
+                   int64_t ld, int64_t stride, const double* bc, int64_t nrhs, double* phi, int64_t ldb,
+                   int64_t bstride);
+
 /* Mixed-precision control (no reference counterpart; the reference solves
  * in FP64 only).  max_sweeps: iterative-refinement sweeps allowed per
  * frequency chunk (default 12, range 1-100); fp64_fallback (default 1): a

@@ -651,6 +651,82 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
     return MH_ERR_NOT_CONVERGED;
 }
 
+// solve(D, S, bc; direct) in mixed precision: an FP32 copy of D (K) factored
+// (split-image trailing updates), the system refined in FP64 against D (K)
+// to a 1e-10 relative residual.  D (K) is left intact unless the refinement
+// fails and the context allows the FP64 fallback, which factors it in place
+// as mh_solve does.
+int mh_solve_mixed(mh_context_t ctx, int64_t n, int64_t batch, int direct, double* DK, const double* S,
+                   int64_t ld, int64_t stride, const double* bc, int64_t nrhs, double* phi, int64_t ldb,
+                   int64_t bstride) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(n >= 0 && batch >= 0 && nrhs >= 0, "solve_mixed: negative size");
+    if (n == 0 || batch == 0 || nrhs == 0) return MH_OK;
+    MH_REQUIRE(DK && S && bc && phi, "solve_mixed: null array");
+    MH_REQUIRE(phi != bc, "solve_mixed: phi must not alias bc");
+    MH_REQUIRE(ld >= n && ldb >= n, "solve_mixed: leading dimension too small");
+    MH_REQUIRE(batch == 1 || (stride >= ld * n && bstride >= ldb * nrhs), "solve_mixed: stride too small");
+    MH_REQUIRE(n <= 4096, "solve_mixed: at most 4096 unknowns");
+    Context* c = CTX(ctx);
+    const int64_t bs = (batch == 1) ? ldb * nrhs : bstride;
+    Bump bump;
+    const size_t o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * (size_t)batch);
+    const size_t o_piv = bump.take(sizeof(int64_t) * (size_t)(n * batch));
+    const size_t o_info = bump.take(sizeof(int) * (size_t)batch);
+    const size_t o_perm = bump.take(sizeof(int) * (size_t)(n * batch));
+    const size_t o_inv = bump.take(sizeof(float2) * (size_t)tri_inv_elems(n, batch));
+    const size_t o_y = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_R = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_R32 = bump.take(sizeof(float2) * (size_t)(n * nrhs * batch));
+    const size_t o_nrm = bump.take(sizeof(double) * (size_t)(nrhs * batch));
+    const size_t o_tmp = bump.take(sizeof(double2) * (size_t)(bs * batch));
+    const size_t o_inv64 = bump.take(sizeof(double2) * (size_t)tri_inv_elems(n, batch));  // FP64 fallback
+    if (!ensure_workspace(c, bump.off)) return MH_ERR_HIP;
+    char* ws = static_cast<char*>(c->ws);
+    float2* D32 = reinterpret_cast<float2*>(ws + o_D32);
+    int* info = reinterpret_cast<int*>(ws + o_info);
+    const double2* dS = reinterpret_cast<const double2*>(S);
+    const double2* dbc = reinterpret_cast<const double2*>(bc);
+    double2* dphi = reinterpret_cast<double2*>(phi);
+    double2* ybuf = reinterpret_cast<double2*>(ws + o_y);
+    TRY(convert_to_f32(c, n, n, batch, reinterpret_cast<const double2*>(DK), ld, stride, D32, n, n * n));
+    TRY(clear_ints(c, info, batch));
+    TRY(lu_factor_f32(c, n, n, batch, D32, n, n * n, reinterpret_cast<int64_t*>(ws + o_piv), info, nullptr,
+                      reinterpret_cast<int*>(ws + o_perm), reinterpret_cast<float2*>(ws + o_inv)));
+    IrSystem sys[1];
+    if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
+        TRY(zgemm_nn_batched(c, n, nrhs, batch, dS, ld, stride, dbc, ldb, bs, ybuf, ldb, bs));
+        sys[0] = {0, ybuf, nrhs, ldb, bs, dphi, reinterpret_cast<double2*>(ws + o_R),
+                  reinterpret_cast<float2*>(ws + o_R32), reinterpret_cast<double*>(ws + o_nrm)};
+    } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
+        sys[0] = {0, dbc, nrhs, ldb, bs, ybuf, reinterpret_cast<double2*>(ws + o_R),
+                  reinterpret_cast<float2*>(ws + o_R32), reinterpret_cast<double*>(ws + o_nrm)};
+    }
+    bool nonfinite = false;
+    int rc = ir_solve(c, n, batch, reinterpret_cast<const double2*>(DK), ld, stride, D32, n * n,
+                      reinterpret_cast<int*>(ws + o_perm), reinterpret_cast<float2*>(ws + o_inv), sys, 1, nullptr,
+                      nullptr, &nonfinite);
+    if (rc == MH_ERR_NOT_CONVERGED) {
+        const int rc_info = check_info(c, info, batch);  // a more specific cause
+        if (rc_info != MH_OK) rc = rc_info;
+    }
+    if (rc == MH_ERR_NOT_CONVERGED || rc == MH_ERR_SINGULAR) {
+        if (nonfinite || !c->fp64_fallback) return rc;
+        c->fallbacks += 1;  // FP64 in place, as mh_solve
+        MH_HIP_CHECK(hipMemcpyAsync(dphi, dbc, sizeof(double2) * (size_t)(bs * batch), hipMemcpyDeviceToDevice,
+                                    c->stream));
+        TRY(solve_impl(c, n, batch, direct, reinterpret_cast<double2*>(DK), dS, ld, stride, dphi, nrhs, ldb, bs,
+                       reinterpret_cast<double2*>(ws + o_tmp), reinterpret_cast<int64_t*>(ws + o_piv),
+                       reinterpret_cast<int*>(ws + o_perm), reinterpret_cast<double2*>(ws + o_inv64), info));
+        TRY(tri_check(c));
+        return check_info(c, info, batch);
+    }
+    TRY(rc);
+    if (!direct) TRY(zgemm_nn_batched(c, n, nrhs, batch, dS, ld, stride, ybuf, ldb, bs, dphi, ldb, bs));
+    TRY(tri_check(c));
+    return MH_OK;
+}
+
 }  // extern "C"
 
 namespace mh {

@@ -70,6 +70,7 @@ _SIGS = {
     "mh_froude_krylov_grad": (_int, [_vp, _vp, _i64, _vp, _d, _d, _i64, _vp, _i64, _vp, _i64, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp]),
     "mh_multi_sweep": (_int, [_vp, _i64, _vp, _int, _d, _d, _int]),
+    "mh_solve_mixed": (_int, [_vp, _i64, _i64, _int, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64]),
     "mh_sweep_graph_create": (_int, [_vp, _vp, _int, _int, _i64, _i64, _vp, _i64, _d, _d, _vp, _vp, _vp, _vp]),
     "mh_sweep_graph_launch": (_int, [_vp, _vp, _vp, _vp]),
     "mh_sweep_graph_destroy": (_int, [_vp]),

@@ -329,12 +329,18 @@ def _as_colmajor_batch(torch, X, device):
     X = X.to(device=f"cuda:{device}", dtype=torch.complex128)
     if X.dim() == 2:
         X = X.unsqueeze(0)
-    # memory [b][j][i] = X[b, i, j]
-    return X.transpose(1, 2).contiguous()
+    # memory [b][j][i] = X[b, i, j], always a fresh buffer: .contiguous()
+    # alone would return X's own storage when it is already column-major
+    # (the assembly's output), and the FP64 solve factors its buffer in place
+    out = torch.empty((X.shape[0], X.shape[2], X.shape[1]), dtype=torch.complex128, device=X.device)
+    out.copy_(X.transpose(1, 2))
+    return out
 
 
-def solve(D, S, bc, direct=True, device=0):
-    """src/matrix_assembly.jl:105-133.  D (or K) is not modified."""
+def solve(D, S, bc, direct=True, device=0, precision="fp64"):
+    """src/matrix_assembly.jl:105-133.  D (or K) is not modified.
+    precision="mixed": FP32 factorisation of D (K) refined in FP64 to a
+    1e-10 relative residual (mh_solve_mixed)."""
     torch = _torch()
     Dm = _as_colmajor_batch(torch, D, device)
     Sm = _as_colmajor_batch(torch, S, device)
@@ -350,8 +356,9 @@ def solve(D, S, bc, direct=True, device=0):
     bm = b.transpose(0, 1).contiguous()  # memory column-major n x nrhs
     phi = torch.empty_like(bm)
     ctx = L.context(device)
-    L.check(L.lib().mh_solve(ctx, n, 1, int(bool(direct)), Dm.data_ptr(), Sm.data_ptr(), n, n * n,
-                             bm.data_ptr(), nrhs, phi.data_ptr(), n, n * nrhs), "solve")
+    fn = {"fp64": L.lib().mh_solve, "mixed": L.lib().mh_solve_mixed}[precision]
+    L.check(fn(ctx, n, 1, int(bool(direct)), Dm.data_ptr(), Sm.data_ptr(), n, n * n,
+               bm.data_ptr(), nrhs, phi.data_ptr(), n, n * nrhs), "solve")
     out = phi.transpose(0, 1)
     return out[:, 0] if vec else out
 

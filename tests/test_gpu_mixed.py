@@ -75,3 +75,43 @@ def test_fp32_factors_converge_in_two_sweeps(mh, gpu, direct):
     for g, r, name in zip(got, ref, ("A", "B", "F")):
         assert relmax(g, r) < 1e-8, name
 
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_mixed_solve_of_systems_far_outside_the_fp16_range(mh, gpu, direct):
+    """mh_solve_mixed on D (K) and S scaled together by 2^40 and by 2^-40
+    (phi = D^-1 S bc and S K^-1 bc are invariant): the FP32 factorisation
+    scales each row of L21 and column of U12 by its own power of two before
+    the fp16 split, so every entry is in range, the refinement converges
+    without falling back to FP64 and the solution is the unscaled FP64 one.
+    512 panels: the factorisation's narrow and wide trailing updates both
+    run on the split-image kernel."""
+    import torch
+
+    m = mh.hemisphere(1.0, ntheta=32, nphi=32).mesh
+    assert m.nfaces == 512
+    S, DK = mh.assemble_matrices((mh.Rankine(), mh.RankineReflected(), mh.GFWu()), m, 0.4, direct=direct)
+    rng = np.random.default_rng(11)
+    bc = torch.from_numpy(rng.normal(size=(512, 2)) + 1j * rng.normal(size=(512, 2))).to(DK.device)
+    ref = mh.solve(DK, S, bc, direct=direct).cpu().numpy()
+    n0 = mh._lib.fallback_count()
+    for s in (2.0 ** 40, 2.0 ** -40, 1.0):
+        got = mh.solve(DK * s, S * s, bc, direct=direct, precision="mixed").cpu().numpy()
+        assert relmax(got, ref) < 1e-9, s
+    assert mh._lib.fallback_count() == n0
+
+
+def test_solve_leaves_the_assembled_matrices_intact(mh, gpu):
+    """solve() on the assembly's own output (column-major memory): D and S
+    are not modified, so a second solve gives the same phi (the FP64 path
+    factors a copy in place)."""
+    import torch
+
+    m = mh.hemisphere(1.0, ntheta=12, nphi=12).mesh
+    S, D = mh.assemble_matrices((mh.Rankine(), mh.RankineReflected(), mh.GFWu()), m, 0.4)
+    D0, S0 = D.clone(), S.clone()
+    bc = torch.ones(m.nfaces, dtype=torch.complex128, device=D.device)
+    p1 = mh.solve(D, S, bc).cpu().numpy()
+    p2 = mh.solve(D, S, bc).cpu().numpy()
+    assert torch.equal(D, D0) and torch.equal(S, S0)
+    assert np.array_equal(p1, p2)
