@@ -9,8 +9,11 @@
  * Every function cites the reference file:line it restates.  Paths are
  * relative to the reference repository root.
  *
- * Pinning (what checks this file independently of its own formulas) is
- * listed in oracle/README.md and enforced by tests/test_oracle_*.py.
+ * Pinning (what checks this file independently of its own formulas):
+ * tests/test_oracle.py -- mpmath vectors at 30 digits (written by
+ * tests/golden/make_golden.py), the reference's own closed forms and
+ * constants, quadrature, finite differences, LAPACK, the Hulme heave and
+ * surge tables, Froude-Krylov closed forms and the Haskind relation.
  */
 #include "mh_oracle.h"
 
@@ -296,7 +299,7 @@ typedef long double complex lcplx;
 /* e^z E1(z), principal branch, for z in the closed upper half plane.
  * Series for small |z| (and along the negative real axis, where the series
  * has no cancellation), Lentz continued fraction elsewhere.  Checked against
- * mpmath at 30 digits in tests/test_oracle_golden.py. */
+ * mpmath at 30 digits in tests/test_oracle.py (tests/golden/delhommeau_mpmath.json). */
 static lcplx exp_e1(lcplx z) {
     long double az = cabsl(z);
     long double re = creall(z), im = cimagl(z);

@@ -20,7 +20,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "liborc.so")
+# MH_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of tests/test_sanitize.py)
+LIB_PATH = os.environ.get("MH_ORACLE_LIB") or os.path.join(HERE, "build", "liborc.so")
 
 RANKINE = 1
 RANKINE_REFLECTED = 2
