@@ -137,6 +137,34 @@ def cpu_baseline(args, mh, mesh, omegas, dofs, betas, g, rho, res, grad):
         ({k: float(v) for k, v in gerrs.items()} if gerrs else None)
 
 
+def uneven_batch(mh, torch, precision):
+    """The uneven multi-body batch: 256 seeded spheres, vertical cylinders
+    and boxes of 200-3,000 panels, 16 random frequencies each, heave and
+    surge, heading 0, gradients w.r.t. each body's dimensions, in one
+    scheduled call (mh_multi_sweep; worker streams, longest body first).
+    The autotuner (worker count, LU leaf width) on the 16 largest bodies
+    first, which also sizes the worker contexts' workspaces for the largest
+    body; then one timed call on all 256, inputs on the device."""
+    items = mh.random_workload(5, 256)
+    prep = mh.batches.prepare(items)
+    largest = sorted(items, key=lambda it: -it.mesh.nfaces)[:16]
+    workers, leaf, tune_ms = mh.multi_autotune(largest, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mh.multi_sweep(items, precision, prepared=prep)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    nfs = sum(len(it.omegas) for it in items)
+    panels = [it.mesh.nfaces for it in items]
+    log(f"bench: uneven batch, {len(items)} bodies ({min(panels)}-{max(panels)} panels), {nfs} freq-solves "
+        f"with gradients in {sec:.2f} s ({nfs / sec:.0f}/s)")
+    return {"workload": f"{len(items)} seeded bodies (spheres, vertical cylinders, boxes) of {min(panels)}-"
+                        f"{max(panels)} panels, 16 frequencies each, heave + surge, heading 0, gradients "
+                        "w.r.t. each body's dimensions",
+            "freq_solves": nfs, "seconds": round(sec, 3), "freq_solves_per_s": round(nfs / sec, 1),
+            "precision": precision, "workers": workers, "leaf_width_cap": leaf}
+
+
 def delhommeau_assembly(mh, torch, np):
     """S and D of the 4,000-panel cylinder at 32 wavenumbers (omega 0.2-2.5
     rad/s), exact and tabulated Delhommeau term, direct formulation: wall
@@ -179,6 +207,8 @@ def main():
                          "for rocprofv3 runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven multi-body batch (256 bodies of 200-3,000 panels)")
     ap.add_argument("--delhommeau-assembly", action="store_true",
                     help="only the assembly of the 4,000-panel cylinder (R 5 m, draft 10 m) at 32 wavenumbers "
                          "with the exact and the tabulated Delhommeau term (timings and their difference)")
@@ -344,6 +374,8 @@ def main():
     kernel_ms = {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"bench: kernel ms per sweep {kernel_ms}")
 
+    batch = None if args.no_batch else uneven_batch(mh, torch, args.precision)
+
     cpu = None
     oracle_err = None
     grad_err = None
@@ -379,6 +411,7 @@ def main():
         "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,
         "assembly_matrix_writes": asm_write,
         "kernel_ms_per_step": kernel_ms,
+        "uneven_batch": batch,
     }
     print(json.dumps(out), flush=True)
 
