@@ -296,7 +296,7 @@ int mh_freq_sweep(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
  * Radii only select the near/far branch and carry no derivative.
  * Outputs (device): A, B, F as mh_freq_sweep; dA, dB: nomega x nparam x
  * ndof x ndof (same inner layout as A); dF: nomega x nparam x nbeta x ndof
- * complex128.  Synchronises.  Limits: ndof >= 1, ndof + nbeta <= 4. */
+ * complex128.  Synchronises.  Limits: ndof >= 1, ndof + nbeta <= 16. */
 int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask,
                        int direct, int precision, int64_t nomega,
                        const double* omegas, double g, double rho, int64_t ndof,
@@ -365,7 +365,7 @@ typedef struct {
  * use and kept by ctx) so that bodies overlap on the GPU.  nworkers = 0 uses
  * the autotuned configuration (or 2 before tuning).  Outputs per item as
  * mh_freq_sweep_grad.  Synchronises; on failure the message names the item.
- * Limits per item as mh_freq_sweep (gradients: ndof + nbeta <= 4). */
+ * Limits per item as mh_freq_sweep_grad when nparam > 0, else as mh_freq_sweep. */
 int mh_multi_sweep(mh_context_t ctx, int64_t nitems, const mh_sweep_item_t* items,
                    int precision, double g, double rho, int nworkers);
 /* Times the given items (a sample of the workload; their outputs are

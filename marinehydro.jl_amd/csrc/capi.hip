@@ -1130,7 +1130,6 @@ int mh_freq_sweep_grad(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, int
     MH_REQUIRE(nparam >= 0, "freq_sweep_grad: negative nparam");
     if (mesh->nfaces == 0 || nomega == 0 || ndof + nbeta == 0) return MH_OK;
     MH_REQUIRE(ndof >= 1, "freq_sweep_grad: at least one dof (the influenced force)");
-    MH_REQUIRE(ndof + nbeta <= 4, "freq_sweep_grad: at most 4 dofs + headings");
     MH_REQUIRE(nparam == 0 || (dA && dB && (nbeta == 0 || dF)), "freq_sweep_grad: null gradient output");
     GradArgs ga{nparam, dvertices, dcenters, dnormals, dareas, ddof_fields, domegas, dA, dB, dF};
     return sweep_impl(CTX(ctx), mesh, gf_mask, direct, precision, nomega, omegas, g, rho, ndof,

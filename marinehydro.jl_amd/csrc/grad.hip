@@ -76,13 +76,15 @@ __device__ __forceinline__ double2 cmac(double2 acc, double ar, double ai, doubl
 //       aS^i_p dS_pq,m sS^r_q - aD^i_p dDK_pq,m sD^r_q
 // FORM 1: DK = D (source normal, gradient w.r.t. the source point);
 // FORM 2: DK = K (collocation normal, gradient w.r.t. the collocation point).
+// One launch covers the right-hand sides r0 .. r0 + rn - 1 (rn <= NR) of the
+// nr in the part layout, and every i.
 // ---------------------------------------------------------------------------
 template <int WAVE, int FORM, int P, int NR>
 __global__ __launch_bounds__(256) void k_tangent_pairs(
     int64_t n, const double* __restrict__ C, const double* __restrict__ N,
     const PanelRec* __restrict__ panels, const double* __restrict__ tan, int gf,
-    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr,
-    const double2* __restrict__ aS, const double2* __restrict__ aD, const double2* __restrict__ sS,
+    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr, int64_t r0,
+    int64_t rn, const double2* __restrict__ aS, const double2* __restrict__ aD, const double2* __restrict__ sS,
     const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s,
     double2* __restrict__ part) {
     typedef Dual<P> D_;
@@ -106,8 +108,8 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
     k.v = ks[w];
 #pragma unroll
     for (int m = 0; m < P; ++m) k.d[m] = dks[w * P + m];
-    const double2* sSw = sS + w * wstride_s;
-    const double2* sDw = sD + w * wstride_s;
+    const double2* sSw = sS + w * wstride_s + r0 * n;
+    const double2* sDw = sD + w * wstride_s + r0 * n;
     double2 RS[NR][P], RD[NR][P];
 #pragma unroll
     for (int r = 0; r < NR; ++r)
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (r < nr) {
+            if (r < rn) {
                 const double2 ss = sSw[j + r * n];
                 const double2 sd = sDw[j + r * n];
 #pragma unroll
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
         const double2 ad = row_ok ? aDw[i + d * n] : make_double2(0.0, 0.0);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (r >= nr) continue;
+            if (r >= rn) continue;
 #pragma unroll
             for (int m = 0; m < P; ++m) {
                 double vx = as.x * RS[r][m].x - as.y * RS[r][m].y - (ad.x * RD[r][m].x - ad.y * RD[r][m].y);
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
                 if (threadIdx.x == 0) {
                     const double2 a = red[0][r * P + m], b = red[1][r * P + m],
                                   c2 = red[2][r * P + m], e = red[3][r * P + m];
-                    part[(((w * nblk + blk) * ndof + d) * nr + r) * P + m] =
+                    part[(((w * nblk + blk) * ndof + d) * nr + r0 + r) * P + m] =
                         make_double2(a.x + b.x + c2.x + e.x, a.y + b.y + c2.y + e.y);
                 }
                 __syncthreads();
@@ -224,14 +226,17 @@ __global__ __launch_bounds__(256) void k_tangent_pairs(
 // diagonal tiles only (p, q), tiles P > Q are covered by their partner.
 // Contraction: (p, q) terms accumulate dS_pq sS_q per row p (weighted by aS_p
 // at the end, as in k_tangent_pairs); (q, p) terms accumulate aS_q dS_qp per
-// column p (weighted by sS_p at the end).
+// column p (weighted by sS_p at the end).  One launch covers the dofs d0 ..
+// d0 + dn - 1 and the right-hand sides r0 .. r0 + rn - 1 (dn, rn <= NR) of
+// the ndof x nr outputs.
 // ---------------------------------------------------------------------------
 template <int WAVE, int FORM, int P, int NR>
 __global__ __launch_bounds__(256) void k_tangent_sym(
     int64_t n, const double* __restrict__ C, const PanelRec* __restrict__ panels, const double* __restrict__ tan,
-    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr,
-    const double2* __restrict__ aS, const double2* __restrict__ aD, const double2* __restrict__ sS,
-    const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s, double2* __restrict__ part) {
+    const double* __restrict__ ks, const double* __restrict__ dks, int64_t ndof, int64_t nr, int64_t d0,
+    int64_t dn, int64_t r0, int64_t rn, const double2* __restrict__ aS, const double2* __restrict__ aD,
+    const double2* __restrict__ sS, const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s,
+    double2* __restrict__ part) {
     typedef Dual<P> D_;
     typedef cplxT<D_> CD;
     const int lane = threadIdx.x & 63;
@@ -242,7 +247,10 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
     const int64_t nout = ndof * nr * P;
     double2* pw = part + (w * nblk + blk) * nout;
     if (blockIdx.x > blockIdx.y) {  // covered by the tile pair (y, x)
-        for (int64_t o = threadIdx.x; o < nout; o += 256) pw[o] = make_double2(0.0, 0.0);
+        for (int64_t o = threadIdx.x; o < dn * rn * P; o += 256) {
+            const int64_t d = o / (rn * P), r = (o / P) % rn, m = o % P;
+            pw[((d0 + d) * nr + r0 + r) * P + m] = make_double2(0.0, 0.0);
+        }
         return;
     }
     const bool sym = blockIdx.x < blockIdx.y;
@@ -261,10 +269,10 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
     k.v = ks[w];
 #pragma unroll
     for (int m = 0; m < P; ++m) k.d[m] = dks[w * P + m];
-    const double2* sSw = sS + w * wstride_s;
-    const double2* sDw = sD + w * wstride_s;
-    const double2* aSw = aS + w * wstride_a;
-    const double2* aDw = aD + w * wstride_a;
+    const double2* sSw = sS + w * wstride_s + r0 * n;
+    const double2* sDw = sD + w * wstride_s + r0 * n;
+    const double2* aSw = aS + w * wstride_a + d0 * n;
+    const double2* aDw = aD + w * wstride_a + d0 * n;
     // (q, p) accumulators in LDS (one slot per thread) when small enough:
     // keeps the pair loop within two waves per SIMD
     constexpr bool kLdsU = NR * P <= 8;
@@ -341,7 +349,7 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
         const CD Dpq = scl(kaq, FORM == 1 ? Eq : Ep);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (r < nr) {
+            if (r < rn) {
                 const double2 ss = sSw[j + r * n];
                 const double2 sd = sDw[j + r * n];
 #pragma unroll
@@ -357,7 +365,7 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
             const CD Dqp = scl(kap, FORM == 1 ? Ep : Eq);
 #pragma unroll
             for (int d = 0; d < NR; ++d) {
-                if (d < ndof) {
+                if (d < dn) {
                     const double2 as = aSw[j + d * n];
                     const double2 ad = aDw[j + d * n];
 #pragma unroll
@@ -379,7 +387,7 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
     (void)cmul;
     // contract and reduce over the workgroup
     __shared__ double2 red[4][NR * P];
-    for (int64_t d = 0; d < ndof; ++d) {
+    for (int64_t d = 0; d < dn; ++d) {
         const double2 as = row_ok ? aSw[i + d * n] : make_double2(0.0, 0.0);
         const double2 ad = row_ok ? aDw[i + d * n] : make_double2(0.0, 0.0);
         double2 usd[P], udd[P];
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (r >= nr) continue;
+            if (r >= rn) continue;
             const double2 ssp = row_ok ? sSw[i + r * n] : make_double2(0.0, 0.0);
             const double2 sdp = row_ok ? sDw[i + r * n] : make_double2(0.0, 0.0);
 #pragma unroll
@@ -418,11 +426,11 @@ __global__ __launch_bounds__(256) void k_tangent_sym(
             }
         }
         __syncthreads();
-        if (threadIdx.x < nr * P) {
+        if (threadIdx.x < rn * P) {
             const int r = threadIdx.x / P, m = threadIdx.x % P;
             const double2 a = red[0][r * P + m], b = red[1][r * P + m], c2 = red[2][r * P + m],
                           e = red[3][r * P + m];
-            pw[(d * nr + r) * P + m] = make_double2(a.x + b.x + c2.x + e.x, a.y + b.y + c2.y + e.y);
+            pw[((d0 + d) * nr + r0 + r) * P + m] = make_double2(a.x + b.x + c2.x + e.x, a.y + b.y + c2.y + e.y);
         }
         __syncthreads();
     }
@@ -772,10 +780,18 @@ __global__ __launch_bounds__(256) void k_bc_tangent(
 template <int WAVE, int FORM, int P, int NR>
 static void launch_tp(dim3 grid, hipStream_t st, int64_t n, const double* C, const double* N,
                       const PanelRec* P_, const double* tan, int gf, const double* ks, const double* dks,
-                      int64_t ndof, int64_t nr, const double2* aS, const double2* aD, const double2* sS,
-                      const double2* sD, int64_t wa, int64_t ws, double2* part) {
+                      int64_t ndof, int64_t nr, int64_t r0, int64_t rn, const double2* aS, const double2* aD,
+                      const double2* sS, const double2* sD, int64_t wa, int64_t ws, double2* part) {
     hipLaunchKernelGGL((k_tangent_pairs<WAVE, FORM, P, NR>), grid, dim3(256), 0, st, n, C, N, P_, tan, gf,
-                       ks, dks, ndof, nr, aS, aD, sS, sD, wa, ws, part);
+                       ks, dks, ndof, nr, r0, rn, aS, aD, sS, sD, wa, ws, part);
+}
+
+// column blocks of at most 4: cnt columns in ceil(cnt / 4) blocks of
+// near-equal width; block b is [b0, b0 + bn)
+static void column_block(int64_t cnt, int64_t b, int64_t* b0, int64_t* bn) {
+    const int64_t nb = (cnt + 3) / 4, base = cnt / nb, extra = cnt % nb;
+    *b0 = b * base + (b < extra ? b : extra);
+    *bn = base + (b < extra ? 1 : 0);
 }
 
 int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, const double* tan,
@@ -787,16 +803,19 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
     const int64_t tiles = (n + 63) / 64;
     *nblk_out = tiles * tiles;
     dim3 grid((unsigned)tiles, (unsigned)tiles, (unsigned)nw);
-    MH_REQUIRE(nr <= 4, "gradients: at most 4 dofs + headings per pass");
+    MH_REQUIRE(ndof >= 1 && ndof <= nr && nr <= 16, "gradients: 1 <= dofs <= dofs + headings <= 16");
     ProfScope ps(ctx, "tangent_pairs", 0.0, 0.0);
     // the tabulated Delhommeau term is differentiated as the exact one
     const int wave = (gf & GF_WU) ? 1 : ((gf & (GF_DELHOMMEAU | GF_DELH_TABLE)) ? 2 : 0);
-    const bool nr2 = nr <= 2;
+    // the pair kernels hold at most 4 right-hand sides (and, wave term only,
+    // 4 dofs) in registers: wider sweeps take one launch per column block,
+    // each writing its own entries of the partial sums
+    const int64_t nrb = (nr + 3) / 4;
     if (wave && !(gf & (GF_RANKINE | GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG))) {
         // wave term only: one evaluation per unordered pair
 #define TS(WV, FM, PP, NRR)                                                                                 \
     hipLaunchKernelGGL((k_tangent_sym<WV, FM, PP, NRR>), grid, dim3(256), 0, ctx->stream, n, m->centers, panels, \
-                       tan, d_k, d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)
+                       tan, d_k, d_dk, ndof, nr, d0, dn, r0, rn, aS, aD, sS, sD, wa, ws, part)
 #define TS_NR(WV, FM, PP)           \
     do {                            \
         if (nr2)                    \
@@ -813,11 +832,19 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
         else                        \
             TS_NR(WV, FM, 4);       \
     } while (0)
-        if (wave == 1) {
-            if (form == 1) TS_P(1, 1); else TS_P(1, 2);
-        } else {
-            if (form == 1) TS_P(2, 1); else TS_P(2, 2);
-        }
+        const int64_t ndb = (ndof + 3) / 4;
+        for (int64_t db = 0; db < ndb; ++db)
+            for (int64_t rb = 0; rb < nrb; ++rb) {
+                int64_t d0, dn, r0, rn;
+                column_block(ndof, db, &d0, &dn);
+                column_block(nr, rb, &r0, &rn);
+                const bool nr2 = dn <= 2 && rn <= 2;
+                if (wave == 1) {
+                    if (form == 1) TS_P(1, 1); else TS_P(1, 2);
+                } else {
+                    if (form == 1) TS_P(2, 1); else TS_P(2, 2);
+                }
+            }
 #undef TS_P
 #undef TS_NR
 #undef TS
@@ -826,7 +853,7 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
     }
 #define TP(WV, FM, PP, NRR)                                                                        \
     launch_tp<WV, FM, PP, NRR>(grid, ctx->stream, n, m->centers, m->normals, panels, tan, gf, d_k, \
-                               d_dk, ndof, nr, aS, aD, sS, sD, wa, ws, part)
+                               d_dk, ndof, nr, r0, rn, aS, aD, sS, sD, wa, ws, part)
 #define TP_NR(WV, FM, PP) \
     do {                  \
         if (nr2)          \
@@ -850,12 +877,17 @@ int launch_tangent_pairs(Context* ctx, const mh_mesh_t* m, const PanelRec* panel
         else                   \
             TP_FORM(0, PP);    \
     } while (0)
-    if (P == 1)
-        TP_WAVE(1);
-    else if (P == 2)
-        TP_WAVE(2);
-    else
-        TP_WAVE(4);
+    for (int64_t rb = 0; rb < nrb; ++rb) {
+        int64_t r0, rn;
+        column_block(nr, rb, &r0, &rn);
+        const bool nr2 = rn <= 2;
+        if (P == 1)
+            TP_WAVE(1);
+        else if (P == 2)
+            TP_WAVE(2);
+        else
+            TP_WAVE(4);
+    }
 #undef TP_WAVE
 #undef TP_FORM
 #undef TP_NR

@@ -49,7 +49,6 @@ static int item_ok(const mh_sweep_item_t& it, int precision, double g, double rh
     MH_REQUIRE(it.nparam >= 0, "multi_sweep: negative nparam");
     if (it.nparam > 0 && it.mesh->nfaces > 0 && it.nomega > 0) {
         MH_REQUIRE(it.ndof >= 1, "multi_sweep: gradients need at least one dof");
-        MH_REQUIRE(it.ndof + it.nbeta <= 4, "multi_sweep: gradients take at most 4 dofs + headings");
         MH_REQUIRE(it.dA && it.dB && (it.nbeta == 0 || it.dF), "multi_sweep: null gradient output");
     }
     return MH_OK;

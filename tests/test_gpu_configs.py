@@ -12,7 +12,9 @@
   config 4: truncated vertical cylinder R = 5, draft 10 (4,000 panels), six
             rigid dofs, 32 frequencies over 0.2 - 2.5 rad/s, exact
             Delhommeau, direct; mixed (FP32 LU + FP64 refinement) against
-            FP64 for all 32, both against the oracle at one frequency.
+            FP64 for all 32, both against the oracle at one frequency;
+            forward-mode gradients w.r.t. radius and draft at two
+            frequencies against central differences of FP64 sweeps.
 
 Errors are per quantity: |x - x_ref| <= tol * max|x_ref| over that
 quantity's array (the off-diagonal coefficients change sign, so a per-entry
@@ -190,3 +192,42 @@ def test_config4_cylinder4000_six_dofs(mh, gpu):
         err = relmax(X, Xo)
         print(f"   vs oracle {name}: {err:.2e}", flush=True)
         assert err < 1e-9, (name, err)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config4_cylinder4000_six_dof_gradients(mh, gpu, precision):
+    """forward-mode gradients of the 4,000-panel cylinder's six-dof A, B and
+    F (heading 0) w.r.t. radius and draft at two of the 32 frequencies,
+    against central differences (h = 1e-6 relative) of FP64 sweeps on
+    regenerated meshes; the FP64 sweep itself is pinned to the oracle above.
+    Seven right-hand sides: the pair kernels run in 2 x 2 column blocks."""
+    fx = _fixtures()["cylinder4000"]
+    R, H = 5.0, 10.0
+    make = lambda r, h: mh.vertical_cylinder(r, h, ntheta=80, nz=30, nr=20)
+    body = make(R, H)
+    c = body.mesh
+    names = fx["dofs"]
+    omegas = np.linspace(0.2, 2.5, 32)[[5, 15]]
+    gfs = _gfs(mh, DELH)
+    axes = [None, None, None, [1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]]
+    T = mh.shape_tangents(c, [body.dvertices["radius"], body.dvertices["draft"]], dof_axes=axes)
+    t0 = time.time()
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(c, omegas, mh.rigid_dofs(c, names), [0.0], T, green_functions=gfs,
+                                             precision=precision, rho=RHO, g=G)
+    t1 = time.time()
+    theta = [R, H]
+    for m in range(2):
+        step = 1e-6 * theta[m]
+        outs = []
+        for sgn in (1, -1):
+            t = list(theta)
+            t[m] += sgn * step
+            mm = make(*t).mesh
+            outs.append(mh.freq_sweep(mm, omegas, mh.rigid_dofs(mm, names), betas=[0.0], green_functions=gfs,
+                                      precision="fp64", rho=RHO, g=G))
+        for q, (got, name) in enumerate(((dA, "dA"), (dB, "dB"), (dF, "dF"))):
+            ref = (np.asarray(outs[0][q]) - np.asarray(outs[1][q])) / (2 * step)
+            err = relmax(got[:, m], ref)
+            print(f"config4 {precision} d/d{('radius', 'draft')[m]} {name}: {err:.1e}", flush=True)
+            assert err < 1e-6, (precision, m, name, err)
+    print(f"config4 {precision} gradient sweep (2 parameters, 2 frequencies): {t1 - t0:.2f} s", flush=True)

@@ -148,3 +148,42 @@ def test_wavebot_shape_gradients(mh, oracle, gpu):
         check(dA[0, m], fA, 1e-6, ("dA", k), A[0])
         check(dB[0, m], fB, 1e-6, ("dB", k), B[0])
         check(dF[0, m, 0], fF[:, 0], 1e-6, ("dF", k), F[0, 0])
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_six_rigid_dofs_and_two_headings(mh, oracle, gpu, direct):
+    """all six rigid-body dofs about an offset centre and two headings: eight
+    right-hand sides, which the pair kernels take in column blocks of at most
+    four (dofs 0-2 / 3-5, right-hand sides 0-3 / 4-7).  Radius, x position
+    and omega against central differences of the oracle; the heave-pitch
+    block against a narrower sweep of those two dofs and one heading."""
+    res, omega = 8, 1.1
+    rc = [0.1, -0.2, -0.3]
+    names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
+    axes = [None, None, None, [1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]]
+    theta0 = [1.0, 0.0, omega]
+    make = lambda t: mh.hemisphere(t[0], center=(t[1], 0.0, 0.0), ntheta=res, nphi=res).mesh
+    body = mh.hemisphere(1.0, ntheta=res, nphi=res)
+    mesh = body.mesh
+    dofs_of = lambda m: mh.rigid_dofs(m, names, rotation_center=rc)
+    betas = [0.0, 0.9]
+    dV = [body.dvertices["radius"], body.dvertices["x"], None]
+    T = mh.shape_tangents(mesh, dV, nomega=1, omega_param=2, dof_axes=axes)
+    A, B, F, dA, dB, dF = mh.freq_sweep_grad(mesh, [omega], dofs_of(mesh), betas, T, direct=direct, rho=1025.0)
+    assert dA.shape == (1, 3, 6, 6) and dF.shape == (1, 3, 2, 6)
+    Ao, Bo, Fo = oracle.freq_solve(GF_WU, mesh, omega, dofs_of(mesh), betas, direct=direct, rho=1025.0)
+    check(A[0], Ao, 1e-10, "A")
+    check(F[0].T, Fo, 1e-10, "F")
+    for m in range(3):
+        gA, gB, gF = fd(oracle, make, theta0, m, lambda t: t[2], dofs_of, betas, direct, 1e-5)
+        check(dA[0, m], gA, 1e-6, ("dA", m), A)
+        check(dB[0, m], gB, 1e-6, ("dB", m), B)
+        check(dF[0, m].T, gF, 1e-6, ("dF", m), F)
+    sub = [2, 4]
+    T2 = mh.shape_tangents(mesh, dV, nomega=1, omega_param=2, dof_axes=[None, axes[4]])
+    d6 = dofs_of(mesh)
+    _, _, _, dA2, dB2, dF2 = mh.freq_sweep_grad(mesh, [omega], [d6[2], d6[4]], [0.9], T2, direct=direct,
+                                                rho=1025.0)
+    check(dA[0][:, sub][:, :, sub], dA2[0], 1e-12, "dA block")
+    check(dB[0][:, sub][:, :, sub], dB2[0], 1e-12, "dB block")
+    check(dF[0][:, 1:2][:, :, sub], dF2[0], 1e-12, "dF block")
