@@ -398,6 +398,35 @@ int mh_sweep_graph_create(mh_context_t ctx, const mh_mesh_t* mesh, int gf_mask, 
 int mh_sweep_graph_launch(mh_context_t ctx, mh_sweep_graph_t graph, const double* omegas, const double* betas);
 int mh_sweep_graph_destroy(mh_sweep_graph_t graph);
 
+/* Captured batches: the items of mh_multi_sweep recorded once as one HIP
+ * graph and replayed with one hipGraphLaunch.  This build captures one FP64
+ * forward item on one worker (precision MH_MIXED, nparam > 0, several items
+ * or nworkers > 1 return MH_ERR_NOT_SUPPORTED: such captures crashed the
+ * host process and the cause was not found); it
+ * replaces the reference's loop over problems (src/solve.jl:17-21) when the
+ * same bodies are solved again and again.  The items are assigned longest
+ * first to nworkers private worker contexts (0: the context's autotuned
+ * worker count, else 2; at most the item count), each body to the worker
+ * with the least work so far.  Creation runs every item once (its outputs
+ * are valid results for the given frequencies) and then records the same
+ * sequence.  A launch takes the items again: their host arrays (omegas,
+ * betas, domegas) may change, their sizes, options and device arrays may
+ * not (the mesh and tangent contents may); it is synchronous, starts after
+ * the work already issued on ctx's stream, and launches of one graph must
+ * not overlap.  An item whose device checks fail (a mesh index out of
+ * range, a zero pivot) runs again without the graph, which reports its
+ * error as mh_multi_sweep does.  mh_batch_graph_info: recorded nodes, items
+ * run again so far, workers (each output may be NULL). */
+typedef struct mh_batch_graph* mh_batch_graph_t;
+int mh_batch_graph_create(mh_context_t ctx, int64_t nitems, const mh_sweep_item_t* items, int precision,
+                          double g, double rho, int nworkers, mh_batch_graph_t* out);
+int mh_batch_graph_launch(mh_context_t ctx, mh_batch_graph_t graph, int64_t nitems, const mh_sweep_item_t* items);
+int mh_batch_graph_info(mh_batch_graph_t graph, int64_t* nodes, int64_t* reruns, int* nworkers);
+/* The recorded graph written to `path` as Graphviz text (every node with its
+ * parameters), for inspection. */
+int mh_batch_graph_dot(mh_batch_graph_t graph, const char* path);
+int mh_batch_graph_destroy(mh_batch_graph_t graph);
+
 /* ---- host-array entry points (one call, synchronous) -------------------
  * calculate_radiation_forces(mesh, dof, omega) -> [A, B]
  * (src/waves.jl:29-37): Wu tuple, direct formulation, uniform dof 3-vector.

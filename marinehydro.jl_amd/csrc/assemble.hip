@@ -295,6 +295,70 @@ int clear_ints(Context* ctx, int* p, int64_t count) {
     return MH_OK;
 }
 
+__global__ void k_fill_zero16(uint4* __restrict__ p, int64_t count) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+int fill_zero(Context* ctx, void* p, size_t bytes) {
+    if (bytes == 0) return MH_OK;
+    MH_REQUIRE(bytes % 16 == 0 && (reinterpret_cast<uintptr_t>(p) & 15) == 0,
+               "fill_zero: 16-byte aligned multiples of 16 bytes only");
+    const int64_t count = (int64_t)(bytes / 16);
+    const int64_t blocks = (count + 255) / 256 < 1024 ? (count + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_fill_zero16, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, static_cast<uint4*>(p),
+                       count);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+__global__ void k_copy_d2(double2* __restrict__ dst, int64_t ldd, const double2* __restrict__ src, int64_t lds,
+                          int64_t rows, int64_t cols) {
+    for (int64_t j = blockIdx.y; j < cols; j += gridDim.y)
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
+            dst[i + j * ldd] = src[i + j * lds];
+}
+
+int copy_d2(Context* ctx, double2* dst, int64_t ldd, const double2* src, int64_t lds, int64_t rows, int64_t cols) {
+    if (rows <= 0 || cols <= 0) return MH_OK;
+    const int64_t bx = (rows + 255) / 256 < 512 ? (rows + 255) / 256 : 512;
+    const int64_t by = cols < 1024 ? cols : 1024;
+    hipLaunchKernelGGL(k_copy_d2, dim3((unsigned)bx, (unsigned)by), dim3(256), 0, ctx->stream, dst, ldd, src, lds,
+                       rows, cols);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+__global__ void k_or_words(const int* __restrict__ src, int64_t count, int* __restrict__ dst) {
+    int v = 0;
+    for (int64_t t = threadIdx.x; t < count; t += blockDim.x) v |= src[t];
+    if (v) atomicOr(dst, v);
+}
+
+int or_words(Context* ctx, const int* src, int64_t count, int* dst) {
+    if (count <= 0) return MH_OK;
+    hipLaunchKernelGGL(k_or_words, dim3(1), dim3(256), 0, ctx->stream, src, count, dst);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
+__global__ void k_ir_verdict(const double* __restrict__ nrm, int64_t count, double tol, int* __restrict__ dst) {
+    int v = 0;
+    for (int64_t t = threadIdx.x; t < count; t += blockDim.x) {
+        const double x = nrm[t];
+        if (!(x < tol)) v |= 1;  // a NaN compares false: not converged
+        if (!(x <= 1e300)) v |= 2;
+    }
+    if (v) atomicOr(dst, v);
+}
+
+int ir_verdict(Context* ctx, const double* nrm, int64_t count, double tol, int* dst) {
+    if (count <= 0) return MH_OK;
+    hipLaunchKernelGGL(k_ir_verdict, dim3(1), dim3(256), 0, ctx->stream, nrm, count, tol, dst);
+    MH_HIP_CHECK(hipGetLastError());
+    return MH_OK;
+}
+
 int launch_pack_panels(Context* ctx, const mh_mesh_t* m, PanelRec* out, int* flag_buf) {
     const int64_t nf = m->nfaces;
     if (nf == 0) return MH_OK;

@@ -546,21 +546,27 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
     // one pass over A (Pa: workspace of zgemv_pair)
     const bool pair = nsys == 2 && Pa && sys[0].trans == 0 && sys[1].trans == 1 && sys[0].nrhs <= 4 &&
                       sys[1].nrhs <= 4;
+    // captured: a fixed number of sweeps, the last one's residual norms
+    // turned into a device verdict (cap->check[2]) instead of a host check
+    const bool cap = c->capturing;
+    MH_REQUIRE(!cap || (c->cap && c->ir_graph_sweeps >= 2), "refinement: capture without check words");
     size_t hcount = 0;
     for (int k = 0; k < nsys; ++k) hcount += (size_t)(batch * sys[k].nrhs);
-    TRY(ensure_pinned(c, hcount));
+    if (!cap) TRY(ensure_pinned(c, hcount));
     if (nsys > 1) TRY(fork_aux(c));
     bool done[2] = {false, false};
     for (int k = 0; k < nsys; ++k) {
         OnAux on(c, k == 1);
         const IrSystem& q = sys[k];
-        MH_HIP_CHECK(hipMemsetAsync(q.X, 0, sizeof(double2) * (size_t)(q.bstride * batch), c->stream));
-        MH_HIP_CHECK(hipMemcpyAsync(q.R, q.Bm, sizeof(double2) * (size_t)(q.bstride * batch),
-                                    hipMemcpyDeviceToDevice, c->stream));
+        TRY(fill_zero(c, q.X, sizeof(double2) * (size_t)(q.bstride * batch)));
+        TRY(copy_d2(c, q.R, 0, q.Bm, 0, q.bstride * batch, 1));
     }
-    const int max_sweeps = c->ir_max_sweeps;
+    const int max_sweeps = cap ? c->ir_graph_sweeps : c->ir_max_sweeps;
     for (int it = 0; it < max_sweeps; ++it) {
         const bool both = pair && !done[0] && !done[1];
+        // residual norms: every sweep after the first (a captured sweep only
+        // at its last one)
+        const bool norms = cap ? it == max_sweeps - 1 : it > 0;
         size_t off = 0;
         for (int k = 0; k < nsys; ++k) {
             const IrSystem& q = sys[k];
@@ -582,10 +588,11 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
                 else
                     TRY(zgemm_nn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
                                          q.bstride, q.Bm));
-                if (it > 0) {
+                if (norms) {
                     TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
-                    MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + off, q.nrm, sizeof(double) * cnt,
-                                                hipMemcpyDeviceToHost, c->stream));
+                    if (!cap)
+                        MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + off, q.nrm, sizeof(double) * cnt,
+                                                    hipMemcpyDeviceToHost, c->stream));
                 }
             }
             off += cnt;
@@ -595,18 +602,26 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
             const IrSystem &f = sys[0], &a = sys[1];
             TRY(zgemv_pair(c, n, batch, A64, ld, stride, (int)f.nrhs, f.X, f.Bm, f.R, f.ldb, f.bstride, (int)a.nrhs,
                            a.X, a.Bm, a.R, a.ldb, a.bstride, Pa));
-            if (it > 0) {
+            if (norms) {
                 size_t o2 = 0;
                 for (int k = 0; k < nsys; ++k) {
                     const IrSystem& q = sys[k];
                     const size_t cnt = (size_t)(batch * q.nrhs);
                     TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
-                    MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + o2, q.nrm, sizeof(double) * cnt, hipMemcpyDeviceToHost,
-                                                c->stream));
+                    if (!cap)
+                        MH_HIP_CHECK(hipMemcpyAsync(c->h_pin + o2, q.nrm, sizeof(double) * cnt,
+                                                    hipMemcpyDeviceToHost, c->stream));
                     o2 += cnt;
                 }
             }
             TRY(fork_aux(c));  // the next solve on aux starts after the residuals
+        }
+        if (cap) {
+            if (it < max_sweeps - 1) continue;
+            if (nsys > 1) TRY(join_aux(c));
+            for (int k = 0; k < nsys; ++k)
+                TRY(ir_verdict(c, sys[k].nrm, batch * sys[k].nrhs, kIrTol, c->cap->check + 2));
+            return MH_OK;
         }
         // one FP32 solve leaves a residual far above the tolerance: the first
         // correction is always made, without a host round trip for the check
@@ -748,7 +763,8 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     if (chunk < 1) chunk = 1;
     if (chunk > nomega) chunk = nomega;
     // tangent directions per gradient pass
-    const int Pg = !ga ? 1 : (ga->nparam <= 1 ? 1 : (ga->nparam == 2 ? 2 : 4));
+    const int Pg = !ga ? 1 : grad_block(ga->nparam);
+    const int64_t nblocks = ga ? (ga->nparam + Pg - 1) / Pg : 0;
     const int64_t wa = n * (ndof > 0 ? ndof : 1);  // per-frequency stride of n x ndof blocks
     Bump bump;
     const size_t o_S = bump.take(mat_aug * chunk);
@@ -798,8 +814,8 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
         o_lam = bump.take(sizeof(double2) * (size_t)(wa * chunk));
         o_mu = bump.take(sizeof(double2) * (size_t)(wa * chunk));
         o_tan = bump.take(sizeof(double) * (size_t)(n * 19 * Pg));
-        o_dk = bump.take(sizeof(double) * (size_t)(chunk * Pg));
-        o_dom = bump.take(sizeof(double) * (size_t)(chunk * Pg));
+        o_dk = bump.take(sizeof(double) * (size_t)(nblocks * chunk * Pg));
+        o_dom = bump.take(sizeof(double) * (size_t)(nblocks * chunk * Pg));
         o_part = bump.take(sizeof(double2) * (size_t)(chunk * tiles * tiles * nout));
         o_T1 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
         o_T2 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
@@ -823,33 +839,23 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     double* d_k = reinterpret_cast<double*>(ws + o_k);
     double* d_be = reinterpret_cast<double*>(ws + o_be);
     PanelRec* P = reinterpret_cast<PanelRec*>(ws + o_P);
-    std::vector<double> hk(nomega);
-    for (int64_t w = 0; w < nomega; ++w) hk[w] = omegas[w] * omegas[w] / g;  // waves.jl:30
-    // graph capture (FP64 forward sweep in one chunk): the frequencies,
-    // wavenumbers and headings are copied in before each launch of the
-    // graph; everything from the panel packing on is recorded
-    struct CaptureEnd {
-        Context* c;
-        bool on = false;
-        ~CaptureEnd() {
-            if (on) {  // an error inside the captured region: end it, drop the graph
-                hipGraph_t gr = nullptr;
-                (void)hipStreamEndCapture(c->stream, &gr);
-                if (gr) (void)hipGraphDestroy(gr);
-            }
-        }
-    } cap_end{c};
-    if (c->capturing) {
-        MH_REQUIRE(!ga && !mixed && chunk >= nomega,
-                   "sweep graph: FP64 forward sweeps whose matrices fit one chunk only");
-        c->cap_om = d_om;
-        c->cap_k = d_k;
-        c->cap_be = d_be;
-        c->cap_info = info;
-        MH_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        cap_end.on = true;
-        TRY(launch_pack_panels(c, mesh, P, c->d_flag));
+    // captured (graph.hip): one chunk; frequencies, wavenumbers, headings and
+    // frequency tangents from the graph's device buffers, filled before
+    // each launch; the check words cleared here and set by the kernels
+    const bool cap = c->capturing;
+    // host staging of the copies below; it lives until the sweep's final
+    // synchronisation (a copy from pageable memory may still read it later)
+    std::vector<double> hk, hdk, hdom;
+    if (cap) {
+        MH_REQUIRE(c->cap && chunk >= nomega, "sweep graph: sweeps whose matrices fit one chunk only");
+        d_om = c->cap->om;
+        d_k = c->cap->k;
+        d_be = c->cap->be;
+        TRY(clear_ints(c, c->cap->check, kCheckWords));
+        TRY(launch_pack_panels(c, mesh, P, c->cap->check));
     } else {
+        hk.resize(nomega);
+        for (int64_t w = 0; w < nomega; ++w) hk[w] = omegas[w] * omegas[w] / g;  // waves.jl:30
         MH_HIP_CHECK(hipMemcpyAsync(d_om, omegas, sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
         MH_HIP_CHECK(hipMemcpyAsync(d_k, hk.data(), sizeof(double) * nomega, hipMemcpyHostToDevice, c->stream));
         if (nbeta)
@@ -890,7 +896,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                 OnSide on(c);
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
             }
-            MH_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int) * nw, c->stream));
+            TRY(clear_ints(c, info, nw));
             // with an adjoint system, the transposed factors are built during the LU
             float2* Dt32 = (ga && ndof) ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr;
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info, Dt32, perm, inv32));  // + solve data
@@ -947,9 +953,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             if (direct) {  // phi = D \ (S bc)   (src/matrix_assembly.jl:125-126)
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, aug, ld, stride));
             } else {  // sigma = K \ bc; phi = S sigma   (:127-131)
-                MH_HIP_CHECK(hipMemcpy2DAsync(aug, sizeof(double2) * stride, rhs, sizeof(double2) * bstride,
-                                              sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
-                                              c->stream));
+                TRY(copy_d2(c, aug, stride, rhs, bstride, bstride, nw));
             }
             TRY(clear_ints(c, info, nw));  // a fill kernel: capturable
             // the transposed FP64 factors only when the workspace was laid out
@@ -964,7 +968,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                 TRY(fork_aux(c));
                 OnAux on(c, true);
                 if (direct)  // lambda = D^-T c
-                    MH_HIP_CHECK(hipMemcpyAsync(lam, cw, sizeof(double2) * wa * nw, hipMemcpyDeviceToDevice, c->stream));
+                    TRY(copy_d2(c, lam, 0, cw, 0, wa * nw, 1));
                 else  // lambda = K^-T S^T c
                     TRY(zgemm_tn_batched(c, n, ndof, nw, dS, ld, stride, cw, n, wa, lam, n, wa));
                 TRY(tri_solve(c, n, nw, dD, ld, stride, perm, inv, 1, 3, ndof, lam, n, wa, dDt));
@@ -972,9 +976,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             TRY(tri_solve(c, n, nw, dD, ld, stride, nullptr, inv, 0, 2, nr, aug, ld, stride));
             if (ga && ndof) TRY(join_aux(c));
             if (direct) {
-                MH_HIP_CHECK(hipMemcpy2DAsync(phi, sizeof(double2) * bstride, aug, sizeof(double2) * stride,
-                                              sizeof(double2) * bstride, nw, hipMemcpyDeviceToDevice,
-                                              c->stream));
+                TRY(copy_d2(c, phi, bstride, aug, stride, bstride, nw));
             } else {
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, aug, ld, stride, phi, ldb, bstride));
             }
@@ -988,9 +990,8 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
         // at the end of the chunk, after the gradient kernels: one host
         // synchronisation per chunk
         if (!ga || ndof == 0) {
-            if (c->capturing) {  // checked after each launch of the graph
-                cap_end.on = false;
-                MH_HIP_CHECK(hipStreamEndCapture(c->stream, &c->cap_graph));
+            if (cap) {  // checked after each launch of the graph
+                TRY(or_words(c, info, nw, c->cap->check + 1));
                 continue;
             }
             TRY(tri_check(c));
@@ -1024,27 +1025,29 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             aS = cw; aDw = lam; sS = aug; sD = aug; wss = stride; rhoW = lam;
         }
         const int64_t np = ga->nparam;
-        // omega and wavenumber tangents of every parameter block; each block
-        // copies from its own slice, which lives until the chunk's final
-        // synchronisation
-        const int64_t nblocks = (np + Pg - 1) / Pg;
-        std::vector<double> hdk(nblocks * nw * Pg), hdom(nblocks * nw * Pg);
+        // omega and wavenumber tangents of every parameter block (nblocks x
+        // nw x Pg each), copied once per chunk (a captured sweep reads them
+        // from the graph's buffers)
+        if (!cap) {
+            hdk.resize(nblocks * nw * Pg);
+            hdom.resize(nblocks * nw * Pg);
+            omega_tangents(omegas, nomega, w0, nw, ga->domega, np, Pg, g, hdk.data(), hdom.data());
+            MH_HIP_CHECK(hipMemcpyAsync(d_dk, hdk.data(), sizeof(double) * hdk.size(), hipMemcpyHostToDevice,
+                                        c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(d_dom, hdom.data(), sizeof(double) * hdom.size(), hipMemcpyHostToDevice,
+                                        c->stream));
+        } else {
+            d_dk = c->cap->dk;
+            d_dom = c->cap->dom;
+        }
         for (int64_t m0 = 0; m0 < np; m0 += Pg) {
-            double* bdk = hdk.data() + (m0 / Pg) * nw * Pg;
-            double* bdom = hdom.data() + (m0 / Pg) * nw * Pg;
-            for (int64_t w = 0; w < nw; ++w)
-                for (int m = 0; m < Pg; ++m) {
-                    const double dw = (ga->domega && m0 + m < np) ? ga->domega[(m0 + m) * nomega + w0 + w] : 0.0;
-                    bdom[w * Pg + m] = dw;
-                    bdk[w * Pg + m] = 2.0 * omegas[w0 + w] / g * dw;
-                }
-            MH_HIP_CHECK(hipMemcpyAsync(d_dk, bdk, sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
-            MH_HIP_CHECK(hipMemcpyAsync(d_dom, bdom, sizeof(double) * nw * Pg, hipMemcpyHostToDevice, c->stream));
+            const double* bdk = d_dk + (m0 / Pg) * nw * Pg;
+            const double* bdom = d_dom + (m0 / Pg) * nw * Pg;
             const bool done = early_rk && m0 == 0;  // tangents and Rankine parts already there
             if (!done) TRY(launch_pack_tangents(c, mesh, Pg, np, m0, ga->dV, ga->dC, ga->dN, ga->dA, tan));
             // the Rankine parts are contracted into T2 on the side stream while
             // the wave-term pair kernel runs; its partial sums are added after
-            MH_HIP_CHECK(hipMemsetAsync(T2, 0, sizeof(double2) * (size_t)(nw * ndof * nr * Pg), c->stream));
+            TRY(fill_zero(c, T2, sizeof(double2) * (size_t)(nw * ndof * nr * Pg)));
             if (rank_mask) {
                 TRY(fork_side(c));
                 OnSide on(c);
@@ -1055,22 +1058,40 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             }
             int64_t nblk = 0;
             if (wave_mask)
-                TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, d_dk, ndof, nr, aS,
+                TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, bdk, ndof, nr, aS,
                                          aDw, sS, sD, wa, wss, part, &nblk));
             if (rank_mask) TRY(join_side(c));
             TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 1));
             if (direct) TRY(join_aux(c));
-            TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, d_dom, g, rho, ndof, dof_fields,
+            TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, bdom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
-            TRY(launch_grad_finish(c, nw, np, m0, Pg, ndof, nr, nbeta, d_om + w0, d_dom, Aw, Bw, T1, T2,
+            TRY(launch_grad_finish(c, nw, np, m0, Pg, ndof, nr, nbeta, d_om + w0, bdom, Aw, Bw, T1, T2,
                                    ga->dAout ? ga->dAout + w0 * np * ndof * ndof : nullptr,
                                    ga->dBout ? ga->dBout + w0 * np * ndof * ndof : nullptr,
                                    (ga->dFout && nbeta) ? ga->dFout + 2 * w0 * np * nbeta * ndof : nullptr));
+        }
+        if (cap) {  // checked after each launch of the graph
+            TRY(or_words(c, info, nw, c->cap->check + 1));
+            continue;
         }
         TRY(tri_check(c));
         TRY(check_info(c, info, nw));
     }
     return MH_OK;
+}
+
+void omega_tangents(const double* omegas, int64_t nomega, int64_t w0, int64_t nw, const double* domega,
+                    int64_t nparam, int Pg, double g, double* dk, double* dom) {
+    for (int64_t m0 = 0; m0 < nparam; m0 += Pg) {
+        double* bdk = dk + (m0 / Pg) * nw * Pg;
+        double* bdom = dom + (m0 / Pg) * nw * Pg;
+        for (int64_t w = 0; w < nw; ++w)
+            for (int m = 0; m < Pg; ++m) {
+                const double dw = (domega && m0 + m < nparam) ? domega[(m0 + m) * nomega + w0 + w] : 0.0;
+                bdom[w * Pg + m] = dw;
+                bdk[w * Pg + m] = 2.0 * omegas[w0 + w] / g * dw;
+            }
+    }
 }
 
 int sweep_args_ok(const mh_mesh_t* mesh, int gf_mask, int precision, int64_t nomega, const double* omegas,

@@ -69,20 +69,44 @@ struct Context {
     int64_t fallbacks = 0;
     // LU leaf width cap (0: the default width for the panel height; 4 or 8)
     int leaf_cap = 0;
-    // graph capture of an FP64 forward sweep (graph.hip): while set,
-    // sweep_impl records its device work on `stream` into cap_graph (the
-    // host-array copies, the memset and the host checks are left out and
-    // the pointers they need are kept here for the launches)
+    // graph capture (graph.hip): while set, sweep_impl only issues device
+    // work (its streams belong to a capture begun by the caller); the
+    // frequencies, wavenumbers, headings and frequency tangents come from
+    // the device buffers in `cap`, which the graph's owner fills before each
+    // launch, and the host checks become device check words in cap->check
     bool capturing = false;
-    hipGraph_t cap_graph = nullptr;
-    double *cap_om = nullptr, *cap_k = nullptr, *cap_be = nullptr;
-    int* cap_info = nullptr;
+    struct CapIO* cap = nullptr;
+    // refinement sweeps recorded in a captured mixed sweep (at least 2)
+    int ir_graph_sweeps = 2;
     // multi-body scheduler: worker contexts (each with a stream of its own)
     // and the configuration the autotuner chose (0: not tuned)
     bool owns_stream = false;
     std::vector<Context*> workers;
     int tuned_workers = 0, tuned_leaf = 0;
 };
+
+// Device inputs and check words of one captured sweep (graph.hip).  om, k:
+// nomega frequencies and wavenumbers; be: the headings; dk, dom: the
+// wavenumber and frequency tangents of every parameter block
+// (omega_tangents' layout); check: kCheckWords words cleared at the start of
+// the sweep, then [0] the mesh-index flag, [1] the OR of the LU info words,
+// [2] the refinement verdict (1: a column above the tolerance, 2: a
+// non-finite residual).  A nonzero word means the sweep must be run again
+// without the graph, which reports or repairs what happened.
+constexpr int kCheckWords = 4;
+struct CapIO {
+    double *om = nullptr, *k = nullptr, *be = nullptr, *dk = nullptr, *dom = nullptr;
+    int* check = nullptr;
+};
+
+// Tangent directions per gradient pass for nparam parameters (1, 2 or 4).
+inline int grad_block(int64_t nparam) { return nparam <= 1 ? 1 : (nparam == 2 ? 2 : 4); }
+// Wavenumber and frequency tangents of every parameter block for the
+// frequencies [w0, w0 + nw) of a sweep: dk and dom hold
+// ceil(nparam / Pg) blocks of nw x Pg values (zero past nparam);
+// dk = 2 omega / g * domega (k = omega^2 / g, src/waves.jl:30).
+void omega_tangents(const double* omegas, int64_t nomega, int64_t w0, int64_t nw, const double* domega,
+                    int64_t nparam, int Pg, double g, double* dk, double* dom);
 
 // Creates a context on `device` (current afterwards) using `stream`, or a
 // new non-blocking stream of default priority owned by the context when
@@ -162,6 +186,17 @@ int ensure_events(Context* ctx, size_t count);
 int launch_pack_panels(Context* ctx, const mh_mesh_t* mesh, PanelRec* out, int* flag_buf = nullptr);
 // p[0 .. count) = 0 by a fill kernel on ctx->stream (capturable, unlike a memset)
 int clear_ints(Context* ctx, int* p, int64_t count);
+// `bytes` (a multiple of 16, 16-byte aligned) of p set to zero by a fill kernel
+int fill_zero(Context* ctx, void* p, size_t bytes);
+// dst[i + j ldd] = src[i + j lds] for i < rows, j < cols, by a copy kernel:
+// the sweeps record no memcpy nodes when captured (a 1D device-to-device
+// copy recorded from hipMemcpyAsync kept zero pitches in its node and its
+// graph crashed the host at launch)
+int copy_d2(Context* ctx, double2* dst, int64_t ldd, const double2* src, int64_t lds, int64_t rows, int64_t cols);
+// *dst |= src[0] | ... | src[count - 1]  (one workgroup)
+int or_words(Context* ctx, const int* src, int64_t count, int* dst);
+// *dst |= 1 if some nrm[e] is not below tol, |= 2 if some is not finite
+int ir_verdict(Context* ctx, const double* nrm, int64_t count, double tol, int* dst);
 // Assembly in one kernel per launch, without the frequency-independent
 // scratch matrices launch_assemble keeps in the context.
 int launch_assemble_onepass(Context* ctx, const mh_mesh_t* mesh, const PanelRec* panels, int gf_mask,

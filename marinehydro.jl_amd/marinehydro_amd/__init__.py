@@ -25,6 +25,6 @@ from .problems import (FloatingBody, combine_floatingbodies, LinearPotentialFlow
                        solve_problem, solve_all_problems, assemble_hydrodynamic_coefficients,
                        compute_hydrodynamic_coefficients, create_DimStack, DimArray,
                        compute_and_label_hydrodynamic_coefficients)
-from .batches import SweepItem, multi_sweep, multi_autotune, random_body, random_workload
+from .batches import BatchGraph, SweepItem, multi_sweep, multi_autotune, random_body, random_workload
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -74,6 +74,11 @@ _SIGS = {
     "mh_sweep_graph_create": (_int, [_vp, _vp, _int, _int, _i64, _i64, _vp, _i64, _d, _d, _vp, _vp, _vp, _vp]),
     "mh_sweep_graph_launch": (_int, [_vp, _vp, _vp, _vp]),
     "mh_sweep_graph_destroy": (_int, [_vp]),
+    "mh_batch_graph_create": (_int, [_vp, _i64, _vp, _int, _d, _d, _int, _vp]),
+    "mh_batch_graph_launch": (_int, [_vp, _vp, _i64, _vp]),
+    "mh_batch_graph_info": (_int, [_vp, _vp, _vp, _vp]),
+    "mh_batch_graph_dot": (_int, [_vp, C.c_char_p]),
+    "mh_batch_graph_destroy": (_int, [_vp]),
     "mh_multi_autotune": (_int, [_vp, _i64, _vp, _int, _d, _d, C.POINTER(_int), C.POINTER(_int),
                                  C.POINTER(_d)]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

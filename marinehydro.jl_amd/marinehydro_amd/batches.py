@@ -127,6 +127,60 @@ def multi_autotune(items, precision="mixed", device=0, g=None, rho=None, prepare
     return w.value, lf.value, ms.value
 
 
+class BatchGraph:
+    """The items of multi_sweep recorded once as one HIP graph and replayed
+    by launch() (mh_batch_graph_*): FP64 or mixed precision, with or
+    without gradients, one body or many.  The items' device arrays (meshes,
+    dof fields, tangents, outputs) are recorded: their contents may change
+    between launches, not their sizes.  launch() takes new host frequencies,
+    headings and frequency tangents per item (None keeps the current ones)
+    and returns per item what multi_sweep returns.  Creation runs every
+    item once, then records; an item whose device checks fail at a launch
+    (mesh index, pivots, refinement verdict) runs again without the graph."""
+
+    def __init__(self, items, precision="mixed", nworkers=0, device=0, g=None, rho=None, prepared=None):
+        self.prep = prepared if prepared is not None else prepare(items, device)
+        self.arr = _array(self.prep)
+        self.device = device
+        prec = {"fp64": L.FP64, "mixed": L.MIXED}[precision]
+        self.handle = C.c_void_p()
+        L.check(L.lib().mh_batch_graph_create(L.context(device), len(self.prep), self.arr, prec,
+                                              float(g or SETTINGS.g), float(rho or SETTINGS.rho), int(nworkers),
+                                              C.byref(self.handle)), "batch_graph_create")
+
+    def launch(self, omegas=None, betas=None, domegas=None):
+        for i, p in enumerate(self.prep):
+            if omegas is not None and omegas[i] is not None:
+                p.om[:] = np.asarray(omegas[i], dtype=np.float64).reshape(p.om.shape)
+            if betas is not None and betas[i] is not None and p.nbeta:
+                p.be[:] = np.asarray(betas[i], dtype=np.float64).reshape(p.be.shape)
+            if domegas is not None and domegas[i] is not None:
+                if not hasattr(p, "dom"):
+                    raise ValueError(f"BatchGraph.launch: item {i} was recorded without frequency tangents")
+                p.dom[:] = np.asarray(domegas[i], dtype=np.float64).reshape(p.dom.shape)
+        L.check(L.lib().mh_batch_graph_launch(L.context(self.device), self.handle, len(self.prep), self.arr),
+                "batch_graph_launch")
+        return [p.results() for p in self.prep]
+
+    def info(self):
+        """(recorded nodes, items run again without the graph so far, workers)"""
+        nodes, reruns, workers = C.c_int64(0), C.c_int64(0), C.c_int(0)
+        L.check(L.lib().mh_batch_graph_info(self.handle, C.byref(nodes), C.byref(reruns), C.byref(workers)),
+                "batch_graph_info")
+        return nodes.value, reruns.value, workers.value
+
+    def close(self):
+        if self.handle:
+            L.lib().mh_batch_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # seeded uneven workload: spheres, vertical cylinders and boxes
 # ---------------------------------------------------------------------------
