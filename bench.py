@@ -311,6 +311,28 @@ def main():
         log(f"bench: rel difference {args.precision} vs FP64 per quantity " +
             str({k: f"{v:.1e}" for k, v in rel_fp64.items()}))
 
+    # the same sweep with the Rankine and image far field (pairs beyond 7
+    # panel radii) assembled in FP32 (mh_set_far_field_fp32): its time and
+    # its difference to the FP64 sweep per quantity
+    far32 = None
+    if rel_fp64 is not None:
+        try:
+            mh._lib.set_far_field_fp32(True)
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res32 = step()
+            torch.cuda.synchronize()
+            far_ms = (time.perf_counter() - t0) * 1e3
+        finally:
+            mh._lib.set_far_field_fp32(False)
+        far32 = {"ms_per_step": round(far_ms, 3),
+                 "rel_diff_vs_fp64": {name: float(np.max(np.abs(np.asarray(a) - np.asarray(b))) /
+                                                  max(float(np.max(np.abs(np.asarray(b)))), 1e-300))
+                                      for name, a, b in zip(("A", "B", "F", "dA", "dB", "dF"), res32, ref)}}
+        log(f"bench: FP32 far-field assembly: {far_ms:.1f} ms per sweep, rel difference vs FP64 " +
+            str({k: f"{v:.1e}" for k, v in far32["rel_diff_vs_fp64"].items()}))
+
     # kernel timer over one more sweep (the library serialises its two streams
     # while the timer is on, so each duration is the kernel's own): dominant
     # kernel and its roofline
@@ -409,6 +431,7 @@ def main():
         "rel_err_grad_vs_oracle_fd": grad_err,
         "fp64_forward_fd_sweep_ms": round(fd_ms, 3) if fd_ms else None,
         "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,
+        "far_field_fp32": far32,
         "assembly_matrix_writes": asm_write,
         "kernel_ms_per_step": kernel_ms,
         "uneven_batch": batch,

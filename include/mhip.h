@@ -149,6 +149,18 @@ int mh_solve_mixed(mh_context_t ctx, int64_t n, int64_t batch, int direct, doubl
  * always returned as MH_ERR_NOT_CONVERGED.  mh_fallback_count: chunks solved
  * again in FP64 since the context was created. */
 int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback);
+/* Far-field panel pairs in FP32 (no reference counterpart; the reference
+ * evaluates every pair in FP64): with on != 0, every assembly on ctx
+ * (mh_assemble, the sweeps) evaluates the Rankine and image terms of pairs
+ * beyond 7 panel radii (the far-field branch of src/green_functions/
+ * rankine.jl:33-34, :74-77: area / r and its gradient) in FP32, from offsets
+ * formed in FP64; the branch is decided on r^2 > 49 R^2.  Nearer pairs, the
+ * wave terms and everything after the assembly stay as they are.  Such an
+ * entry is within ~2e-7 of its FP64 value relative to its magnitude; the
+ * refinement of a mixed sweep then converges to the solution of these
+ * matrices, so the results differ from the FP64 path by that order instead
+ * of ~1e-13.  Default off; the gradient kernels always use FP64. */
+int mh_set_far_field_fp32(mh_context_t ctx, int on);
 int mh_fallback_count(mh_context_t ctx, int64_t* count);
 
 /* ---- Green functions ---------------------------------------------------

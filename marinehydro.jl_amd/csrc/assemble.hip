@@ -408,6 +408,7 @@ int launch_assemble(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, in
                     int64_t stride, float2* DK32) {
     const int64_t n = m->nfaces;
     if (n == 0 || nk == 0) return MH_OK;
+    if (ctx->far32) gf |= kGfFar32;  // read by the Rankine pair code only
     DelhTab tab{};
     if (gf & GF_DELH_TABLE) TRY_RC(get_delh_table(ctx, &tab));
     const int64_t tiles = (n + 63) / 64;

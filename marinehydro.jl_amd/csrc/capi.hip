@@ -1275,6 +1275,12 @@ int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback) {
     return MH_OK;
 }
 
+int mh_set_far_field_fp32(mh_context_t ctx, int on) {
+    TRY(check_ctx(ctx));
+    CTX(ctx)->far32 = on != 0;
+    return MH_OK;
+}
+
 int mh_fallback_count(mh_context_t ctx, int64_t* count) {
     TRY(check_ctx(ctx));
     MH_REQUIRE(count != nullptr, "fallback_count: null output");

@@ -31,6 +31,9 @@ enum : int {
     GF_DELH_TABLE = 32,  // the Delhommeau term from the table (delh_table.hpp)
 };
 constexpr int kGfWaveMask = GF_WU | GF_DELHOMMEAU | GF_DELH_TABLE;
+// Internal bit (never accepted from the C-ABI): the Rankine and image far
+// field of the FP64 pair code evaluated in FP32 (rankine_panel).
+constexpr int kGfFar32 = 1 << 8;
 
 // Packed source-panel record (one per face, 192 B), read wave-uniformly by
 // the pair kernels so it lands in scalar registers.
@@ -68,10 +71,28 @@ typedef cplxT<double> cplx;
 // PT: PanelRec (T = double) or PanelT<T>.
 // ---------------------------------------------------------------------------
 template <class T, class PT>
-__device__ __forceinline__ void rankine_panel(T x0, T x1, T x2, const PT& p, T& I, T g[3]) {
+__device__ __forceinline__ void rankine_panel(T x0, T x1, T x2, const PT& p, T& I, T g[3], bool far32 = false) {
     const T rb0 = x0 - p.c[0], rb1 = x1 - p.c[1], rb2 = x2 - p.c[2];
-    const T r = sqrt(rb0 * rb0 + rb1 * rb1 + rb2 * rb2);
     const double R = p.radius;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        // far field in FP32 (the offsets formed in FP64, then area / r and
+        // its gradient from the FP32 reciprocal square root of r^2): the same
+        // branch as below, decided on r^2 > 49 R^2 instead of r > 7 R
+        if (far32) {
+            const double r2 = rb0 * rb0 + rb1 * rb1 + rb2 * rb2;
+            if (r2 > 49.0 * R * R) {
+                const float ir = rsqrtf((float)r2);
+                const float a = (float)p.area;
+                I = (double)(a * ir);
+                const float s = (ir * ir) * (ir * a);
+                g[0] = (double)((float)rb0 * s);
+                g[1] = (double)((float)rb1 * s);
+                g[2] = (double)((float)rb2 * s);
+                return;
+            }
+        }
+    }
+    const T r = sqrt(rb0 * rb0 + rb1 * rb1 + rb2 * rb2);
     if (val(r) > 7.0 * R) {  // rankine.jl:33-34, :74-77
         const T ir = 1.0 / r;
         I = p.area * ir;
@@ -356,7 +377,7 @@ __device__ __forceinline__ void rankine_terms(int gf, T x0, T x1, T x2, const PT
     }
     T v, g[3];
     if (gf & GF_RANKINE) {  // rankine.jl:24-128
-        rankine_panel(x0, x1, x2, p, v, g);
+        rankine_panel(x0, x1, x2, p, v, g, (gf & kGfFar32) != 0);
         I += v;
         gxi[0] += g[0];
         gxi[1] += g[1];
@@ -367,7 +388,7 @@ __device__ __forceinline__ void rankine_terms(int gf, T x0, T x1, T x2, const PT
     }
     if (gf & (GF_RANKINE_REFLECTED | GF_RANKINE_REFLECTED_NEG)) {
         // collocation point reflected across z = 0 (rankine_reflected.jl:19-35)
-        rankine_panel(x0, x1, -x2, p, v, g);
+        rankine_panel(x0, x1, -x2, p, v, g, (gf & kGfFar32) != 0);
         const double s = ((gf & GF_RANKINE_REFLECTED) ? 1.0 : 0.0) -
                          ((gf & GF_RANKINE_REFLECTED_NEG) ? 1.0 : 0.0);
         I += s * v;

@@ -69,6 +69,9 @@ struct Context {
     int64_t fallbacks = 0;
     // LU leaf width cap (0: the default width for the panel height; 4 or 8)
     int leaf_cap = 0;
+    // assembly: the Rankine and image far field (pairs beyond 7 panel radii)
+    // in FP32 (mh_set_far_field_fp32)
+    bool far32 = false;
     // graph capture (graph.hip): while set, sweep_impl only issues device
     // work (its streams belong to a capture begun by the caller); the
     // frequencies, wavenumbers, headings and frequency tangents come from

@@ -79,6 +79,7 @@ _SIGS = {
     "mh_batch_graph_info": (_int, [_vp, _vp, _vp, _vp]),
     "mh_batch_graph_dot": (_int, [_vp, C.c_char_p]),
     "mh_batch_graph_destroy": (_int, [_vp]),
+    "mh_set_far_field_fp32": (_int, [_vp, _int]),
     "mh_multi_autotune": (_int, [_vp, _i64, _vp, _int, _d, _d, C.POINTER(_int), C.POINTER(_int),
                                  C.POINTER(_d)]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -193,6 +194,12 @@ def set_refinement(max_sweeps=12, fp64_fallback=True, device=0):
     does not converge is solved again in FP64 (mh_set_refinement)."""
     check(lib().mh_set_refinement(context(device), int(max_sweeps), int(bool(fp64_fallback))),
           "set_refinement")
+
+
+def set_far_field_fp32(on=True, device=0):
+    """Rankine and image far-field pairs (beyond 7 panel radii) of every
+    assembly on the device's context in FP32 (mh_set_far_field_fp32)."""
+    check(lib().mh_set_far_field_fp32(context(device), int(bool(on))), "set_far_field_fp32")
 
 
 def fallback_count(device=0) -> int:
