@@ -22,6 +22,12 @@ import os
 import sys
 import time
 
+# One hardware queue per HIP stream: the multi-body scheduler runs up to 6
+# worker contexts of 3 streams each, and HIP multiplexes streams onto
+# GPU_MAX_HW_QUEUES queues (4 by default), which serialises streams that
+# share a queue.  Set before HIP initialises (the import of torch).
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "marinehydro.jl_amd"))
 
@@ -142,13 +148,15 @@ def uneven_batch(mh, torch, precision):
     and boxes of 200-3,000 panels, 16 random frequencies each, heave and
     surge, heading 0, gradients w.r.t. each body's dimensions, in one
     scheduled call (mh_multi_sweep; worker streams, longest body first).
-    The autotuner (worker count, LU leaf width) on the 16 largest bodies
-    first, which also sizes the worker contexts' workspaces for the largest
-    body; then one timed call on all 256, inputs on the device."""
+    The autotuner (worker count, LU leaf width) first, on every 8th body in
+    order of size (32 bodies spanning the panel counts, the largest
+    included, which also sizes the worker contexts' workspaces for the
+    largest body); then one timed call on all 256, inputs on the device."""
     items = mh.random_workload(5, 256)
     prep = mh.batches.prepare(items)
-    largest = sorted(items, key=lambda it: -it.mesh.nfaces)[:16]
-    workers, leaf, tune_ms = mh.multi_autotune(largest, precision)
+    by_size = sorted(items, key=lambda it: -it.mesh.nfaces)
+    sample = by_size[::8]
+    workers, leaf, tune_ms = mh.multi_autotune(sample, precision)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mh.multi_sweep(items, precision, prepared=prep)
@@ -350,22 +358,44 @@ def main():
     dom_name, dom = max(mfma.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     pkey, mult = MFMA_KERNELS[dom_name]
-    # achieved = algorithmic complex-GEMM flops (8 per complex multiply-add)
-    # per second, against the peak of the unit that executes them: for the
-    # split-fp16 kernel that is the fp16 matrix peak, although every real
-    # product costs it three fp16 products
-    alg = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(alg, 3), "peak": PEAK[pkey], "unit": "TFLOP/s",
-            "frac": round(alg / PEAK[pkey], 4), "traffic": None}
+    # The roofline model: the kernel's arithmetic intensity (algorithmic
+    # complex-GEMM flops, 8 per complex multiply-add, over its algorithmic HBM
+    # bytes: C read and written once, the operands once) against the ridge
+    # point of the unit that executes it (its peak in algorithmic flops over
+    # 8 TB/s).  Below the ridge the kernel is bound by HBM and its roofline is
+    # algorithmic bytes per second against 8 TB/s; above it, algorithmic
+    # flops per second against the unit's peak.  Both views are reported.
+    # For the split-fp16 kernel the unit is the fp16 matrix core, whose peak
+    # in algorithmic flops is a third of 2.5 PFLOP/s (three fp16 products per
+    # real product).
+    sec = dom["ms"] * 1e-3
+    alg = dom["flops"] / sec / 1e12  # TFLOP/s
+    gbs = dom["bytes"] / sec / 1e9  # GB/s
+    intensity = dom["flops"] / dom["bytes"]
+    ridge = PEAK[pkey] / mult * 1e12 / (HBM_PEAK * 1e9)
+    mfma_view = {"achieved_tflops": round(alg, 3), "peak_tflops": PEAK[pkey], "frac": round(alg / PEAK[pkey], 4),
+                 "frac_of_algorithmic_peak": round(alg / (PEAK[pkey] / mult), 4)}
+    hbm_view = {"achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK, "frac": round(gbs / HBM_PEAK, 4)}
+    if intensity < ridge:
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK, 4), "traffic": None}
+    else:
+        roof = {"bound": "mfma", "achieved": round(alg, 3), "peak": PEAK[pkey], "unit": "TFLOP/s",
+                "frac": round(alg / PEAK[pkey], 4), "traffic": None}
     roof["kernel"] = dom_name
     roof["avg_launch_ms"] = round(avg_ms, 4)
     roof["launches_per_step"] = dom["launches"]
     roof["algorithmic_flops_per_launch"] = round(dom["flops"] / dom["launches"], 1)
+    roof["algorithmic_bytes_per_launch"] = round(dom["bytes"] / dom["launches"], 1)
+    roof["arithmetic_intensity_flop_per_byte"] = round(intensity, 2)
+    roof["ridge_flop_per_byte"] = round(ridge, 2)
+    roof["mfma_view"] = mfma_view
+    roof["hbm_view"] = hbm_view
     if mult != 1:
         roof["frac_of_fp32_matrix_peak"] = round(alg / PEAK["fp32"], 4)
         roof["issued_fp16_tflops"] = round(mult * alg, 3)
         roof["note"] = (f"FP32 operands split into fp16 halves, {mult} fp16 products per real product; "
-                        "achieved and frac count algorithmic flops only")
+                        "achieved flops count algorithmic flops only")
     roof["longest_kernel"] = {"name": top_name, "ms_per_step": round(top["ms"], 3)}
     vpath = os.path.join(ROOT, "profiles", "valu.json")
     if top_name != dom_name and os.path.exists(vpath):

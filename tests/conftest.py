@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+# one hardware queue per HIP stream for the multi-body scheduler's worker
+# streams (HIP's default of 4 serialises streams that share a queue); set
+# before HIP initialises
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "marinehydro.jl_amd")):
     if p not in sys.path:
