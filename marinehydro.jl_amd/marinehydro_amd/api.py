@@ -103,24 +103,48 @@ class TabulatedDelhommeau(GreensFunction):
     mask = L.GF_DELHOMMEAU_TABLE
 
 
-def gf_mask(gfs) -> int:
-    """Bit mask of a GreensFunction or an iterable of them (the reference
-    sums over tuples/vectors, abstract_greens_function.jl:18-83)."""
+_WAVE_BITS = L.GF_WU | L.GF_DELHOMMEAU | L.GF_DELHOMMEAU_TABLE
+
+
+def gf_layers(gfs) -> list:
+    """A GreensFunction or any tuple/vector of them (the reference sums over
+    the elements, abstract_greens_function.jl:18-83, repeats and several wave
+    terms included) as a list of masks whose sum is the tuple: each mask
+    holds every term at most once and at most one wave term, which one
+    library call evaluates.  Most tuples are one mask."""
     if isinstance(gfs, GreensFunction):
-        return gfs.mask
-    m = 0
+        return [gfs.mask]
+    counts = {}
     for g in gfs:
         if not isinstance(g, GreensFunction):
             raise TypeError(f"not a GreensFunction: {g!r}")
-        if m & g.mask:
-            raise ValueError(f"{g!r} appears twice in the Green-function tuple")
-        m |= g.mask
-    w = m & (L.GF_WU | L.GF_DELHOMMEAU | L.GF_DELHOMMEAU_TABLE)
-    if w & (w - 1):
-        raise ValueError("choose one wave term (GFWu, ExactGuevelDelhommeau or TabulatedDelhommeau)")
-    if m == 0:
+        counts[g.mask] = counts.get(g.mask, 0) + 1
+    if not counts:
         raise ValueError("empty Green-function tuple")
-    return m
+    layers = []
+    while any(counts.values()):
+        m = 0
+        for bit in sorted(counts):
+            if counts[bit] and not (bit & _WAVE_BITS and m & _WAVE_BITS):
+                m |= bit
+                counts[bit] -= 1
+        layers.append(m)
+    return layers
+
+
+def gf_mask(gfs) -> int:
+    """Bit mask of a GreensFunction or a tuple of them that one library call
+    evaluates: every term at most once and at most one wave term.  Other
+    tuples are summed term group by term group where the result is linear
+    in the Green function (gf_eval and the functions on it,
+    assemble_matrices, assemble_matrices_vjp); the sweeps need a single
+    mask."""
+    layers = gf_layers(gfs)
+    if len(layers) > 1:
+        raise ValueError("this path takes each Green-function term at most once and at most one wave term "
+                         "(GFWu, ExactGuevelDelhommeau or TabulatedDelhommeau); assemble_matrices and "
+                         "solve accept any tuple")
+    return layers[0]
 
 
 # ---------------------------------------------------------------------------
@@ -191,10 +215,13 @@ def gf_eval(gfs, points, panels, wavenumber=1.0, op=L.OP_INTEGRAL, wrt_first=Fal
     grad = op in (L.OP_GRADIENT_GREENS, L.OP_INTEGRAL_GRADIENT)
     out = torch.empty((3 if grad else 1, npairs), dtype=torch.complex128, device=dev)
     ctx = L.context(device)
-    L.check(L.lib().mh_gf_eval(ctx, gf_mask(gfs), op, int(bool(wrt_first)), npairs,
-                               d_pts.data_ptr(), d_pan.data_ptr(), float(wavenumber),
-                               out.data_ptr()), "gf_eval")
-    res = out.cpu().numpy()
+    res = None
+    for m in gf_layers(gfs):  # the sum over the tuple, term group by term group
+        L.check(L.lib().mh_gf_eval(ctx, m, op, int(bool(wrt_first)), npairs,
+                                   d_pts.data_ptr(), d_pan.data_ptr(), float(wavenumber),
+                                   out.data_ptr()), "gf_eval")
+        part = out.cpu().numpy()
+        res = part if res is None else res + part
     return res.T if grad else res[0]
 
 
@@ -296,10 +323,24 @@ def assemble_matrices_batched(green_functions, mesh, wavenumbers, direct=True, d
     D = _colmajor_empty(torch, n, nk, dm.device) if form & L.DIRECT else None
     K = _colmajor_empty(torch, n, nk, dm.device) if form & L.INDIRECT else None
     ctx = L.context(dm.device)
-    L.check(L.lib().mh_assemble(ctx, C.byref(dm.struct), gf_mask(green_functions), form, nk,
-                                ks.ctypes.data, S.data_ptr(), D.data_ptr() if D is not None else None,
-                                K.data_ptr() if K is not None else None, n, n * n),
-            "assemble_matrices")
+    layers = gf_layers(green_functions)
+    acc = None
+    for q, m in enumerate(layers):  # the sum over the tuple, term group by term group
+        L.check(L.lib().mh_assemble(ctx, C.byref(dm.struct), m, form, nk,
+                                    ks.ctypes.data, S.data_ptr(), D.data_ptr() if D is not None else None,
+                                    K.data_ptr() if K is not None else None, n, n * n),
+                "assemble_matrices")
+        parts = [X for X in (S, D, K) if X is not None]
+        if len(layers) > 1:
+            acc = [X.clone() for X in parts] if acc is None else [a + X for a, X in zip(acc, parts)]
+    if acc is not None:
+        # D and K carry the free term 1/2 on the diagonal once, not per group
+        # (src/matrix_assembly.jl:18, :23)
+        S, rest = acc[0], acc[1:]
+        eye = torch.eye(n, dtype=torch.complex128, device=S.device) * (0.5 * (len(layers) - 1))
+        rest = [X - eye for X in rest]
+        D = rest.pop(0) if D is not None else None
+        K = rest.pop(0) if K is not None else None
     tr = lambda X: X.transpose(1, 2)
     if form == L.BOTH:
         return tr(S), tr(D), tr(K)
@@ -767,17 +808,22 @@ def assemble_matrices_vjp(green_functions, mesh, wavenumbers, Sbar, DKbar=None, 
     Sb = _cotangent_batch(torch, Sbar, n, nk, dm.device)
     Db = _cotangent_batch(torch, DKbar, n, nk, dm.device)
     dev = f"cuda:{dm.device}"
-    dV = torch.zeros((3, nv), dtype=torch.float64, device=dev)
-    dC = torch.zeros((3, n), dtype=torch.float64, device=dev)
-    dN = torch.zeros((3, n), dtype=torch.float64, device=dev)
-    dA = torch.zeros(n, dtype=torch.float64, device=dev)
-    dk = torch.zeros(nk, dtype=torch.float64, device=dev)
     form = L.DIRECT if direct else L.INDIRECT
-    L.check(L.lib().mh_assemble_vjp(L.context(dm.device), C.byref(dm.struct), gf_mask(green_functions), form,
-                                    nk, ks.ctypes.data, Sb.data_ptr() if Sb is not None else None,
-                                    Db.data_ptr() if Db is not None else None, n, n * n, dV.data_ptr(),
-                                    dC.data_ptr(), dN.data_ptr(), dA.data_ptr(), dk.data_ptr()),
-            "assemble_matrices_vjp")
+    total = None
+    for m in gf_layers(green_functions):  # the pullbacks of the term groups add up
+        dV = torch.zeros((3, nv), dtype=torch.float64, device=dev)
+        dC = torch.zeros((3, n), dtype=torch.float64, device=dev)
+        dN = torch.zeros((3, n), dtype=torch.float64, device=dev)
+        dA = torch.zeros(n, dtype=torch.float64, device=dev)
+        dk = torch.zeros(nk, dtype=torch.float64, device=dev)
+        L.check(L.lib().mh_assemble_vjp(L.context(dm.device), C.byref(dm.struct), m, form,
+                                        nk, ks.ctypes.data, Sb.data_ptr() if Sb is not None else None,
+                                        Db.data_ptr() if Db is not None else None, n, n * n, dV.data_ptr(),
+                                        dC.data_ptr(), dN.data_ptr(), dA.data_ptr(), dk.data_ptr()),
+                "assemble_matrices_vjp")
+        part = (dV, dC, dN, dA, dk)
+        total = part if total is None else tuple(a + b for a, b in zip(total, part))
+    dV, dC, dN, dA, dk = total
     return {"vertices": dV.cpu().numpy().T.copy(), "centers": dC.cpu().numpy().T.copy(),
             "normals": dN.cpu().numpy().T.copy(), "areas": dA.cpu().numpy(),
             "wavenumbers": dk.cpu().numpy()}

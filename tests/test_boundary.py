@@ -101,3 +101,24 @@ def test_rigid_dofs(mh):
     assert np.all(d[0] == [0, 0, 1])
     pos = m.centers - np.array([1.0, 1.0, 0.0])
     assert np.allclose(d[1], np.cross([0, 1, 0], pos))
+
+
+def test_green_function_tuples_split_into_library_masks(mh):
+    """gf_layers: any tuple as masks that one library call evaluates (each
+    term at most once, at most one wave term), summing to the tuple."""
+    R, RR, W, D = mh.Rankine(), mh.RankineReflected(), mh.GFWu(), mh.ExactGuevelDelhommeau()
+    assert mh.api.gf_layers(W) == [8]
+    assert mh.api.gf_layers((R, RR, W)) == [1 | 2 | 8]
+    assert mh.api.gf_layers((R, R)) == [1, 1]
+    layers = mh.api.gf_layers((R, RR, W, D))
+    assert sorted(layers) == sorted([1 | 2 | 8, 16]) or sorted(layers) == sorted([1 | 2 | 16, 8])
+    assert sum(bin(x).count("1") for x in mh.api.gf_layers((R, R, RR, W, W, D))) == 6
+    for x in mh.api.gf_layers((R, R, RR, W, W, D)):
+        assert bin(x & (8 | 16 | 32)).count("1") <= 1
+    assert mh.api.gf_mask((R, RR, W)) == 11
+    with pytest.raises(ValueError, match="at most once"):
+        mh.api.gf_mask((R, R))
+    with pytest.raises(ValueError, match="empty"):
+        mh.api.gf_layers(())
+    with pytest.raises(TypeError):
+        mh.api.gf_layers((R, 3))

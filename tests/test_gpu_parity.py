@@ -424,3 +424,43 @@ def test_2048_panel_assembly_and_solve_properties(mh, oracle, gpu):
         res = Aq @ x[q].transpose(0, 1) - b[q].transpose(0, 1)
         rel = (torch.linalg.norm(res) / torch.linalg.norm(b[q])).item()
         assert rel < 1e-12, rel
+
+
+def test_any_green_function_tuple_sums_like_the_reference(mh, oracle, gpu):
+    """The reference sums any tuple of Green functions
+    (abstract_greens_function.jl:18-83), repeats and several wave terms
+    included.  Two wave terms against the oracle's sum of the same terms;
+    a repeated term against twice the oracle's term (the free term 1/2 of D
+    and K counted once); the pullback of a two-wave-term tuple against the
+    sum of the pullbacks of its parts; the sweep refuses such a tuple."""
+    m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
+    k = 0.8
+    both = (mh.Rankine(), mh.RankineReflected(), mh.GFWu(), mh.ExactGuevelDelhommeau())
+    for direct in (True, False):
+        S, DK = mh.assemble_matrices(both, m, k, direct=direct)
+        So, DKo = oracle.assemble(oracle.RANKINE | oracle.RANKINE_REFLECTED | oracle.WU | oracle.DELHOMMEAU,
+                                  m, k, direct)
+        assert relmax(S.cpu().numpy(), So) < 1e-11 and relmax(DK.cpu().numpy(), DKo) < 1e-11, direct
+        twice = (mh.Rankine(), mh.RankineReflected(), mh.Rankine())
+        S2, DK2 = mh.assemble_matrices(twice, m, k, direct=direct)
+        Sa, DKa = oracle.assemble(oracle.RANKINE | oracle.RANKINE_REFLECTED, m, k, direct)
+        Sb, DKb = oracle.assemble(oracle.RANKINE, m, k, direct)
+        eye = np.eye(m.nfaces)
+        assert relmax(S2.cpu().numpy(), Sa + Sb) < 1e-12
+        assert relmax(DK2.cpu().numpy(), DKa + DKb - 0.5 * eye) < 1e-12
+    # pair evaluations: a repeated term is twice the term
+    e1, e2 = mh.element(m, 3), mh.element(m, 17)
+    one = mh.integral(mh.Rankine(), e1, e2)
+    two = mh.integral((mh.Rankine(), mh.Rankine()), e1, e2)
+    assert abs(two - 2 * one) <= 1e-14 * abs(one)
+    # the pullback is linear in the tuple
+    rng = np.random.default_rng(2)
+    Sbar = rng.normal(size=(m.nfaces, m.nfaces)) + 1j * rng.normal(size=(m.nfaces, m.nfaces))
+    Dbar = rng.normal(size=(m.nfaces, m.nfaces)) + 1j * rng.normal(size=(m.nfaces, m.nfaces))
+    got = mh.assemble_matrices_vjp(both, m, [k], Sbar, Dbar)
+    pa = mh.assemble_matrices_vjp((mh.Rankine(), mh.RankineReflected(), mh.GFWu()), m, [k], Sbar, Dbar)
+    pb = mh.assemble_matrices_vjp((mh.ExactGuevelDelhommeau(),), m, [k], Sbar, Dbar)
+    for key in got:
+        assert relmax(got[key], pa[key] + pb[key]) < 1e-13, key
+    with pytest.raises(ValueError, match="at most once"):
+        mh.freq_sweep(m, [1.0], [[0, 0, 1.0]], green_functions=both)
