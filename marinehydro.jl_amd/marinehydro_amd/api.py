@@ -490,6 +490,30 @@ class _Outputs:
         return (a.view(np.complex128) if cplx else a).reshape(shape)
 
 
+def device_dof_fields(mesh, dof_fields, device=0):
+    """The dof fields (3-vectors or (nfaces, 3) fields) as one device tensor
+    (ndof, 3, nfaces), the layout the sweeps read: pass it as dof_fields to
+    keep them resident across calls."""
+    torch = _torch()
+    n = mesh.nfaces
+    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+    return torch.from_numpy(np.ascontiguousarray(np.stack([f.T for f in fields]))).to(f"cuda:{device}")
+
+
+def _dof_input(torch, dof_fields, n, dev):
+    """(device tensor (ndof, 3, n), ndof) from host fields or a tensor of
+    device_dof_fields."""
+    if torch.is_tensor(dof_fields):
+        if dof_fields.dim() != 3 or dof_fields.shape[1:] != (3, n) or dof_fields.dtype != torch.float64:
+            raise ValueError(f"device dof fields must be float64 (ndof, 3, {n})")
+        return dof_fields.to(dev).contiguous(), dof_fields.shape[0]
+    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
+    ndof = len(fields)
+    dofs = torch.from_numpy(np.ascontiguousarray(
+        np.stack([f.T for f in fields]) if ndof else np.zeros((1, 3, n)))).to(dev)
+    return dofs, ndof
+
+
 def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=True,
                precision="fp64", device=0, g=None, rho=None):
     """All frequencies in one batched pass (one LU per frequency, shared by
@@ -506,11 +530,9 @@ def freq_sweep(mesh, omegas, dof_fields, betas=(), green_functions=None, direct=
     gfs = green_functions if green_functions is not None else (Rankine(), RankineReflected(), GFWu())
     om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
     be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
-    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
-    ndof, nbeta, nw = len(fields), be.shape[0], om.shape[0]
     dev = f"cuda:{dm.device}"
-    dofs = torch.from_numpy(np.ascontiguousarray(
-        np.stack([f.T for f in fields]) if ndof else np.zeros((1, 3, n)))).to(dev)
+    dofs, ndof = _dof_input(torch, dof_fields, n, dev)
+    nbeta, nw = be.shape[0], om.shape[0]
     nd = max(ndof, 1)
     out = _Outputs(torch, dev, [("A", (nw, nd, nd), False), ("B", (nw, nd, nd), False),
                                 ("F", (nw, max(nbeta, 1), nd), True)])
@@ -660,11 +682,11 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
     gfs = green_functions if green_functions is not None else (Rankine(), RankineReflected(), GFWu())
     om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).reshape(-1))
     be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(-1))
-    fields = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, 3)) for f in dof_fields]
-    ndof, nbeta, nw = len(fields), be.shape[0], om.shape[0]
+    dev = f"cuda:{dm.device}"
+    dofs, ndof = _dof_input(torch, dof_fields, n, dev)
+    nbeta, nw = be.shape[0], om.shape[0]
     T = tangents
     npar = T.nparam
-    dev = f"cuda:{dm.device}"
 
     def up(a, shape):  # (nparam, n, c) -> column-major per parameter
         if a is None:
@@ -684,7 +706,6 @@ def freq_sweep_grad(mesh, omegas, dof_fields, betas=(), tangents: Tangents = Non
     dA_ = up(T.dareas, (npar, n))
     ddof = up(T.ddof_fields, (npar, ndof, n, 3))
     dom = None if T.domegas is None else np.ascontiguousarray(np.asarray(T.domegas, dtype=np.float64).reshape(npar, nw))
-    dofs = torch.from_numpy(np.ascontiguousarray(np.stack([f.T for f in fields]))).to(dev)
     mp, nb = max(npar, 1), max(nbeta, 1)
     out = _Outputs(torch, dev, [("A", (nw, ndof, ndof), False), ("B", (nw, ndof, ndof), False),
                                 ("F", (nw, nb, ndof), True), ("gA", (nw, mp, ndof, ndof), False),
@@ -764,6 +785,21 @@ def froude_krylov_grad(mesh, omegas, dof_fields, betas=(0.0,), tangents: "Tangen
         out.ptr("dFK")), "froude_krylov_grad")
     return (np.ascontiguousarray(out.get("FK")[:, :nb, :nd]),
             np.ascontiguousarray(out.get("dFK")[:, :npar, :nb, :nd]))
+
+
+def device_tangents(tangents, mesh, ndof, device=0):
+    """The tangents with their mesh and dof-field arrays on the device in the
+    layout the sweeps read (domegas stays a host array), so that repeated
+    sweeps find them resident."""
+    torch = _torch()
+    T = tangents
+    n, nv, npar = mesh.nfaces, mesh.nvertices, T.nparam
+    dev = f"cuda:{device}"
+    return Tangents(npar, _dev_tangent(torch, T.dvertices, (npar, nv, 3), dev),
+                    _dev_tangent(torch, T.dcenters, (npar, n, 3), dev),
+                    _dev_tangent(torch, T.dnormals, (npar, n, 3), dev),
+                    _dev_tangent(torch, T.dareas, (npar, n), dev),
+                    _dev_tangent(torch, T.ddof_fields, (npar, ndof, n, 3), dev), T.domegas)
 
 
 def _dev_tangent(torch, a, shape, dev):
