@@ -217,6 +217,8 @@ def main():
                     help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven multi-body batch (256 bodies of 200-3,000 panels)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the uneven multi-body batch (autotuner, then one timed call); for rocprofv3 runs")
     ap.add_argument("--delhommeau-assembly", action="store_true",
                     help="only the assembly of the 4,000-panel cylinder (R 5 m, draft 10 m) at 32 wavenumbers "
                          "with the exact and the tabulated Delhommeau term (timings and their difference)")
@@ -246,6 +248,9 @@ def main():
         return
     if args.delhommeau_assembly:
         delhommeau_assembly(mh, torch, np)
+        return
+    if args.batch_only:
+        log("bench: " + json.dumps(uneven_batch(mh, torch, args.precision)))
         return
     rho, g = 1025.0, 9.81
     body = mh.hemisphere(1.0, ntheta=args.ntheta, nphi=args.nphi)
