@@ -173,6 +173,67 @@ def uneven_batch(mh, torch, precision):
             "precision": precision, "workers": workers, "leaf_width_cap": leaf}
 
 
+def shape_modes(U, count=16):
+    """Radial shape modes of a sphere's vertices: unit directions U times
+    low-order monomials of U (1, x, y, z, x^2, ...), `count` of them."""
+    import numpy as np
+
+    x, y, z = U[:, 0], U[:, 1], U[:, 2]
+    f = [np.ones_like(x), x, y, z, x * x, y * y, z * z, x * y, x * z, y * z, x ** 3, y ** 3, z ** 3,
+         x * y * z, x * x * z, y * y * z]
+    return [U * fm[:, None] for fm in f[:count]]
+
+
+def many_parameters(mh, torch, args, omegas, dofs, betas, g, rho, nparam=16):
+    """The same sweep with gradients w.r.t. nparam shape parameters (radial
+    modes of the hemisphere's vertices) against the FP64 forward + central
+    finite-difference sweep that the adjoint replaces (2 nparam + 1 forward
+    sweeps on meshes already on the device), and the adjoint gradients'
+    difference to those finite differences (h = 1e-6)."""
+    import numpy as np
+
+    body = mh.hemisphere(1.0, ntheta=args.ntheta, nphi=args.nphi)
+    mesh = body.mesh
+    modes = shape_modes(body.dvertices["radius"], nparam)
+    T = mh.shape_tangents(mesh, modes)
+    dm = mh.DeviceMesh(mesh)
+    dofs_dev = mh.device_dof_fields(mesh, dofs)
+    T_dev = mh.device_tangents(T, mesh, len(dofs))
+
+    def adj():
+        return mh.freq_sweep_grad(dm, omegas, dofs_dev, betas, T_dev, precision=args.precision, g=g, rho=rho)
+
+    adj()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        res = adj()
+    torch.cuda.synchronize()
+    adj_ms = (time.perf_counter() - t0) / 2 * 1e3
+    h = 1e-6
+    V0, F = mesh.vertices, mesh.faces
+    pert = [mh.DeviceMesh(mh.mesh_from_vertices(V0 + sg * h * dv, F)) for dv in modes for sg in (1, -1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mh.freq_sweep(dm, omegas, dofs_dev, betas, precision="fp64", g=g, rho=rho)
+    outs = [mh.freq_sweep(pm, omegas, dofs_dev, betas, precision="fp64", g=g, rho=rho) for pm in pert]
+    torch.cuda.synchronize()
+    fd_ms = (time.perf_counter() - t0) * 1e3
+    err = {}
+    for c, key in enumerate(("dA", "dB", "dF")):
+        fd = np.stack([(outs[2 * m][c] - outs[2 * m + 1][c]) / (2 * h) for m in range(nparam)], axis=1)
+        got = res[3 + c][:, :nparam]
+        err[key] = float(np.max(np.abs(got - fd)) / max(float(np.max(np.abs(fd))), 1e-300))
+    log(f"bench: {nparam} shape parameters: adjoint sweep {adj_ms:.1f} ms, FP64 forward + central-FD sweep "
+        f"({2 * nparam + 1} forward sweeps) {fd_ms:.1f} ms ({fd_ms / adj_ms:.1f}x); adjoint vs FD " +
+        str({k: f"{v:.1e}" for k, v in err.items()}))
+    return {"parameters": f"{nparam} radial shape modes of the hemisphere's vertices (unit direction x "
+                          "monomials of degree <= 3)",
+            "adjoint_sweep_ms": round(adj_ms, 3), "fp64_forward_fd_sweep_ms": round(fd_ms, 3),
+            "forward_sweeps_in_fd": 2 * nparam + 1, "speedup": round(fd_ms / adj_ms, 3),
+            "rel_err_adjoint_vs_fd": err}
+
+
 def delhommeau_assembly(mh, torch, np):
     """S and D of the 4,000-panel cylinder at 32 wavenumbers (omega 0.2-2.5
     rad/s), exact and tabulated Delhommeau term, direct formulation: wall
@@ -217,6 +278,8 @@ def main():
                     help="only a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven multi-body batch (256 bodies of 200-3,000 panels)")
+    ap.add_argument("--no-many", action="store_true",
+                    help="skip the 16-parameter adjoint vs finite-difference comparison")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the uneven multi-body batch (autotuner, then one timed call); for rocprofv3 runs")
     ap.add_argument("--delhommeau-assembly", action="store_true",
@@ -434,6 +497,8 @@ def main():
     kernel_ms = {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}
     log(f"bench: kernel ms per sweep {kernel_ms}")
 
+    many = many_parameters(mh, torch, args, omegas, dofs, betas, g, rho) if grad and not args.no_many else None
+
     batch = None if args.no_batch else uneven_batch(mh, torch, args.precision)
 
     cpu = None
@@ -469,6 +534,7 @@ def main():
         "rel_err_grad_vs_oracle_fd": grad_err,
         "fp64_forward_fd_sweep_ms": round(fd_ms, 3) if fd_ms else None,
         "speedup_vs_fp64_fd_sweep": round(fd_ms / (dt * 1e3), 3) if fd_ms else None,
+        "many_parameters": many,
         "far_field_fp32": far32,
         "assembly_matrix_writes": asm_write,
         "kernel_ms_per_step": kernel_ms,

@@ -161,6 +161,16 @@ int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback);
  * matrices, so the results differ from the FP64 path by that order instead
  * of ~1e-13.  Default off; the gradient kernels always use FP64. */
 int mh_set_far_field_fp32(mh_context_t ctx, int on);
+/* How mh_freq_sweep_grad differentiates the wave-term pair sums (no
+ * reference counterpart; the reference takes Zygote's reverse mode through
+ * the whole path): 1 = forward tangents, one pass over the panel pairs per
+ * block of at most 4 parameters (and 4 dofs x 4 right-hand sides); 2 =
+ * reverse, one pass per block of 2 outputs (dof x right-hand side) that
+ * leaves cotangents of every panel's centre, normal and area and of k,
+ * contracted afterwards with any number of parameter tangents; 0 (default)
+ * = whichever needs fewer passes, a reverse pass counted as 1.5.  Both give
+ * the same derivatives up to rounding. */
+int mh_set_gradient_mode(mh_context_t ctx, int mode);
 int mh_fallback_count(mh_context_t ctx, int64_t* count);
 
 /* ---- Green functions ---------------------------------------------------

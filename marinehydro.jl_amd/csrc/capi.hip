@@ -809,14 +809,26 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                                                               : gf_mask;
     const int64_t tiles = (n + 63) / 64;
     const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
+    // reverse mode of the wave-term pair sums when it takes fewer pair-kernel
+    // launches (output blocks of 2) than the forward tangents (parameter
+    // blocks of 4 times dof and right-hand-side blocks of 4), counting a
+    // reverse launch as 1.5 forward ones
+    bool rev = false;
+    if (ga && ndof && wave_mask) {
+        const int64_t fwd = nblocks * ((ndof + 3) / 4) * ((nr + 3) / 4);
+        const int64_t bwd = (ndof * nr + 1) / 2;
+        rev = c->grad_mode == 2 || (c->grad_mode == 0 && 3 * bwd < 2 * fwd);
+    }
+    size_t o_rev = 0;
     if (ga) {
+        if (rev) o_rev = bump.take((size_t)wave_cot_ws_bytes(n, chunk, ndof, nr));
         o_cw = bump.take(sizeof(double2) * (size_t)(wa * chunk));
         o_lam = bump.take(sizeof(double2) * (size_t)(wa * chunk));
         o_mu = bump.take(sizeof(double2) * (size_t)(wa * chunk));
         o_tan = bump.take(sizeof(double) * (size_t)(n * 19 * Pg));
         o_dk = bump.take(sizeof(double) * (size_t)(nblocks * chunk * Pg));
         o_dom = bump.take(sizeof(double) * (size_t)(nblocks * chunk * Pg));
-        o_part = bump.take(sizeof(double2) * (size_t)(chunk * tiles * tiles * nout));
+        if (!rev) o_part = bump.take(sizeof(double2) * (size_t)(chunk * tiles * tiles * nout));
         o_T1 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
         o_T2 = bump.take(sizeof(double2) * (size_t)(chunk * nout));
         if (rank_mask) {
@@ -1040,6 +1052,11 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             d_dk = c->cap->dk;
             d_dom = c->cap->dom;
         }
+        // reverse mode: the wave-term cotangents once for every parameter
+        double2 *cot = nullptr, *cotk = nullptr;
+        if (rev)
+            TRY(launch_wave_cot(c, mesh, P, wave_mask, form, nw, d_k + w0, ndof, nr, aS, aDw, sS, sD, wa, wss,
+                                ws + o_rev, &cot, &cotk));
         for (int64_t m0 = 0; m0 < np; m0 += Pg) {
             const double* bdk = d_dk + (m0 / Pg) * nw * Pg;
             const double* bdom = d_dom + (m0 / Pg) * nw * Pg;
@@ -1057,11 +1074,15 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                 TRY(launch_rankine_contract(c, n, Pg, nw, ndof, nr, dR, aS, aDw, sS, sD, wa, wss, Yb, T2));
             }
             int64_t nblk = 0;
-            if (wave_mask)
+            if (wave_mask && !rev)
                 TRY(launch_tangent_pairs(c, mesh, P, tan, Pg, wave_mask, form, nw, d_k + w0, bdk, ndof, nr, aS,
                                          aDw, sS, sD, wa, wss, part, &nblk));
             if (rank_mask) TRY(join_side(c));
-            TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 1));
+            if (rev)
+                TRY(launch_wave_cot_contract(c, n, nw, ndof, nr, Pg, np, m0, cot, cotk, ga->dC, ga->dN, ga->dA,
+                                             bdk, T2));
+            else
+                TRY(launch_reduce_partials(c, nw, nblk, ndof * nr * Pg, part, T2, 1));
             if (direct) TRY(join_aux(c));
             TRY(launch_bc_tangent(c, mesh, Pg, tan, nw, d_om + w0, bdom, g, rho, ndof, dof_fields,
                                   ga->ddofs, np, m0, nbeta, d_be, phi, bstride, rhoW, wa, T1));
@@ -1272,6 +1293,13 @@ int mh_set_refinement(mh_context_t ctx, int max_sweeps, int fp64_fallback) {
     MH_REQUIRE(max_sweeps >= 1 && max_sweeps <= 100, "set_refinement: max_sweeps must be in [1, 100]");
     CTX(ctx)->ir_max_sweeps = max_sweeps;
     CTX(ctx)->fp64_fallback = fp64_fallback != 0;
+    return MH_OK;
+}
+
+int mh_set_gradient_mode(mh_context_t ctx, int mode) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(mode >= 0 && mode <= 2, "set_gradient_mode: mode must be 0 (chosen per sweep), 1 (forward) or 2 (reverse)");
+    CTX(ctx)->grad_mode = mode;
     return MH_OK;
 }
 

@@ -72,6 +72,9 @@ struct Context {
     // assembly: the Rankine and image far field (pairs beyond 7 panel radii)
     // in FP32 (mh_set_far_field_fp32)
     bool far32 = false;
+    // gradients: how the wave-term pair sums are differentiated
+    // (mh_set_gradient_mode: 0 chosen per sweep, 1 forward tangents, 2 reverse)
+    int grad_mode = 0;
     // graph capture (graph.hip): while set, sweep_impl only issues device
     // work (its streams belong to a capture begun by the caller); the
     // frequencies, wavenumbers, headings and frequency tangents come from
@@ -355,6 +358,23 @@ int launch_bc_tangent(Context* ctx, const mh_mesh_t* m, int P, const double* tan
                       const double2* rhoW, int64_t wstride, double2* out);
 int launch_reduce_partials(Context* ctx, int64_t nw, int64_t nblk, int64_t nout, const double2* part,
                            double2* out, int accumulate);
+// ---- reverse mode of the wave-term pair sums (grad_rev.hip) ------------------
+// Bytes of the work buffer of launch_wave_cot for n panels, nw frequencies
+// and ndof x nr outputs.
+int64_t wave_cot_ws_bytes(int64_t n, int64_t nw, int64_t ndof, int64_t nr);
+// Per-panel cotangents (centre, normal, area; n x 7 complex per frequency and
+// output) and per-frequency k cotangents of the wave-term pair sums, weights
+// as launch_tangent_pairs; *cot_out, *cotk_out point into `work`.
+int launch_wave_cot(Context* ctx, const mh_mesh_t* m, const PanelRec* panels, int gf, int form, int64_t nw,
+                    const double* d_k, int64_t ndof, int64_t nr, const double2* aS, const double2* aD,
+                    const double2* sS, const double2* sD, int64_t wa, int64_t ws, void* work, double2** cot_out,
+                    double2** cotk_out);
+// out[w][d][r][m] += the cotangents contracted with the tangents of
+// parameters m0 .. m0 + Pg - 1 (dC, dN: nparam x n x 3 column-major per
+// parameter, dA: nparam x n; null = zero) and dk (nw x Pg).
+int launch_wave_cot_contract(Context* ctx, int64_t n, int64_t nw, int64_t ndof, int64_t nr, int Pg, int64_t nparam,
+                             int64_t m0, const double2* cot, const double2* cotk, const double* dC, const double* dN,
+                             const double* dA, const double* dk, double2* out);
 // dA, dB (nw x nparam x ndof x ndof) and dF (complex, nw x nparam x nbeta x
 // ndof) of parameters m0 .. m0+P-1 from the pair terms T1 + T2, the forces A,
 // B of the same frequencies and the omega tangents dom (nw x P); null

@@ -80,6 +80,7 @@ _SIGS = {
     "mh_batch_graph_dot": (_int, [_vp, C.c_char_p]),
     "mh_batch_graph_destroy": (_int, [_vp]),
     "mh_set_far_field_fp32": (_int, [_vp, _int]),
+    "mh_set_gradient_mode": (_int, [_vp, _int]),
     "mh_multi_autotune": (_int, [_vp, _i64, _vp, _int, _d, _d, C.POINTER(_int), C.POINTER(_int),
                                  C.POINTER(_d)]),
     "mh_calculate_radiation_forces_host": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -200,6 +201,13 @@ def set_far_field_fp32(on=True, device=0):
     """Rankine and image far-field pairs (beyond 7 panel radii) of every
     assembly on the device's context in FP32 (mh_set_far_field_fp32)."""
     check(lib().mh_set_far_field_fp32(context(device), int(bool(on))), "set_far_field_fp32")
+
+
+def set_gradient_mode(mode="auto", device=0):
+    """How the sweep gradients differentiate the wave-term pair sums
+    (mh_set_gradient_mode): "auto", "forward" or "reverse"."""
+    m = {"auto": 0, "forward": 1, "reverse": 2}[mode]
+    check(lib().mh_set_gradient_mode(context(device), m), "set_gradient_mode")
 
 
 def fallback_count(device=0) -> int:
