@@ -168,8 +168,9 @@ int mh_set_far_field_fp32(mh_context_t ctx, int on);
  * reverse, one pass per block of 2 outputs (dof x right-hand side) that
  * leaves cotangents of every panel's centre, normal and area and of k,
  * contracted afterwards with any number of parameter tangents; 0 (default)
- * = whichever needs fewer passes, a reverse pass counted as 1.5.  Both give
- * the same derivatives up to rounding. */
+ * = whichever costs less, a reverse pass counted as 1.6 forward passes of
+ * up to 2 parameters and 0.8 of one of 3-4.  Both give the same
+ * derivatives up to rounding. */
 int mh_set_gradient_mode(mh_context_t ctx, int mode);
 int mh_fallback_count(mh_context_t ctx, int64_t* count);
 

@@ -809,15 +809,17 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                                                               : gf_mask;
     const int64_t tiles = (n + 63) / 64;
     const int64_t nout = (ndof > 0 ? ndof : 1) * nr * Pg;
-    // reverse mode of the wave-term pair sums when it takes fewer pair-kernel
-    // launches (output blocks of 2) than the forward tangents (parameter
-    // blocks of 4 times dof and right-hand-side blocks of 4), counting a
-    // reverse launch as 1.5 forward ones
+    // reverse mode of the wave-term pair sums when its passes (one per block
+    // of 2 outputs) cost less than the forward tangents' (one per block of
+    // up to 4 parameters, times dof and right-hand-side blocks of 4); in
+    // units of a 2-parameter forward pass (measured at 2,048 panels x 64
+    // frequencies: 9.7 ms), a 4-parameter forward pass costs 2 (19.6 ms) and
+    // a reverse pass 1.6 (15.5 ms)
     bool rev = false;
     if (ga && ndof && wave_mask) {
-        const int64_t fwd = nblocks * ((ndof + 3) / 4) * ((nr + 3) / 4);
-        const int64_t bwd = (ndof * nr + 1) / 2;
-        rev = c->grad_mode == 2 || (c->grad_mode == 0 && 3 * bwd < 2 * fwd);
+        const int64_t fwd10 = nblocks * ((ndof + 3) / 4) * ((nr + 3) / 4) * (Pg == 4 ? 20 : 10);
+        const int64_t rev10 = (ndof * nr + 1) / 2 * 16;
+        rev = c->grad_mode == 2 || (c->grad_mode == 0 && rev10 < fwd10);
     }
     size_t o_rev = 0;
     if (ga) {

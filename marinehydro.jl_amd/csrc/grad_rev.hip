@@ -12,9 +12,13 @@
 // the wave term depends on the pair only through the local inputs
 //   dx = c_p,x - c_q,x, dy = c_p,y - c_q,y, zs = c_p,z + c_q,z, and k,
 // linearly on the source area and on one normal.  The Wu (or exact
-// Delhommeau) function runs with local (r, z) tangents, lifted to the four
-// local inputs (Dual<4>); the pair's contribution to T_o is differentiated
-// with respect to them, and the cotangents are summed per panel field
+// Delhommeau) function runs with local (r, z) tangents (r = k hd(dx, dy),
+// z = k zs); per output the pair's contribution to T_o is
+//   H = k al G - k^2 b1 Gz - k kh b2 Gr     (kh = k / hd)
+// with complex coefficients from the weights, areas and normals (b2 linear
+// in dx, dy), and its derivatives w.r.t. the local inputs follow from the
+// (r, z) derivatives of G, Gr, Gz by the chain rule plus the explicit
+// dependence of k, kh and b2; the cotangents are summed per panel field
 // (centre 3, normal 3, area 1, per output) and per frequency (k):
 //   rows p of a tile in LDS, columns q by a reduction across the wave.
 // A second kernel contracts the cotangents with the tangents of any number
@@ -26,22 +30,38 @@ namespace mh {
 
 constexpr int kCotF = 7;  // cotangent fields per panel and output: c[3], n[3], area
 
-template <int P>
-__device__ __forceinline__ Dual<P> seed_local(double v, int idx) {
-    Dual<P> r;
-    r.v = v;
-#pragma unroll
-    for (int m = 0; m < P; ++m) r.d[m] = (m == idx) ? 1.0 : 0.0;
-    return r;
-}
-
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-// a * (re + i im)
-__device__ __forceinline__ double2 cmulr(double2 a, double re, double im) {
-    return make_double2(a.x * re - a.y * im, a.x * im + a.y * re);
+
+// v of another lane of the same row of 16 by a DPP move of both halves
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// v of the last lane of the previous row (ROW_BCAST15, rows 1 and 3) or of
+// lane 31 (ROW_BCAST31, rows 2 and 3); 0 in the other rows
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_bcast(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// sum over the 64 lanes, in lane 63: within quads (DPP quad_perm [1,0,3,2]
+// and [2,3,0,1]), across the quads of a row (row rotations by 4 and 8: every
+// lane of a row then holds the row's sum), then the row sums carried forward
+// by the row broadcasts (rows 1, 3 add the row before; rows 2, 3 add lane 31)
+__device__ __forceinline__ double wave_sum63(double v) {
+    v += dpp_row<0xB1>(v);
+    v += dpp_row<0x4E>(v);
+    v += dpp_row<0x124>(v);
+    v += dpp_row<0x128>(v);
+    v += dpp_bcast<0x142, 0xA>(v);
+    v += dpp_bcast<0x143, 0xC>(v);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -60,8 +80,6 @@ __global__ __launch_bounds__(256) void k_wave_cot_sym(
     int64_t nr, int64_t o0, int on, const double2* __restrict__ aS, const double2* __restrict__ aD,
     const double2* __restrict__ sS, const double2* __restrict__ sD, int64_t wstride_a, int64_t wstride_s,
     int64_t tiles, double2* __restrict__ prow, double2* __restrict__ pcol, double2* __restrict__ pk) {
-    typedef Dual<4> D4;
-    typedef cplxT<D4> C4;
     const int I = blockIdx.x, J = blockIdx.y;
     if (I > J) return;  // covered by the tile pair (J, I)
     const bool sym = I < J;
@@ -101,62 +119,63 @@ __global__ __launch_bounds__(256) void k_wave_cot_sym(
     double2 ck[OB];
 #pragma unroll
     for (int ol = 0; ol < OB; ++ol) ck[ol] = make_double2(0.0, 0.0);
-    auto scl = [](const D4& a, const C4& b) {
-        C4 o;
-        o.re = a * b.re;
-        o.im = a * b.im;
-        return o;
-    };
     const int64_t jb = (int64_t)J * 64;
     for (int jj = wave; jj < 64; jj += 4) {
         const int64_t j = jb + jj;
         if (j >= n) break;
         const PanelRec& pq = panels[j];
         const double nq0 = pq.n[0], nq1 = pq.n[1], nq2 = pq.n[2], aq = pq.area;
-        // local inputs (dx, dy, zs, k)
-        const D4 dx = seed_local<4>(x0 - pq.c[0], 0), dy = seed_local<4>(x1 - pq.c[1], 1);
-        const D4 zs = seed_local<4>(x2 + pq.c[2], 2), kk = seed_local<4>(kv, 3);
-        const D4 rr = kk * horiz_dist(dx, dy);
-        const D4 zz = kk * zs;
+        // local inputs: dx = c_p,x - c_q,x, dy, zs = c_p,z + c_q,z, and k
+        const double dx = x0 - pq.c[0], dy = x1 - pq.c[1], zs = x2 + pq.c[2];
+        const double dsq = dx * dx + dy * dy;
+        const double hd = horiz_dist(dx, dy);
+        // d hd / d(dx, dy) as the dual numbers give it: dx / sqrt(dsq), with
+        // the perturbation of horiz_dist at zero (then dx = dy = 0 and so is it)
+        const double sq = (dsq == 0.0) ? ::sqrt(dsq + 1e-36) : hd;
+        const double hx = dx / sq, hy = dy / sq;
+        const double rr = kv * hd, zz = kv * zs;
         Dual<2> rl, zl;
-        rl.v = rr.v;
+        rl.v = rr;
         rl.d[0] = 1.0;
         rl.d[1] = 0.0;
-        zl.v = zz.v;
+        zl.v = zz;
         zl.d[0] = 0.0;
         zl.d[1] = 1.0;
         cplxT<Dual<2>> Gl, Grl, Gzl;
         if (WAVE == 1) wu_dimless<true>(rl, zl, Gl, Grl, Gzl);
         if (WAVE == 2) delhommeau_dimless(rl, zl, Gl, Grl, Gzl);
-        C4 G, Gr, Gz;
-        G.re = lift_rz(Gl.re, rr, zz);
-        G.im = lift_rz(Gl.im, rr, zz);
-        Gr.re = lift_rz(Grl.re, rr, zz);
-        Gr.im = lift_rz(Grl.im, rr, zz);
-        Gz.re = lift_rz(Gzl.re, rr, zz);
-        Gz.im = lift_rz(Gzl.im, rr, zz);
-        D4 d1 = dconst<4>(0.0), d2 = dconst<4>(0.0);
-        if (::fabs(rr.v) > 1e-6) {
-            const D4 kq = kk * kk / rr;
-            d1 = kq * (-dx);
-            d2 = kq * (-dy);
-        }
-        const C4 A1 = scl(d1, Gr), A2 = scl(d2, Gr), A3 = scl(kk, Gz);
-        // n . (A1, A2, A3) for the source normal of (p, q) [FORM 1] and the
-        // collocation normal of (q, p) [FORM 2]: EQ; (-A1, -A2, A3) with n_p: EP
-        C4 EQ, EP;
-        EQ.re = nq0 * A1.re + nq1 * A2.re + nq2 * A3.re;
-        EQ.im = nq0 * A1.im + nq1 * A2.im + nq2 * A3.im;
-        EP.re = np2 * A3.re - np0 * A1.re - np1 * A2.re;
-        EP.im = np2 * A3.im - np0 * A1.im - np1 * A2.im;
-        const C4 KG = scl(kk, G);
-        const C4 KQ = scl(kk, FORM == 1 ? EQ : EP);  // D_pq (K_pq) / a_q
-        const C4 KP = scl(kk, FORM == 1 ? EP : EQ);  // D_qp (K_qp) / a_p
-        // d KQ / d normal, d KP / d normal (values): k (A1, A2, A3) for n_q,
-        // k (-A1, -A2, A3) for n_p
-        const double2 vq[3] = {make_double2(kv * A1.re.v, kv * A1.im.v), make_double2(kv * A2.re.v, kv * A2.im.v),
-                               make_double2(kv * A3.re.v, kv * A3.im.v)};
+        // values and (r, z) derivatives of G, dG/dr, dG/dz
+        const double2 G0 = make_double2(Gl.re.v, Gl.im.v), Gr0 = make_double2(Grl.re.v, Grl.im.v),
+                      Gz0 = make_double2(Gzl.re.v, Gzl.im.v);
+        const double2 G_r = make_double2(Gl.re.d[0], Gl.im.d[0]), G_z = make_double2(Gl.re.d[1], Gl.im.d[1]);
+        const double2 Gr_r = make_double2(Grl.re.d[0], Grl.im.d[0]), Gr_z = make_double2(Grl.re.d[1], Grl.im.d[1]);
+        const double2 Gz_r = make_double2(Gzl.re.d[0], Gzl.im.d[0]), Gz_z = make_double2(Gzl.re.d[1], Gzl.im.d[1]);
+        // d1 = -kh dx, d2 = -kh dy (kh = k / hd; zero near r = 0, as the
+        // forward code sets them); A = (d1 Gr, d2 Gr, k Gz);
+        // EQ(n) = n . A, EP(n) = n2 A3 - n0 A1 - n1 A2
+        const bool far = ::fabs(rr) > 1e-6;
+        const double kh = far ? kv * kv / rr : 0.0;
+        const double2 A1 = make_double2(-kh * dx * Gr0.x, -kh * dx * Gr0.y);
+        const double2 A2 = make_double2(-kh * dy * Gr0.x, -kh * dy * Gr0.y);
+        const double2 A3 = make_double2(kv * Gz0.x, kv * Gz0.y);
+        const double2 EQ = make_double2(nq0 * A1.x + nq1 * A2.x + nq2 * A3.x, nq0 * A1.y + nq1 * A2.y + nq2 * A3.y);
+        const double2 EP = make_double2(np2 * A3.x - np0 * A1.x - np1 * A2.x, np2 * A3.y - np0 * A1.y - np1 * A2.y);
+        // values: KG = k G, KQ = D_pq / a_q (K_pq / a_q), KP = D_qp / a_p
+        const double2 KG = make_double2(kv * G0.x, kv * G0.y);
+        const double2 KQ = FORM == 1 ? make_double2(kv * EQ.x, kv * EQ.y) : make_double2(kv * EP.x, kv * EP.y);
+        const double2 KP = FORM == 1 ? make_double2(kv * EP.x, kv * EP.y) : make_double2(kv * EQ.x, kv * EQ.y);
+        // the normals and signs of the two D (K) entries: K_beta = k E(n_beta)
+        // with E(n) = n2 k Gz + s (kh (n0 dx + n1 dy)) Gr, s = -1 for EQ, +1 for EP
+        const double nb0 = FORM == 1 ? nq0 : np0, nb1 = FORM == 1 ? nq1 : np1, nb2 = FORM == 1 ? nq2 : np2;
+        const double ng0 = FORM == 1 ? np0 : nq0, ng1 = FORM == 1 ? np1 : nq1, ng2 = FORM == 1 ? np2 : nq2;
+        const double sb = FORM == 1 ? -1.0 : 1.0, sg = -sb;
+        const double rhob = nb0 * dx + nb1 * dy, rhog = ng0 * dx + ng1 * dy;
+        // d (k E) / d normal (values): k (A1, A2, A3) with n_q [EQ],
+        // k (-A1, -A2, A3) with n_p [EP]
+        const double2 vq[3] = {make_double2(kv * A1.x, kv * A1.y), make_double2(kv * A2.x, kv * A2.y),
+                               make_double2(kv * A3.x, kv * A3.y)};
         const double2 vp[3] = {make_double2(-vq[0].x, -vq[0].y), make_double2(-vq[1].x, -vq[1].y), vq[2]};
+        const double k2 = kv * kv, kkh = kv * kh;
 #pragma unroll
         for (int ol = 0; ol < OB; ++ol) {
             if (ol >= on) continue;
@@ -173,20 +192,36 @@ __global__ __launch_bounds__(256) void k_wave_cot_sym(
                 al.y += u2.y * ap;
                 ga = make_double2(v2.x * ap, v2.y * ap);
             }
+            // H = al KG - be KQ - ga KP = k al G - k^2 b1 Gz - k kh b2 Gr
+            const double2 b1 = make_double2(be.x * nb2 + ga.x * ng2, be.y * nb2 + ga.y * ng2);
+            const double2 b2 = make_double2(sb * rhob * be.x + sg * rhog * ga.x, sb * rhob * be.y + sg * rhog * ga.y);
+            const double2 b2x = make_double2(sb * nb0 * be.x + sg * ng0 * ga.x, sb * nb0 * be.y + sg * ng0 * ga.y);
+            const double2 b2y = make_double2(sb * nb1 * be.x + sg * ng1 * ga.x, sb * nb1 * be.y + sg * ng1 * ga.y);
+            const double2 aGr = cmul2(al, G_r), aGz = cmul2(al, G_z);
+            const double2 b1r = cmul2(b1, Gz_r), b1z = cmul2(b1, Gz_z);
+            const double2 b2r = cmul2(b2, Gr_r), b2z = cmul2(b2, Gr_z);
+            const double2 Hr = make_double2(kv * aGr.x - k2 * b1r.x - kkh * b2r.x, kv * aGr.y - k2 * b1r.y - kkh * b2r.y);
+            const double2 Hz = make_double2(kv * aGz.x - k2 * b1z.x - kkh * b2z.x, kv * aGz.y - k2 * b1z.y - kkh * b2z.y);
+            // explicit dependence on k (H's factors k, k^2, k kh with kh = k / hd)
+            // and on dx, dy (kh and b2)
+            const double2 aG = cmul2(al, G0), b1G = cmul2(b1, Gz0), b2G = cmul2(b2, Gr0);
+            const double2 Hk = make_double2(aG.x - 2.0 * kv * b1G.x - 2.0 * kh * b2G.x,
+                                            aG.y - 2.0 * kv * b1G.y - 2.0 * kh * b2G.y);
+            const double ihd = far ? 1.0 / hd : 0.0;
+            const double2 tx = make_double2(b2.x * hx * ihd - b2x.x, b2.y * hx * ihd - b2x.y);
+            const double2 ty = make_double2(b2.x * hy * ihd - b2y.x, b2.y * hy * ihd - b2y.y);
+            const double2 Gtx = cmul2(Gr0, tx), Gty = cmul2(Gr0, ty);
             double2 Fd[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const double2 t = cmulr(al, KG.re.d[m], KG.im.d[m]);
-                const double2 t1 = cmulr(be, KQ.re.d[m], KQ.im.d[m]);
-                const double2 t2 = cmulr(ga, KP.re.d[m], KP.im.d[m]);
-                Fd[m] = make_double2(c4 * (t.x - t1.x - t2.x), c4 * (t.y - t1.y - t2.y));
-            }
+            Fd[0] = make_double2(c4 * (Hr.x * kv * hx + kkh * Gtx.x), c4 * (Hr.y * kv * hx + kkh * Gtx.y));
+            Fd[1] = make_double2(c4 * (Hr.x * kv * hy + kkh * Gty.x), c4 * (Hr.y * kv * hy + kkh * Gty.y));
+            Fd[2] = make_double2(c4 * Hz.x * kv, c4 * Hz.y * kv);
+            Fd[3] = make_double2(c4 * (Hr.x * hd + Hz.x * zs + Hk.x), c4 * (Hr.y * hd + Hz.y * zs + Hk.y));
             // area cotangents
-            const double2 gq = cmulr(u, KG.re.v, KG.im.v), hq = cmulr(v, KQ.re.v, KQ.im.v);
+            const double2 gq = cmul2(u, KG), hq = cmul2(v, KQ);
             const double2 daq = make_double2(c4 * (gq.x - hq.x), c4 * (gq.y - hq.y));
             double2 dap = make_double2(0.0, 0.0);
             if (sym) {
-                const double2 gp = cmulr(u2, KG.re.v, KG.im.v), hp = cmulr(v2, KP.re.v, KP.im.v);
+                const double2 gp = cmul2(u2, KG), hp = cmul2(v2, KP);
                 dap = make_double2(c4 * (gp.x - hp.x), c4 * (gp.y - hp.y));
             }
             // normal cotangents: -c4 be (d KQ / dn), -c4 ga (d KP / dn)
@@ -214,20 +249,13 @@ __global__ __launch_bounds__(256) void k_wave_cot_sym(
                 ck[ol] = cadd2(ck[ol], Fd[3]);
             }
             // columns: centre q (-d/ddx, -d/ddy, +d/dzs), normal, area; summed
-            // over the lanes (rows), written by lane 0
+            // over the lanes (rows), written by lane 63
             double2 cv[kCotF] = {make_double2(-Fd[0].x, -Fd[0].y), make_double2(-Fd[1].x, -Fd[1].y), Fd[2],
                                  ncol[0], ncol[1], ncol[2], daq};
 #pragma unroll
-            for (int f = 0; f < kCotF; ++f) {
-                double vx = row_ok ? cv[f].x : 0.0, vy = row_ok ? cv[f].y : 0.0;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    vx += __shfl_xor(vx, off);
-                    vy += __shfl_xor(vy, off);
-                }
-                cv[f] = make_double2(vx, vy);
-            }
-            if (lane == 0) {
+            for (int f = 0; f < kCotF; ++f)
+                cv[f] = make_double2(wave_sum63(row_ok ? cv[f].x : 0.0), wave_sum63(row_ok ? cv[f].y : 0.0));
+            if (lane == 63) {
                 double2* pc = pcol + (((int64_t)w * tiles + I) * n + j) * (OB * kCotF) + ol * kCotF;
 #pragma unroll
                 for (int f = 0; f < kCotF; ++f) pc[f] = cv[f];
