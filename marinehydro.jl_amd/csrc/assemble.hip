@@ -153,10 +153,31 @@ __global__ __launch_bounds__(256) void k_rankine_parts(int64_t n, const double* 
 }
 
 constexpr int kSymFG = 8;  // wavenumbers per workgroup (blockIdx.y group)
-constexpr int kSymFB = 4;  // wavenumbers staged in LDS per round
+constexpr int kSymRG = 4;  // rounds (4 source columns each, one per wave) per transposed write-out
+
+// Loop order: groups of kSymRG rounds (16 source columns j), then the
+// workgroup's wavenumbers, then the rounds of the group.  A round's lanes
+// write the (i, j) entries straight down column j; its (j, i) entries are
+// staged in LDS and, once the group's 16 rows are there, written 16 rows at a
+// time: 256-byte runs per column (128 for the FP32 copy) instead of the
+// 64-byte runs of one round, so the L2 lines of the transposed half leave the
+// cache whole rather than half written.  The group's horizontal distances
+// and Rankine parts stay in registers across the wavenumbers.
+// four per-round values in named registers, read by selection (an indexed
+// register array would be placed in scratch memory)
+struct Held4 {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    __device__ void set(int t, double v) {
+        if (t == 0) a0 = v;
+        if (t == 1) a1 = v;
+        if (t == 2) a2 = v;
+        if (t == 3) a3 = v;
+    }
+    __device__ double get(int t) const { return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3)); }
+};
 
 template <int WAVE, int FORM>
-__global__ __launch_bounds__(256) void k_assemble_sym(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVE == 1 && FORM != 3 ? 3 : 1))) void k_assemble_sym(
     int64_t n, const double* __restrict__ C, const double* __restrict__ N, const PanelRec* __restrict__ panels,
     int64_t nk, const double* __restrict__ ks, const double* __restrict__ RS, const double* __restrict__ RD,
     const double* __restrict__ RK, double2* __restrict__ S, double2* __restrict__ D, double2* __restrict__ K,
@@ -178,102 +199,120 @@ __global__ __launch_bounds__(256) void k_assemble_sym(
     const double ai = panels[ic].area;
     const int64_t f0 = (int64_t)blockIdx.y * kSymFG;
     const int nfg = (int)((nk - f0) < kSymFG ? (nk - f0) : kSymFG);
-    // wave parts of the (j, i) entries of one round: [wavenumber][j - row of
-    // the round][i - column]
-    __shared__ double2 sS[kSymFB][4][64];
-    __shared__ double2 sD[(FORM & 1) ? kSymFB : 1][4][64];
-    __shared__ double2 sK[(FORM & 2) ? kSymFB : 1][4][64];
-    for (int t = 0; t < 16; ++t) {
-        const int64_t j = (int64_t)J * 64 + 4 * t + wave;  // wave-uniform source column
-        const bool jv = j < n;
-        const bool dir = row_ok && jv && (!dtile || j >= i);
-        const bool trn = row_ok && jv && (!dtile || j > i);
-        const PanelRec& pj = panels[jv ? j : n - 1];
-        const double nj0 = pj.n[0], nj1 = pj.n[1], nj2 = pj.n[2];
-        const double aj = pj.area;
-        const double dx = x0 - pj.c[0], dy = x1 - pj.c[1], zs = x2 + pj.c[2];
-        const double hd = horiz_dist(dx, dy);
-        double rs = 0.0, rd = 0.0, rk = 0.0;
-        if (dir && RS) {
-            rs = RS[i + j * n];
-            if (FORM & 1) rd = RD[i + j * n];
-            if (FORM & 2) rk = RK[i + j * n];
+    constexpr int kRows = 4 * kSymRG;  // rows j of a group
+    // wave parts of the group's (j, i) entries for one wavenumber:
+    // [j - first row of the group][i - column], rows padded against bank conflicts
+    __shared__ double2 sS[kRows][64 + 1];
+    __shared__ double2 sD[(FORM & 1) ? kRows : 1][64 + 1];
+    __shared__ double2 sK[(FORM & 2) ? kRows : 1][64 + 1];
+    // write-out: thread -> entries (row j = first + (e & 15), column i = I*64 + (e >> 4)), e = tid + 256 u
+    constexpr int kOut = kRows * 64 / 256;
+    static_assert(kSymRG == 4, "the held values are four named registers");
+    for (int tg = 0; tg < 16 / kSymRG; ++tg) {
+        Held4 hd, rs, rd, rk;
+#pragma unroll
+        for (int tt = 0; tt < kSymRG; ++tt) {
+            const int64_t j = (int64_t)J * 64 + 4 * (tg * kSymRG + tt) + wave;  // wave-uniform source column
+            const bool jv = j < n;
+            const bool dir = row_ok && jv && (!dtile || j >= i);
+            const PanelRec& pj = panels[jv ? j : n - 1];
+            double vs = 0.0, vd = 0.0, vk = 0.0;
+            if (dir && RS) {
+                vs = RS[i + j * n];
+                if (FORM & 1) vd = RD[i + j * n];
+                if (FORM & 2) vk = RK[i + j * n];
+            }
+            hd.set(tt, horiz_dist(x0 - pj.c[0], x1 - pj.c[1]));
+            rs.set(tt, vs);
+            rd.set(tt, vd);
+            rk.set(tt, vk);
         }
-        const double dg = (i == j) ? 0.5 : 0.0;
-        for (int fb = 0; fb < nfg; fb += kSymFB) {
+        // Rankine parts of this thread's transposed entries
+        const int64_t qrow0 = (int64_t)J * 64 + kRows * tg + (threadIdx.x & (kRows - 1));
+        auto qcol = [&](int u) { return (int64_t)I * 64 + (int)(threadIdx.x + 256 * u) / kRows; };
+        auto qok = [&](int u) { return qrow0 < n && qcol(u) < n && (!dtile || qrow0 > qcol(u)); };
+        double qs[kOut], qd[kOut], qk[kOut];
+#pragma unroll
+        for (int u = 0; u < kOut; ++u) {
+            qs[u] = qd[u] = qk[u] = 0.0;
+            if (qok(u) && RS) {
+                qs[u] = RS[qrow0 + qcol(u) * n];
+                if (FORM & 1) qd[u] = RD[qrow0 + qcol(u) * n];
+                if (FORM & 2) qk[u] = RK[qrow0 + qcol(u) * n];
+            }
+        }
 #pragma unroll 1
-            for (int f = 0; f < kSymFB; ++f) {
-                const int64_t m = f0 + fb + f;
-                if (fb + f < nfg && (dir || trn)) {
-                    const double k = ks[m];
-                    cplx G, Gr, Gz;
-                    double d1, d2;
-                    wave_raw<WAVE>(k, dx, dy, zs, hd, G, Gr, Gz, d1, d2, tab);
-                    // gradient w.r.t. the source point of (i, j), without the area:
-                    // (d1 Gr, d2 Gr, k Gz); of (j, i): (-d1 Gr, -d2 Gr, k Gz)
-                    const double kaj = k * aj, kai = k * ai;
-                    if (dir) {
-                        const int64_t off = m * stride + i + j * ld;
-                        S[off] = make_double2(-kInv4Pi * (rs + kaj * G.re), -kInv4Pi * (kaj * G.im));
-                        if (FORM & 1) {
-                            const double wr = kaj * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re));
-                            const double wi = kaj * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im));
-                            const double2 v = make_double2(dg - kInv4Pi * (rd + wr), -kInv4Pi * wi);
-                            D[off] = v;
-                            if (DK32) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
-                        }
-                        if (FORM & 2) {
-                            const double wr = kaj * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re));
-                            const double wi = kaj * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im));
-                            const double2 v = make_double2(dg - kInv4Pi * (rk + wr), -kInv4Pi * wi);
-                            K[off] = v;
-                            if (DK32 && !(FORM & 1)) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
-                        }
+        for (int f = 0; f < nfg; ++f) {
+            const int64_t m = f0 + f;
+            const double k = ks[m];
+#pragma unroll 1
+            for (int tt = 0; tt < kSymRG; ++tt) {
+                const int r = 4 * tt + wave;  // row of the group
+                const int64_t j = (int64_t)J * 64 + kRows * tg + r;
+                const bool jv = j < n;
+                const bool dir = row_ok && jv && (!dtile || j >= i);
+                const bool trn = row_ok && jv && (!dtile || j > i);
+                if (!(dir || trn)) continue;
+                const PanelRec& pj = panels[j];
+                const double nj0 = pj.n[0], nj1 = pj.n[1], nj2 = pj.n[2];
+                const double aj = pj.area;
+                const double dx = x0 - pj.c[0], dy = x1 - pj.c[1], zs = x2 + pj.c[2];
+                cplx G, Gr, Gz;
+                double d1, d2;
+                wave_raw<WAVE>(k, dx, dy, zs, hd.get(tt), G, Gr, Gz, d1, d2, tab);
+                // gradient w.r.t. the source point of (i, j), without the area:
+                // (d1 Gr, d2 Gr, k Gz); of (j, i): (-d1 Gr, -d2 Gr, k Gz)
+                const double kaj = k * aj, kai = k * ai;
+                if (dir) {
+                    const double dg = (i == j) ? 0.5 : 0.0;
+                    const int64_t off = m * stride + i + j * ld;
+                    S[off] = make_double2(-kInv4Pi * (rs.get(tt) + kaj * G.re), -kInv4Pi * (kaj * G.im));
+                    if (FORM & 1) {
+                        const double wr = kaj * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re));
+                        const double wi = kaj * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im));
+                        const double2 v = make_double2(dg - kInv4Pi * (rd.get(tt) + wr), -kInv4Pi * wi);
+                        D[off] = v;
+                        if (DK32) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
                     }
-                    if (trn) {
-                        sS[f][wave][lane] = make_double2(kai * G.re, kai * G.im);
-                        if (FORM & 1)
-                            sD[f][wave][lane] =
-                                make_double2(kai * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re)),
-                                             kai * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im)));
-                        if (FORM & 2)
-                            sK[f][wave][lane] =
-                                make_double2(kai * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re)),
-                                             kai * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im)));
+                    if (FORM & 2) {
+                        const double wr = kaj * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re));
+                        const double wi = kaj * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im));
+                        const double2 v = make_double2(dg - kInv4Pi * (rk.get(tt) + wr), -kInv4Pi * wi);
+                        K[off] = v;
+                        if (DK32 && !(FORM & 1)) DK32[m * n * n + i + j * n] = make_float2((float)v.x, (float)v.y);
                     }
+                }
+                if (trn) {
+                    sS[r][lane] = make_double2(kai * G.re, kai * G.im);
+                    if (FORM & 1)
+                        sD[r][lane] = make_double2(kai * (ni2 * (k * Gz.re) - ni0 * (d1 * Gr.re) - ni1 * (d2 * Gr.re)),
+                                                   kai * (ni2 * (k * Gz.im) - ni0 * (d1 * Gr.im) - ni1 * (d2 * Gr.im)));
+                    if (FORM & 2)
+                        sK[r][lane] = make_double2(kai * (nj0 * (d1 * Gr.re) + nj1 * (d2 * Gr.re) + nj2 * (k * Gz.re)),
+                                                   kai * (nj0 * (d1 * Gr.im) + nj1 * (d2 * Gr.im) + nj2 * (k * Gz.im)));
                 }
             }
             __syncthreads();
-            {  // entries (row = J*64 + 4t + jj, column = I*64 + ii), four rows per column
-                const int jj = threadIdx.x & 3, ii = threadIdx.x >> 2;
-                const int64_t row = (int64_t)J * 64 + 4 * t + jj, col = (int64_t)I * 64 + ii;
-                if (row < n && col < n && (!dtile || row > col)) {
-                    double qs = 0.0, qd = 0.0, qk = 0.0;
-                    if (RS) {
-                        qs = RS[row + col * n];
-                        if (FORM & 1) qd = RD[row + col * n];
-                        if (FORM & 2) qk = RK[row + col * n];
-                    }
-#pragma unroll 1
-                    for (int f = 0; f < kSymFB && fb + f < nfg; ++f) {
-                        const int64_t m = f0 + fb + f;
-                        const int64_t off = m * stride + row + col * ld;
-                        const double2 ws = sS[f][jj][ii];
-                        S[off] = make_double2(-kInv4Pi * (qs + ws.x), -kInv4Pi * ws.y);
-                        if (FORM & 1) {
-                            const double2 w = sD[f][jj][ii];
-                            const double2 v = make_double2(-kInv4Pi * (qd + w.x), -kInv4Pi * w.y);
-                            D[off] = v;
-                            if (DK32) DK32[m * n * n + row + col * n] = make_float2((float)v.x, (float)v.y);
-                        }
-                        if (FORM & 2) {
-                            const double2 w = sK[f][jj][ii];
-                            const double2 v = make_double2(-kInv4Pi * (qk + w.x), -kInv4Pi * w.y);
-                            K[off] = v;
-                            if (DK32 && !(FORM & 1))
-                                DK32[m * n * n + row + col * n] = make_float2((float)v.x, (float)v.y);
-                        }
-                    }
+#pragma unroll
+            for (int u = 0; u < kOut; ++u) {
+                if (!qok(u)) continue;
+                const int e = threadIdx.x + 256 * u;
+                const int r = e & (kRows - 1), c = e / kRows;
+                const int64_t off = m * stride + qrow0 + qcol(u) * ld;
+                const double2 ws = sS[r][c];
+                S[off] = make_double2(-kInv4Pi * (qs[u] + ws.x), -kInv4Pi * ws.y);
+                if (FORM & 1) {
+                    const double2 w = sD[r][c];
+                    const double2 v = make_double2(-kInv4Pi * (qd[u] + w.x), -kInv4Pi * w.y);
+                    D[off] = v;
+                    if (DK32) DK32[m * n * n + qrow0 + qcol(u) * n] = make_float2((float)v.x, (float)v.y);
+                }
+                if (FORM & 2) {
+                    const double2 w = sK[r][c];
+                    const double2 v = make_double2(-kInv4Pi * (qk[u] + w.x), -kInv4Pi * w.y);
+                    K[off] = v;
+                    if (DK32 && !(FORM & 1))
+                        DK32[m * n * n + qrow0 + qcol(u) * n] = make_float2((float)v.x, (float)v.y);
                 }
             }
             __syncthreads();
