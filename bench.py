@@ -475,6 +475,9 @@ def main():
         ent = vj.get(f"{top_name}|{args.precision}|{n}x{args.nomega}")
         if ent:
             roof["longest_kernel"]["valu_issue_utilisation"] = ent["valu_issue_utilisation"]
+            for key in ("valu_active_per_wave", "waves_per_simd"):
+                if key in ent:
+                    roof["longest_kernel"][key] = ent[key]
             roof["longest_kernel"]["source"] = ent["summary"]
     # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of
     # the same workload, scaled by the counters' reading for a copy of known
