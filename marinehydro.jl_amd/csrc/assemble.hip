@@ -153,15 +153,22 @@ __global__ __launch_bounds__(256) void k_rankine_parts(int64_t n, const double* 
 }
 
 constexpr int kSymFG = 8;  // wavenumbers per workgroup (blockIdx.y group)
-constexpr int kSymRG = 4;  // rounds (4 source columns each, one per wave) per transposed write-out
+// rounds (4 source columns each, one per wave) per transposed write-out: 4
+// where the kernel's time goes into its matrix writes (Wu, tabulated
+// Delhommeau), 1 for the exact Delhommeau term, whose quadrature per pair
+// dominates and which keeps its registers (and waves per SIMD) that way
+template <int WAVE>
+constexpr int sym_rounds() {
+    return WAVE == 2 ? 1 : 4;
+}
 
-// Loop order: groups of kSymRG rounds (16 source columns j), then the
+// Loop order: groups of RG rounds (4 RG source columns j), then the
 // workgroup's wavenumbers, then the rounds of the group.  A round's lanes
 // write the (i, j) entries straight down column j; its (j, i) entries are
-// staged in LDS and, once the group's 16 rows are there, written 16 rows at a
-// time: 256-byte runs per column (128 for the FP32 copy) instead of the
-// 64-byte runs of one round, so the L2 lines of the transposed half leave the
-// cache whole rather than half written.  The group's horizontal distances
+// staged in LDS and, once the group's 4 RG rows are there, written 4 RG rows
+// at a time: with RG = 4, 256-byte runs per column (128 for the FP32 copy)
+// instead of the 64-byte runs of one round, so the L2 lines of the
+// transposed half leave the cache whole rather than half written.  The group's horizontal distances
 // and Rankine parts stay in registers across the wavenumbers.
 // four per-round values in named registers, read by selection (an indexed
 // register array would be placed in scratch memory)
@@ -199,7 +206,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVE == 1 &
     const double ai = panels[ic].area;
     const int64_t f0 = (int64_t)blockIdx.y * kSymFG;
     const int nfg = (int)((nk - f0) < kSymFG ? (nk - f0) : kSymFG);
-    constexpr int kRows = 4 * kSymRG;  // rows j of a group
+    constexpr int RG = sym_rounds<WAVE>();
+    constexpr int kRows = 4 * RG;  // rows j of a group
     // wave parts of the group's (j, i) entries for one wavenumber:
     // [j - first row of the group][i - column], rows padded against bank conflicts
     __shared__ double2 sS[kRows][64 + 1];
@@ -207,12 +215,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVE == 1 &
     __shared__ double2 sK[(FORM & 2) ? kRows : 1][64 + 1];
     // write-out: thread -> entries (row j = first + (e & 15), column i = I*64 + (e >> 4)), e = tid + 256 u
     constexpr int kOut = kRows * 64 / 256;
-    static_assert(kSymRG == 4, "the held values are four named registers");
-    for (int tg = 0; tg < 16 / kSymRG; ++tg) {
+    static_assert(RG == 1 || RG == 4, "the held values are four named registers");
+    for (int tg = 0; tg < 16 / RG; ++tg) {
         Held4 hd, rs, rd, rk;
 #pragma unroll
-        for (int tt = 0; tt < kSymRG; ++tt) {
-            const int64_t j = (int64_t)J * 64 + 4 * (tg * kSymRG + tt) + wave;  // wave-uniform source column
+        for (int tt = 0; tt < RG; ++tt) {
+            const int64_t j = (int64_t)J * 64 + 4 * (tg * RG + tt) + wave;  // wave-uniform source column
             const bool jv = j < n;
             const bool dir = row_ok && jv && (!dtile || j >= i);
             const PanelRec& pj = panels[jv ? j : n - 1];
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVE == 1 &
             const int64_t m = f0 + f;
             const double k = ks[m];
 #pragma unroll 1
-            for (int tt = 0; tt < kSymRG; ++tt) {
+            for (int tt = 0; tt < RG; ++tt) {
                 const int r = 4 * tt + wave;  // row of the group
                 const int64_t j = (int64_t)J * 64 + kRows * tg + r;
                 const bool jv = j < n;
