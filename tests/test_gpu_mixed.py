@@ -1,7 +1,8 @@
 """Mixed precision: the FP64 fallback of a chunk whose refinement does not
 converge (forced from the host by allowing a single refinement sweep; every
-kernel runs normally), and non-finite input reported as an error rather
-than 'converged'."""
+kernel runs normally), two sweeps sufficing on ragged meshes (which pins
+the assembly's FP32 copy of the matrix the FP32 LU factors), and non-finite
+input reported as an error rather than 'converged'."""
 import numpy as np
 import pytest
 
@@ -38,6 +39,29 @@ def test_fp64_fallback_when_refinement_cannot_converge(mh, gpu, direct):
     assert mh._lib.fallback_count() == n0
     for g, r in zip(got, ref):
         assert relmax(g, r) < 1e-8
+
+
+@pytest.mark.parametrize("shape", [(18, 25), (30, 41)])
+@pytest.mark.parametrize("direct", [True, False])
+def test_two_sweeps_suffice_on_ragged_meshes(mh, gpu, shape, direct):
+    # The FP32 copy of D (K) that the mixed assembly writes, both halves of
+    # every 64 x 64 tile pair and the ragged last tiles and row groups
+    # included (225 and 615 panels), is the matrix the FP32 LU factors; a
+    # wrong entry in it would leave the refinement short of its 1e-10
+    # tolerance after two sweeps, which (without the FP64 fallback) is an error.
+    body = mh.hemisphere(1.0, ntheta=shape[0], nphi=shape[1])
+    m = body.mesh
+    om = np.linspace(0.3, 3.0, 11)
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=len(om), omega_param=1)
+    ref = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.7], T, direct=direct, precision="fp64")
+    try:
+        mh._lib.set_refinement(max_sweeps=2, fp64_fallback=False)
+        got = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.7], T, direct=direct, precision="mixed")
+    finally:
+        mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
+    for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
+        assert relmax(g, r) < 1e-9, name
 
 
 def test_nan_in_the_system_is_an_error_not_converged(mh, gpu):
