@@ -25,9 +25,9 @@ import time
 # One hardware queue per HIP stream: the multi-body scheduler runs up to 6
 # worker contexts of 3 streams each, and HIP multiplexes streams onto
 # GPU_MAX_HW_QUEUES queues (4 by default), which serialises streams that
-# share a queue.  Set before HIP initialises (the import of torch); a value
-# in the environment takes precedence.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
+# share a queue.  Set before HIP initialises (the import of torch), over any
+# value in the environment (HIP's default of 4 may be exported there).
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "marinehydro.jl_amd"))

@@ -1240,6 +1240,15 @@ struct HImg {
     int slot = 0;  // slot of the block column the next updates use
 };
 
+// Unused LDS reserved by each workgroup of the wide trailing update of the
+// mixed LU (k_lu_gemm_hs, 66,560 B of its own): 16 KB more puts it above
+// half of a CU's 160 KB, so one fits per CU instead of two and half of every
+// CU stays free for the panel chain's kernels beside it.  Measured at 2,048
+// panels x 64 frequencies: 34.2-34.5 -> 33.0-33.5 ms per sweep (16-48 KB
+// alike, more slower; reserving LDS in the wide TRSM as well did not help);
+// the uneven 256-body batch is the same with or without it.
+constexpr unsigned kWideLdsPad = 16384;
+
 template <class T>
 struct LuPlan {
     typedef typename Cx<T>::t C2;
@@ -1251,6 +1260,7 @@ struct LuPlan {
     int64_t pc_lo, pc_hi;  // block column being factored
     HImg* img = nullptr;   // FP32 only
     bool chain = false;    // updates issued on the panel chain's stream
+    bool wide = false;     // the wide trailing update (beside the panel chain)
 };
 
 // Bytes of the image buffer for `batch` matrices of n rows and ncols columns.
@@ -1350,7 +1360,13 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
                            (int)c_lo, (int)c_hi,   H->L[sl],    H->rexp[sl],     (int)H->npad,
                            H->U,      H->cexp,     (int)H->ncpad, (int)ntr,      (int)ntc,
                            (int)ntiles, P.chain ? 1 : 0};
-            hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), 0, P.ctx->stream, ha);
+            // The wide update reserves LDS it does not use (kWideLdsPad), so
+            // that one of its workgroups fits per CU instead of two: half of
+            // every CU's registers and LDS stay free for the panel chain's
+            // kernels beside it (the recursion's GEMMs and TRSMs, the narrow
+            // leaves), which otherwise wait for a tile to retire.
+            hipLaunchKernelGGL(k_lu_gemm_hs, dim3((unsigned)nwg), dim3(512), P.wide ? kWideLdsPad : 0u,
+                               P.ctx->stream, ha);
         } else {
             ProfScope ps(P.ctx, name, 8.0 * (double)mr * (double)mc * kw * P.batch,
                          2.0 * sizeof(T) * (2.0 * mr * mc + (double)kw * (mr + mc)) * P.batch);
@@ -1514,10 +1530,12 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         // wide update on aux
         ctx->stream = sB;
         MH_HIP_CHECK(hipStreamWaitEvent(sB, evF[k], 0));
+        P.wide = true;
         rc = lu_swap(P, k0, w, 0, k0);
         if (rc == MH_OK) rc = lu_swap(P, k0, w, cw0, ncols);
         if (rc == MH_OK) rc = lu_trsm(P, k0, w, cw0, ncols);
         if (rc == MH_OK) rc = lu_gemm(P, kn0, n, cw0, ncols, k0, w);
+        P.wide = false;
         ctx->stream = sA;
         if (rc != MH_OK) break;
         MH_HIP_CHECK(hipEventRecord(evB[k], sB));
