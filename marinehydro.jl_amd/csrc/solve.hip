@@ -1245,7 +1245,8 @@ struct HImg {
 // half of a CU's 160 KB, so one fits per CU instead of two and half of every
 // CU stays free for the panel chain's kernels beside it.  Measured at 2,048
 // panels x 64 frequencies: 34.2-34.5 -> 33.0-33.5 ms per sweep (16-48 KB
-// alike, more slower; reserving LDS in the wide TRSM as well did not help);
+// alike, more slower; reserving LDS in the wide TRSM as well did not help,
+// nor reserving it only for the smaller updates of the later block columns);
 // the uneven 256-body batch is the same with or without it.
 constexpr unsigned kWideLdsPad = 16384;
 

@@ -630,7 +630,13 @@ static int tri_transpose_t(Context* c, int64_t n, int64_t batch, const typename 
     MH_REQUIRE(r_lo >= 0 && r_hi <= n, "transpose: row range outside the matrix");
     const unsigned tr = (unsigned)((r_hi - r_lo + 63) / 64), tc = (unsigned)((n + 63) / 64);
     ProfScope ps(c, "transpose", 0.0, 2.0 * sizeof(typename Cx<T>::t) * (double)(r_hi - r_lo) * n * batch);
-    hipLaunchKernelGGL(k_transpose<typename Cx<T>::t>, dim3(tr, tc, (unsigned)batch), dim3(256), 0, c->stream, n,
+    // The FP32 transposes run beside the LU's panel chain: 24 KB of LDS they
+    // do not use (57 KB a workgroup) keeps them at two workgroups per CU
+    // instead of four, leaving room for the chain (33.3-33.5 -> 33.1 ms per
+    // sweep at 2,048 panels x 64 frequencies; 48 KB was slower).  The FP64
+    // tile already takes 66 KB.
+    const unsigned pad = sizeof(T) == 4 ? 24576u : 0u;
+    hipLaunchKernelGGL(k_transpose<typename Cx<T>::t>, dim3(tr, tc, (unsigned)batch), dim3(256), pad, c->stream, n,
                        LU, ld, stride, LUt, r_lo, r_hi);
     MH_HIP_CHECK(hipGetLastError());
     return MH_OK;
