@@ -173,6 +173,19 @@ int mh_set_far_field_fp32(mh_context_t ctx, int on);
  * derivatives up to rounding. */
 int mh_set_gradient_mode(mh_context_t ctx, int mode);
 int mh_fallback_count(mh_context_t ctx, int64_t* count);
+/* Deferred refinement check (no reference counterpart; the reference solves
+ * in FP64): a mixed-precision sweep with gradients makes `sweeps` refinement
+ * sweeps (default 2; at most the mh_set_refinement limit) and checks the
+ * last one's residuals on the device,
+ * beside the gradient kernels, reading the verdict at the end of each
+ * frequency chunk; a chunk that missed the 1e-10 tolerance (or whose FP32
+ * factors had a zero pivot or overflowed) is solved again with the check
+ * after every sweep of mh_set_refinement, so the results and errors are
+ * those of the undeferred sweep.  sweeps = 0: no deferral.
+ * mh_deferred_rerun_count: chunks solved again that way since the context
+ * was created. */
+int mh_set_deferred_check(mh_context_t ctx, int sweeps);
+int mh_deferred_rerun_count(mh_context_t ctx, int64_t* count);
 
 /* ---- Green functions ---------------------------------------------------
  * Batched per-pair evaluation of the sum of the selected terms.

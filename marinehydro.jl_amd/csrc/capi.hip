@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+
 #include "mh_internal.hpp"
 
 namespace mh {
@@ -538,9 +540,16 @@ struct IrSystem {
 constexpr double kIrTol = 1e-10;
 // LUt32 (optional): the transposed factors (built during the LU); transposed
 // systems solve from it with coalesced reads.
+// defer_word (not under capture): a fixed c->ir_deferred_sweeps sweeps (at
+// most c->ir_max_sweeps); the
+// last sweep's residuals and their convergence verdict (bits set in
+// *defer_word, which the caller cleared) are formed on the aux stream, after
+// the corrections, and the call returns without waiting for them: the
+// caller's stream goes on with what only reads the solutions, and the
+// caller reads the verdict later.
 static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, int64_t ld, int64_t stride,
                     const float2* LU32, int64_t stride32, const int* perm, const float2* inv32, IrSystem* sys,
-                    int nsys, float2* LUt32, double2* Pa, bool* nonfinite) {
+                    int nsys, float2* LUt32, double2* Pa, bool* nonfinite, int* defer_word = nullptr) {
     *nonfinite = false;
     // a forward and a transposed system form their residuals together, in
     // one pass over A (Pa: workspace of zgemv_pair)
@@ -561,8 +570,25 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
         TRY(fill_zero(c, q.X, sizeof(double2) * (size_t)(q.bstride * batch)));
         TRY(copy_d2(c, q.R, 0, q.Bm, 0, q.bstride * batch, 1));
     }
-    const int max_sweeps = cap ? c->ir_graph_sweeps : c->ir_max_sweeps;
+    const bool defer = !cap && defer_word != nullptr;
+    MH_REQUIRE(!defer || c->ir_deferred_sweeps >= 1, "refinement: deferred check without sweeps");
+    const int max_sweeps =
+        cap ? c->ir_graph_sweeps : (defer ? std::min(c->ir_deferred_sweeps, c->ir_max_sweeps) : c->ir_max_sweeps);
+    bool deferred = false;
     for (int it = 0; it < max_sweeps; ++it) {
+        if (defer && it == max_sweeps - 1) {  // the last correction; its residuals below
+            deferred = true;
+            for (int k = 0; k < nsys; ++k) {
+                OnAux on(c, k == 1);
+                const IrSystem& q = sys[k];
+                const int64_t s32 = n * q.nrhs;
+                TRY(convert_to_f32(c, n, q.nrhs, batch, q.R, q.ldb, q.bstride, q.R32, n, s32));
+                TRY(tri_solve_f32(c, n, batch, LU32, n, stride32, perm, inv32, q.trans, 3, q.nrhs, q.R32, n, s32,
+                                  q.trans ? LUt32 : nullptr));
+                TRY(add_from_f32(c, n, q.nrhs, batch, q.R32, n, s32, q.X, q.ldb, q.bstride));
+            }
+            break;
+        }
         const bool both = pair && !done[0] && !done[1];
         // residual norms: every sweep after the first (a captured sweep only
         // at its last one)
@@ -660,6 +686,33 @@ static int ir_solve(Context* c, int64_t n, int64_t batch, const double2* A64, in
         }
         if (all) return MH_OK;
         if (*nonfinite) break;  // NaN or Inf: more sweeps cannot help
+    }
+    if (deferred) {
+        // aux starts after every correction (those of system 0 on stream)
+        if (nsys > 1) TRY(join_aux(c));
+        TRY(fork_aux(c));
+        OnAux on(c, true);
+        if (pair) {
+            const IrSystem &f = sys[0], &a = sys[1];
+            TRY(zgemv_pair(c, n, batch, A64, ld, stride, (int)f.nrhs, f.X, f.Bm, f.R, f.ldb, f.bstride, (int)a.nrhs,
+                           a.X, a.Bm, a.R, a.ldb, a.bstride, Pa));
+        } else {
+            for (int k = 0; k < nsys; ++k) {
+                const IrSystem& q = sys[k];
+                if (q.trans)
+                    TRY(zgemm_tn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
+                                         q.bstride, q.Bm));
+                else
+                    TRY(zgemm_nn_batched(c, n, q.nrhs, batch, A64, ld, stride, q.X, q.ldb, q.bstride, q.R, q.ldb,
+                                         q.bstride, q.Bm));
+            }
+        }
+        for (int k = 0; k < nsys; ++k) {
+            const IrSystem& q = sys[k];
+            TRY(rel_residual(c, n, q.nrhs, batch, q.R, q.Bm, q.ldb, q.bstride, q.nrm));
+            TRY(ir_verdict(c, q.nrm, batch * q.nrhs, kIrTol, defer_word));
+        }
+        return MH_OK;
     }
     set_error(*nonfinite ? "mixed precision: non-finite residual (NaN or Inf in the system or its solution)"
                          : "mixed precision: iterative refinement did not reach a 1e-10 relative residual");
@@ -789,6 +842,8 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                                      : 0;
     // partial sums of the paired residuals of the forward and adjoint systems
     const size_t o_pa = (mixed && ga && ndof) ? bump.take(sizeof(double2) * (size_t)zgemv_pair_ws_elems(n, chunk)) : 0;
+    // verdict word of a deferred refinement check
+    const size_t o_dv = (mixed && ga && ndof) ? bump.take(sizeof(int)) : 0;
     if (mixed) {
         o_D32 = bump.take(sizeof(float2) * (size_t)n * (size_t)n * chunk);
         o_y = bump.take(sizeof(double2) * (size_t)(n * ncol_ir * chunk));
@@ -879,6 +934,11 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     const int form = direct ? MH_DIRECT : MH_INDIRECT;
     double2* aug = dD + n * ld;  // first right-hand-side column of item 0
     for (int64_t w0 = 0; w0 < nomega; w0 += chunk) {
+        // mixed with gradients: the refinement's convergence check runs beside
+        // the gradient kernels and is read at the end of the chunk
+        bool defer = mixed && !cap && ga && ndof && c->ir_deferred_sweeps > 0;
+        int* dword = o_dv ? reinterpret_cast<int*>(ws + o_dv) : nullptr;
+    chunk_start:
         const int64_t nw = (nomega - w0) < chunk ? (nomega - w0) : chunk;
         TRY(launch_assemble(c, mesh, P, asm_mask, form, nw, d_k + w0, dS, direct ? dD : nullptr,
                             direct ? nullptr : dD, ld, stride,
@@ -911,6 +971,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                 TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, rhs, ldb, bstride, ybuf, ldb, bstride));
             }
             TRY(clear_ints(c, info, nw));
+            if (defer) TRY(clear_ints(c, dword, 1));
             // with an adjoint system, the transposed factors are built during the LU
             float2* Dt32 = (ga && ndof) ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr;
             TRY(lu_factor_f32(c, n, n, nw, D32, n, n * n, piv, info, Dt32, perm, inv32));  // + solve data
@@ -942,7 +1003,8 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
             bool nonfinite = false;
             int rc = ir_solve(c, n, nw, dD, ld, stride, D32, n * n, perm, inv32, sys, nsys,
                               nsys > 1 ? reinterpret_cast<float2*>(ws + o_Dt) : nullptr,
-                              o_pa ? reinterpret_cast<double2*>(ws + o_pa) : nullptr, &nonfinite);
+                              o_pa ? reinterpret_cast<double2*>(ws + o_pa) : nullptr, &nonfinite,
+                              defer ? dword : nullptr);
             if (rc == MH_ERR_NOT_CONVERGED) {
                 const int rc_info = check_info(c, info, nw);  // a more specific cause
                 if (rc_info != MH_OK) rc = rc_info;
@@ -958,7 +1020,9 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
                 c->fallbacks += 1;
             } else {
                 TRY(rc);
-                if (nsys > 1) TRY(join_aux(c));
+                // (deferred: the stream already waited for the corrections on
+                // aux, which now forms the residuals)
+                if (nsys > 1 && !defer) TRY(join_aux(c));
                 if (!direct)
                     TRY(zgemm_nn_batched(c, n, nr, nw, dS, ld, stride, ybuf, ldb, bstride, phi, ldb, bstride));
             }
@@ -1096,6 +1160,26 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
         if (cap) {  // checked after each launch of the graph
             TRY(or_words(c, info, nw, c->cap->check + 1));
             continue;
+        }
+        if (defer) {
+            // the deferred refinement verdict and the FP32 factorisation's
+            // info words: a chunk that missed the tolerance (or whose FP32
+            // factors had a zero pivot or overflowed) is solved again with a
+            // check after every sweep, which refines further, falls back to
+            // FP64 or reports the error as without the deferral
+            TRY(join_aux(c));
+            int hv = 0;
+            std::vector<int> hinfo((size_t)nw);
+            MH_HIP_CHECK(hipMemcpyAsync(&hv, dword, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            MH_HIP_CHECK(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * nw, hipMemcpyDeviceToHost, c->stream));
+            MH_HIP_CHECK(hipStreamSynchronize(c->stream));
+            bool bad = hv != 0;
+            for (int v : hinfo) bad = bad || v != 0;
+            if (bad) {
+                c->deferred_reruns += 1;
+                defer = false;
+                goto chunk_start;
+            }
         }
         TRY(tri_check(c));
         TRY(check_info(c, info, nw));
@@ -1308,6 +1392,20 @@ int mh_set_gradient_mode(mh_context_t ctx, int mode) {
 int mh_set_far_field_fp32(mh_context_t ctx, int on) {
     TRY(check_ctx(ctx));
     CTX(ctx)->far32 = on != 0;
+    return MH_OK;
+}
+
+int mh_set_deferred_check(mh_context_t ctx, int sweeps) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(sweeps >= 0 && sweeps <= 12, "set_deferred_check: sweeps must be in [0, 12]");
+    CTX(ctx)->ir_deferred_sweeps = sweeps;
+    return MH_OK;
+}
+
+int mh_deferred_rerun_count(mh_context_t ctx, int64_t* count) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(count != nullptr, "deferred_rerun_count: null output");
+    *count = CTX(ctx)->deferred_reruns;
     return MH_OK;
 }
 

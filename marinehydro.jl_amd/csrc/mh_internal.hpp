@@ -84,6 +84,13 @@ struct Context {
     struct CapIO* cap = nullptr;
     // refinement sweeps recorded in a captured mixed sweep (at least 2)
     int ir_graph_sweeps = 2;
+    // mixed sweeps with gradients: refinement sweeps made before the
+    // convergence check, which then runs on the aux stream beside the
+    // gradient kernels and is read at the end of the chunk (0: the check
+    // after every sweep); chunks whose deferred check failed and were
+    // solved again with a check after every sweep
+    int ir_deferred_sweeps = 2;
+    int64_t deferred_reruns = 0;
     // multi-body scheduler: worker contexts (each with a stream of its own)
     // and the configuration the autotuner chose (0: not tuned)
     bool owns_stream = false;

@@ -50,6 +50,8 @@ _SIGS = {
     "mh_profile_report": (_int, [_vp, C.c_char_p, _i64]),
     "mh_set_refinement": (_int, [_vp, _int, _int]),
     "mh_fallback_count": (_int, [_vp, C.POINTER(_i64)]),
+    "mh_set_deferred_check": (_int, [_vp, _int]),
+    "mh_deferred_rerun_count": (_int, [_vp, C.POINTER(_i64)]),
     "mh_gf_eval": (_int, [_vp, _int, _int, _int, _i64, _vp, _vp, _d, _vp]),
     "mh_wave_term_eval": (_int, [_vp, _int, _i64, _vp, _vp, _vp]),
     "mh_assemble": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
@@ -208,6 +210,20 @@ def set_gradient_mode(mode="auto", device=0):
     (mh_set_gradient_mode): "auto", "forward" or "reverse"."""
     m = {"auto": 0, "forward": 1, "reverse": 2}[mode]
     check(lib().mh_set_gradient_mode(context(device), m), "set_gradient_mode")
+
+
+def set_deferred_check(sweeps=2, device=0):
+    """Mixed sweeps with gradients: refinement sweeps made before the
+    convergence check, which then runs beside the gradient kernels (0: a
+    check after every sweep) (mh_set_deferred_check)."""
+    check(lib().mh_set_deferred_check(context(device), int(sweeps)), "set_deferred_check")
+
+
+def deferred_rerun_count(device=0) -> int:
+    """Frequency chunks solved again after a failed deferred check."""
+    out = _i64(0)
+    check(lib().mh_deferred_rerun_count(context(device), C.byref(out)), "deferred_rerun_count")
+    return int(out.value)
 
 
 def fallback_count(device=0) -> int:

@@ -1,8 +1,9 @@
 """Mixed precision: the FP64 fallback of a chunk whose refinement does not
 converge (forced from the host by allowing a single refinement sweep; every
 kernel runs normally), two sweeps sufficing on ragged meshes (which pins
-the assembly's FP32 copy of the matrix the FP32 LU factors), and non-finite
-input reported as an error rather than 'converged'."""
+the assembly's FP32 copy of the matrix the FP32 LU factors), the deferred
+convergence check (same results; a chunk that misses the tolerance solved
+again), and non-finite input reported as an error rather than 'converged'."""
 import numpy as np
 import pytest
 
@@ -62,6 +63,57 @@ def test_two_sweeps_suffice_on_ragged_meshes(mh, gpu, shape, direct):
         mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
     for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
         assert relmax(g, r) < 1e-9, name
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_deferred_check_gives_the_checked_results(mh, gpu, direct):
+    # the default deferral makes the same two sweeps as the check after every
+    # sweep (which converges after the second), only its last residual pass
+    # runs beside the gradient kernels: the same results
+    body = mh.hemisphere(1.0, ntheta=20, nphi=22)
+    m = body.mesh
+    om = np.linspace(0.4, 2.8, 9)
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=len(om), omega_param=1)
+    try:
+        mh._lib.set_deferred_check(0)
+        ref = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.5], T, direct=direct, precision="mixed")
+    finally:
+        mh._lib.set_deferred_check(2)
+    n0 = mh._lib.deferred_rerun_count()
+    got = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.5], T, direct=direct, precision="mixed")
+    assert mh._lib.deferred_rerun_count() == n0
+    for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
+        assert relmax(g, r) < 1e-14, name
+
+
+@pytest.mark.parametrize("direct", [True, False])
+def test_deferred_check_reruns_a_chunk_that_misses_the_tolerance(mh, gpu, direct):
+    # one deferred sweep never meets 1e-10: the device verdict fails and the
+    # chunk is solved again with a check after every sweep; with a single
+    # sweep allowed there as well, the FP64 fallback then solves it
+    body = mh.hemisphere(1.0, ntheta=16, nphi=18)
+    m = body.mesh
+    om = [0.5, 1.3, 2.2]
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=len(om), omega_param=1)
+    ref_mixed = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="mixed")
+    ref64 = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="fp64")
+    try:
+        mh._lib.set_deferred_check(1)
+        n0, f0 = mh._lib.deferred_rerun_count(), mh._lib.fallback_count()
+        got = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="mixed")
+        assert mh._lib.deferred_rerun_count() == n0 + 1 and mh._lib.fallback_count() == f0
+        for g, r, name in zip(got, ref_mixed, ("A", "B", "F", "dA", "dB", "dF")):
+            assert relmax(g, r) < 1e-14, name
+        mh._lib.set_refinement(max_sweeps=1, fp64_fallback=True)
+        got = mh.freq_sweep_grad(m, om, dofs, [0.0], T, direct=direct, precision="mixed")
+        assert mh._lib.deferred_rerun_count() == n0 + 2 and mh._lib.fallback_count() == f0 + 1
+        for g, r, name in zip(got, ref64, ("A", "B", "F", "dA", "dB", "dF")):
+            assert relmax(g, r) < 1e-12, name
+    finally:
+        mh._lib.set_deferred_check(2)
+        mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
 
 
 def test_nan_in_the_system_is_an_error_not_converged(mh, gpu):
