@@ -186,6 +186,12 @@ int mh_fallback_count(mh_context_t ctx, int64_t* count);
  * was created. */
 int mh_set_deferred_check(mh_context_t ctx, int sweeps);
 int mh_deferred_rerun_count(mh_context_t ctx, int64_t* count);
+/* Frequencies per chunk of the sweeps at most (no reference counterpart;
+ * the reference solves one frequency at a time): the sweeps assemble,
+ * factor and solve their frequencies in chunks that keep the matrices under
+ * 32 GiB; max_frequencies > 0 caps a chunk further (for callers short of
+ * device memory), 0 restores the default. */
+int mh_set_sweep_chunk(mh_context_t ctx, int64_t max_frequencies);
 
 /* ---- Green functions ---------------------------------------------------
  * Batched per-pair evaluation of the sum of the selected terms.

@@ -813,6 +813,7 @@ int sweep_impl(Context* c, const mh_mesh_t* mesh, int gf_mask, int direct, int p
     const int64_t ld = n, stride = n * ncols, ldb = n, bstride = n * nr;
     const size_t mat_aug = sizeof(double2) * (size_t)n * (size_t)ncols;
     int64_t chunk = (int64_t)((size_t(32) << 30) / (2 * mat_aug));
+    if (c->chunk_limit > 0 && chunk > c->chunk_limit) chunk = c->chunk_limit;
     if (chunk < 1) chunk = 1;
     if (chunk > nomega) chunk = nomega;
     // tangent directions per gradient pass
@@ -1399,6 +1400,13 @@ int mh_set_deferred_check(mh_context_t ctx, int sweeps) {
     TRY(check_ctx(ctx));
     MH_REQUIRE(sweeps >= 0 && sweeps <= 12, "set_deferred_check: sweeps must be in [0, 12]");
     CTX(ctx)->ir_deferred_sweeps = sweeps;
+    return MH_OK;
+}
+
+int mh_set_sweep_chunk(mh_context_t ctx, int64_t max_frequencies) {
+    TRY(check_ctx(ctx));
+    MH_REQUIRE(max_frequencies >= 0, "set_sweep_chunk: max_frequencies must be >= 0");
+    CTX(ctx)->chunk_limit = max_frequencies;
     return MH_OK;
 }
 

@@ -91,6 +91,8 @@ struct Context {
     // solved again with a check after every sweep
     int ir_deferred_sweeps = 2;
     int64_t deferred_reruns = 0;
+    // sweeps: frequencies per chunk at most (0: as many as fit 32 GiB)
+    int64_t chunk_limit = 0;
     // multi-body scheduler: worker contexts (each with a stream of its own)
     // and the configuration the autotuner chose (0: not tuned)
     bool owns_stream = false;

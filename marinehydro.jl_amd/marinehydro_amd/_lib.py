@@ -52,6 +52,7 @@ _SIGS = {
     "mh_fallback_count": (_int, [_vp, C.POINTER(_i64)]),
     "mh_set_deferred_check": (_int, [_vp, _int]),
     "mh_deferred_rerun_count": (_int, [_vp, C.POINTER(_i64)]),
+    "mh_set_sweep_chunk": (_int, [_vp, _i64]),
     "mh_gf_eval": (_int, [_vp, _int, _int, _int, _i64, _vp, _vp, _d, _vp]),
     "mh_wave_term_eval": (_int, [_vp, _int, _i64, _vp, _vp, _vp]),
     "mh_assemble": (_int, [_vp, _vp, _int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _i64]),
@@ -217,6 +218,12 @@ def set_deferred_check(sweeps=2, device=0):
     convergence check, which then runs beside the gradient kernels (0: a
     check after every sweep) (mh_set_deferred_check)."""
     check(lib().mh_set_deferred_check(context(device), int(sweeps)), "set_deferred_check")
+
+
+def set_sweep_chunk(max_frequencies=0, device=0):
+    """Frequencies per chunk of the sweeps at most (0: as many as fit 32 GiB)
+    (mh_set_sweep_chunk)."""
+    check(lib().mh_set_sweep_chunk(context(device), int(max_frequencies)), "set_sweep_chunk")
 
 
 def deferred_rerun_count(device=0) -> int:

@@ -116,6 +116,36 @@ def test_deferred_check_reruns_a_chunk_that_misses_the_tolerance(mh, gpu, direct
         mh._lib.set_refinement(max_sweeps=12, fp64_fallback=True)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_sweeps_in_several_frequency_chunks(mh, gpu, precision):
+    # 8 frequencies in chunks of at most 3 (3 + 3 + 2): each chunk its own
+    # assembly, factorisation, solves and gradients, at its offset in the
+    # frequency list; the results are those of the single chunk.  Mixed:
+    # with one deferred sweep every chunk's verdict fails and the chunk is
+    # solved again with the check after every sweep.
+    body = mh.hemisphere(1.0, ntheta=14, nphi=16)
+    m = body.mesh
+    om = np.linspace(0.3, 2.9, 8)
+    dofs = [[0, 0, 1.0], [1.0, 0, 0]]
+    T = mh.shape_tangents(m, [body.dvertices["radius"], None], nomega=len(om), omega_param=1)
+    ref = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.9], T, precision=precision)
+    try:
+        mh._lib.set_sweep_chunk(3)
+        got = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.9], T, precision=precision)
+        for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
+            assert relmax(g, r) < 1e-13, name
+        if precision == "mixed":
+            mh._lib.set_deferred_check(1)
+            n0 = mh._lib.deferred_rerun_count()
+            got = mh.freq_sweep_grad(m, om, dofs, [0.0, 0.9], T, precision=precision)
+            assert mh._lib.deferred_rerun_count() == n0 + 3
+            for g, r, name in zip(got, ref, ("A", "B", "F", "dA", "dB", "dF")):
+                assert relmax(g, r) < 1e-13, name
+    finally:
+        mh._lib.set_sweep_chunk(0)
+        mh._lib.set_deferred_check(2)
+
+
 def test_nan_in_the_system_is_an_error_not_converged(mh, gpu):
     m = mh.hemisphere(1.0, ntheta=10, nphi=10).mesh
     dof = np.tile([0.0, 0.0, 1.0], (m.nfaces, 1))
