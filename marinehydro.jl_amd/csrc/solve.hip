@@ -440,6 +440,34 @@ __global__ __launch_bounds__(256) void k_lu_swap(C2* __restrict__ A, int64_t ld,
     }
 }
 
+// Split operand images of the mixed LU's trailing updates (layout with
+// k_split below): 16-byte units of 8 halves, hi/lo planes, and the
+// per-row or per-column scaling exponent.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+constexpr int kHK = 16;                      // complex depths per chunk
+constexpr int kHChunks = kNBOuter / kHK;     // chunks of a full block column
+constexpr int kHUnits = kHChunks * 2 * 4;    // image units per row or column
+constexpr int kHEMax = 14;                   // largest |x| of a row or column scaled into [2^13, 2^14)
+
+__device__ __forceinline__ void split_h(float xs, _Float16& hi, _Float16& lo) {
+    hi = (_Float16)xs;
+    lo = (_Float16)(xs - (float)hi);  // exact in FP32
+}
+
+// exponent scaling a row's or column's largest magnitude mx into [2^13, 2^14)
+__device__ __forceinline__ int split_exp(float mx) {
+    if (!(mx > 0.0f)) return 0;
+    int E;
+    (void)frexpf(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+    return kHEMax - E;
+}
+
+__device__ __forceinline__ bool finite2(float2 v) {
+    return ::fabsf(v.x) <= 3.402823466e38f && ::fabsf(v.y) <= 3.402823466e38f;  // false for NaN
+}
+
 // ---------------------------------------------------------------------------
 // X = L11^{-1} X in one launch for a whole block column: L11 = unit lower
 // block at (k0, k0) of size w <= kNBOuter, X = rows [k0, k0+w) of columns
@@ -518,9 +546,18 @@ __device__ __forceinline__ void tb_stage(T (&xr)[kTbSlots], T (&xi)[kTbSlots], t
     if (S + kTbStage < w) tb_load<T, S + kTbStage>(tx, ty, M, ld, k0, w, tid);  // next block, during this one
     TbStage<T, S, kTbStage>::run(xr, xi, sL, g, w);
 }
-template <class T>
+//
+// IMG (FP32 only): the solved columns are the U12 operand of a split-image
+// trailing update; the epilogue also writes their image units and column
+// exponents (the values k_split<false> would write from them) from X staged
+// in LDS, so the update needs no separate split pass over U12.
+template <class T, bool IMG>
 __global__ __launch_bounds__(256) void k_lu_trsm_blk(typename Cx<T>::t* __restrict__ A, int64_t ld, int64_t stride,
-                                                     int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+                                                     int64_t k0, int w, int64_t c_lo, int64_t c_hi,
+                                                     u4v* __restrict__ img, int64_t img_stride, int64_t pad,
+                                                     int* __restrict__ xexp, int64_t xexp_stride,
+                                                     int* __restrict__ info) {
+    static_assert(!IMG || sizeof(T) == 4, "split images are FP32 only");
     // panel-chain kernel: its waves win issue arbitration against those of
     // a trailing-update GEMM sharing the CU (the chain is the LU's critical
     // path)
@@ -553,6 +590,62 @@ __global__ __launch_bounds__(256) void k_lu_trsm_blk(typename Cx<T>::t* __restri
         for (int i = 0; i < kTbSlots; ++i) {
             const int r = g + 16 * i;
             if (r < w) col[r] = mkc<C2>(xr[i], xi[i]);
+        }
+    }
+    if constexpr (IMG) {
+        // the column's largest |re| or |im| and non-finite flag over its 16
+        // lanes (rows from w on hold zeros)
+        float mx = 0.0f, bad = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kTbSlots; ++i) {
+            if (!finite2(make_float2(xr[i], xi[i]))) bad = 1.0f;
+            mx = fmaxf(mx, fmaxf(::fabsf(xr[i]), ::fabsf(xi[i])));
+        }
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, o));
+            bad = fmaxf(bad, __shfl_xor(bad, o));
+        }
+        __shared__ int sE[kTbCols];
+        __syncthreads();  // the last stage's reads of sL are done: X goes there, sL[column][row]
+        const int jl = tid >> 4;
+#pragma unroll
+        for (int i = 0; i < kTbSlots; ++i) sL[jl][g + 16 * i] = mkc<C2>(xr[i], xi[i]);
+        if (g == 0) {
+            int e = split_exp(mx);
+            if (bad != 0.0f) {
+                if (ok) atomicOr(info + blockIdx.y, 2);
+                e = 0;
+            }
+            sE[jl] = e;
+            if (ok) xexp[(int64_t)blockIdx.y * xexp_stride + j] = e;
+        }
+        __syncthreads();
+        // as k_split<false>: lane (column xl, unit q) writes the 8 depths of
+        // unit q of chunks c and c + 4 (wave c), hi and lo planes
+        const int xl = (tid >> 2) & 15, q = tid & 3;
+        const int64_t x = c_lo + (int64_t)blockIdx.x * kTbCols + xl;
+        if (x < c_hi) {
+            const int e = sE[xl];
+            u4v* out = img + (int64_t)blockIdx.y * img_stride;
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                const int c = (tid >> 6) + 4 * h2;
+                if (kHK * c < w) {
+                    const int d0 = kHK * c + 8 * (q & 1);
+                    h8 hi, lo;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const C2 z = sL[xl][d0 + t];
+                        _Float16 a0, a1;
+                        split_h(__builtin_amdgcn_ldexpf((q & 2) ? z.y : z.x, e), a0, a1);
+                        hi[t] = a0;
+                        lo[t] = a1;
+                    }
+                    out[((int64_t)(2 * c) * pad + x) * 4 + q] = __builtin_bit_cast(u4v, hi);
+                    out[((int64_t)(2 * c + 1) * pad + x) * 4 + q] = __builtin_bit_cast(u4v, lo);
+                }
+            }
         }
     }
 }
@@ -820,31 +913,6 @@ __global__ __launch_bounds__(512) void k_lu_gemm_big(typename Cx<T>::t* __restri
 // [n = l%16], result register r is row 4 (l/16) + r.  As in k_lu_gemm_big
 // the tile's columns are the instruction's rows, so the results run along
 // columns of C.
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-typedef unsigned int v2u __attribute__((ext_vector_type(2)));
-constexpr int kHK = 16;                      // complex depths per chunk
-constexpr int kHChunks = kNBOuter / kHK;     // chunks of a full block column
-constexpr int kHUnits = kHChunks * 2 * 4;    // image units per row or column
-constexpr int kHEMax = 14;                   // largest |x| of a row or column scaled into [2^13, 2^14)
-
-__device__ __forceinline__ void split_h(float xs, _Float16& hi, _Float16& lo) {
-    hi = (_Float16)xs;
-    lo = (_Float16)(xs - (float)hi);  // exact in FP32
-}
-
-// exponent scaling a row's or column's largest magnitude mx into [2^13, 2^14)
-__device__ __forceinline__ int split_exp(float mx) {
-    if (!(mx > 0.0f)) return 0;
-    int E;
-    (void)frexpf(mx, &E);  // mx = m 2^E, m in [0.5, 1)
-    return kHEMax - E;
-}
-
-__device__ __forceinline__ bool finite2(float2 v) {
-    return ::fabsf(v.x) <= 3.402823466e38f && ::fabsf(v.y) <= 3.402823466e38f;  // false for NaN
-}
-
 // Operand images: 16 rows of L21 (ROWS) or columns of U12 per 512-thread
 // workgroup; wave c takes depth chunk c, lane (row or column x, unit q) the
 // 8 depths of image unit q (re or im), so each store of a wave writes 1 KB
@@ -1325,15 +1393,24 @@ static int lu_swap(const LuPlan<T>& P, int64_t k0, int npiv, int64_t c_lo, int64
     return MH_OK;
 }
 
+// Whether the FP32 trailing update of rows [r_lo, r_hi) x columns [c_lo,
+// c_hi) by kw depths runs on the split images: the split-image kernel takes
+// tiles that start on multiples of kBT (the images and exponents end at the
+// padded size) and full-depth blocks.
+template <class T>
+static bool lu_split_update(int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi, int kw) {
+    return sizeof(T) == 4 && r_hi - r_lo >= kBT && c_hi - c_lo >= kBT && r_lo % kBT == 0 && c_lo % kBT == 0 &&
+           kw == kNBOuter;
+}
+
+// u_ready: the TRSM that solved U12 has written its image (k_lu_trsm_blk<.., true>)
 template <class T>
 static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo, int64_t c_hi,
-                   int64_t k_lo, int kw) {
+                   int64_t k_lo, int kw, bool u_ready = false) {
     if (r_hi <= r_lo || c_hi <= c_lo || kw <= 0) return MH_OK;
     const int64_t mr = r_hi - r_lo, mc = c_hi - c_lo;
-    // the split-image kernel takes tiles that start on multiples of kBT
-    // (the images and exponents end at the padded size) and full-depth blocks
-    const bool big = mr >= kBT && mc >= kBT &&
-                     (sizeof(T) == 8 || (r_lo % kBT == 0 && c_lo % kBT == 0 && kw == kNBOuter));
+    const bool big = sizeof(T) == 8 ? (mr >= kBT && mc >= kBT) : lu_split_update<T>(r_lo, r_hi, c_lo, c_hi, kw);
+    MH_REQUIRE(!u_ready || (big && sizeof(T) == 4), "lu: U12 image written for an update that does not read it");
     const char* name = big ? (sizeof(T) == 8 ? "lu_gemm_big" : "lu_gemm_f16x3") : (sizeof(T) == 8 ? "lu_gemm" : "lu_gemm_f32");
     if (big) {
         const dim3 grid((unsigned)((mr + kBT - 1) / kBT), (unsigned)((mc + kBT - 1) / kBT), (unsigned)P.batch);
@@ -1344,7 +1421,7 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
             MH_REQUIRE(H->r0[sl] >= 0 && r_lo >= H->r0[sl] && k_lo == H->k_lo[sl] && kw == H->kw[sl] &&
                            kw == kNBOuter && r_hi <= H->npad && c_hi <= H->ncpad,
                        "lu: trailing update does not match the split L21 image");
-            {  // U12 image of these columns
+            if (!u_ready) {  // U12 image of these columns
                 ProfScope pss(P.ctx, "lu_split", 0.0, (8.0 * 2 + 8.0) * (double)mc * kw * P.batch);
                 hipLaunchKernelGGL(k_split<false>, dim3((unsigned)((mc + 15) / 16), (unsigned)P.batch),
                                    dim3(512), 0, P.ctx->stream, P.A, P.ld, P.stride, P.n, c_lo, c_hi, H->ncpad,
@@ -1384,14 +1461,28 @@ static int lu_gemm(const LuPlan<T>& P, int64_t r_lo, int64_t r_hi, int64_t c_lo,
 }
 
 // X = L[k0:k0+w, k0:k0+w]^{-1} X on columns [c_lo, c_hi), one launch
+// img (FP32): also write the U12 image of the solved columns for the split-image update
 template <class T>
-static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi) {
+static int lu_trsm(const LuPlan<T>& P, int64_t k0, int w, int64_t c_lo, int64_t c_hi, bool img = false) {
     if (c_hi <= c_lo) return MH_OK;
     const int64_t nc = c_hi - c_lo;
     MH_REQUIRE(w <= kNBOuter, "lu: TRSM block wider than a block column");
+    const dim3 grid((unsigned)((nc + kTbCols - 1) / kTbCols), (unsigned)P.batch);
+    if constexpr (sizeof(T) == 4) {
+        if (img) {
+            HImg* H = P.img;
+            MH_REQUIRE(H != nullptr && w == kNBOuter && c_hi <= H->ncpad, "lu: U12 image does not fit the TRSM");
+            ProfScope ps(P.ctx, "lu_trsm", 4.0 * w * w * (double)nc * P.batch,
+                         (4.0 * sizeof(T) + 8.0) * w * (double)nc * P.batch);
+            hipLaunchKernelGGL((k_lu_trsm_blk<T, true>), grid, dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, k0, w,
+                               c_lo, c_hi, H->U, (int64_t)kHUnits * H->ncpad, H->ncpad, H->cexp, H->ncpad, P.info);
+            return MH_OK;
+        }
+    }
+    MH_REQUIRE(!img, "lu: split images are FP32 only");
     ProfScope ps(P.ctx, "lu_trsm", 4.0 * w * w * (double)nc * P.batch, 4.0 * sizeof(T) * w * (double)nc * P.batch);
-    hipLaunchKernelGGL(k_lu_trsm_blk<T>, dim3((unsigned)((nc + kTbCols - 1) / kTbCols), (unsigned)P.batch), dim3(256),
-                       0, P.ctx->stream, P.A, P.ld, P.stride, k0, w, c_lo, c_hi);
+    hipLaunchKernelGGL((k_lu_trsm_blk<T, false>), grid, dim3(256), 0, P.ctx->stream, P.A, P.ld, P.stride, k0, w, c_lo,
+                       c_hi, nullptr, (int64_t)0, (int64_t)0, nullptr, (int64_t)0, nullptr);
     return MH_OK;
 }
 
@@ -1534,8 +1625,9 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         P.wide = true;
         rc = lu_swap(P, k0, w, 0, k0);
         if (rc == MH_OK) rc = lu_swap(P, k0, w, cw0, ncols);
-        if (rc == MH_OK) rc = lu_trsm(P, k0, w, cw0, ncols);
-        if (rc == MH_OK) rc = lu_gemm(P, kn0, n, cw0, ncols, k0, w);
+        const bool uw = lu_split_update<T>(kn0, n, cw0, ncols, w);  // the TRSM writes U12's image
+        if (rc == MH_OK) rc = lu_trsm(P, k0, w, cw0, ncols, uw);
+        if (rc == MH_OK) rc = lu_gemm(P, kn0, n, cw0, ncols, k0, w, uw);
         P.wide = false;
         ctx->stream = sA;
         if (rc != MH_OK) break;
@@ -1544,9 +1636,10 @@ static int lu_factor_t(Context* ctx, int64_t n, int64_t ncols, int64_t batch, ty
         if (wn > 0) {
             if (k >= 1) MH_HIP_CHECK(hipStreamWaitEvent(sA, evB[k - 1], 0));
             rc = lu_swap(P, k0, w, kn0, cw0);
-            if (rc == MH_OK) rc = lu_trsm(P, k0, w, kn0, cw0);
+            const bool un = lu_split_update<T>(kn0, n, kn0, cw0, w);
+            if (rc == MH_OK) rc = lu_trsm(P, k0, w, kn0, cw0, un);
             P.chain = true;
-            if (rc == MH_OK) rc = lu_gemm(P, kn0, n, kn0, cw0, k0, w);
+            if (rc == MH_OK) rc = lu_gemm(P, kn0, n, kn0, cw0, k0, w, un);
             P.chain = false;
         }
         if (LUt && rc == MH_OK) {  // block row k is final: transpose it on the side stream
